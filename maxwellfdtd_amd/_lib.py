@@ -1,0 +1,137 @@
+"""ctypes binding of the C ABI in include/maxwell_fdtd.h.
+
+`Binding(path, prefix)` wraps any shared library that exports the ABI's functions
+under `prefix` ("mfd_" for the engine).  `engine()` loads the in-tree HIP library and
+raises if it is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ENGINE_LIB = os.path.join(HERE, "libmaxwellfdtd_amd.so")
+
+MFD_OK, MFD_ERR_NAN, MFD_ERR_INF = 0, -5, -6
+NAN_MSG = "Something went wrong. I found NaN field values."
+INF_MSG = "Something went wrong. I found Inf field values."
+
+
+class mfd_model_desc(C.Structure):
+    _fields_ = [("ndims", C.c_int32), ("Nx", C.c_int32), ("Ny", C.c_int32), ("Nz", C.c_int32),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("dt", C.c_double), ("Nt", C.c_int32), ("bcs", C.c_int32 * 6),
+                ("Mh", C.c_double), ("Me", C.c_double), ("Md1", C.c_double),
+                ("Md2", C.c_double)]
+
+
+class mfd_material_desc(C.Structure):
+    _fields_ = [("Mh", C.c_double), ("Me", C.c_double), ("Md1", C.c_double),
+                ("Md2", C.c_double), ("nq", C.c_int32),
+                ("Aq", C.POINTER(C.c_double)), ("Bq", C.POINTER(C.c_double)),
+                ("Cq", C.POINTER(C.c_double)), ("iskerr", C.c_int32),
+                ("Mk2", C.c_double), ("Mk3", C.c_double), ("isplasma", C.c_int32),
+                ("Rava", C.c_double), ("ksi", C.c_double), ("rho0", C.c_double),
+                ("Ap", C.c_double), ("Bp", C.c_double), ("Cp", C.c_double),
+                ("Ma", C.c_double), ("rate_kind", C.c_int32), ("rate_c", C.c_double),
+                ("rate_k", C.c_double), ("rate_n", C.c_int32),
+                ("rate_x", C.POINTER(C.c_double)), ("rate_y", C.POINTER(C.c_double))]
+
+
+class mfd_tfsf_desc(C.Structure):
+    _fields_ = [("box", C.c_int32 * 6), ("mode", C.c_int32), ("ntables", C.c_int32),
+                ("tables", C.POINTER(C.POINTER(C.c_double))), ("aux_n", C.c_int32),
+                ("aux_comp", C.POINTER(C.c_int32)), ("aux_iz", C.POINTER(C.c_int32)),
+                ("aux_coeff", C.POINTER(C.c_double)), ("aux_wave", C.POINTER(C.c_double))]
+
+
+P = C.c_void_p
+I32 = C.c_int32
+DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int32)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "version": (I32, []),
+    "last_error": (C.c_char_p, []),
+    "model_create": (I32, [C.POINTER(mfd_model_desc), C.POINTER(P)]),
+    "model_set_geometry": (I32, [P, IP]),
+    "model_add_material": (I32, [P, C.POINTER(mfd_material_desc)]),
+    "model_set_pml": (I32, [P, I32, I32, I32, DP, DP, DP]),
+    "model_add_source": (I32, [P, I32, I32, C.c_double, I32, IP, DP]),
+    "model_add_tfsf": (I32, [P, C.POINTER(mfd_tfsf_desc)]),
+    "model_add_field_monitor": (I32, [P, I32, IP, I32]),
+    "model_add_spectral_monitor": (I32, [P, I32, IP, I32, DP, C.c_double, I32]),
+    "model_enable_integrals": (I32, [P, I32]),
+    "model_footprint": (I32, [P, C.POINTER(C.c_int64)]),
+    "model_upload": (I32, [P]),
+    "model_step": (I32, [P, I32]),
+    "model_get_it": (I32, [P, IP]),
+    "model_get_field": (I32, [P, I32, DP]),
+    "model_set_field": (I32, [P, I32, DP]),
+    "model_get_monitor": (I32, [P, I32, I32, DP]),
+    "model_get_output": (I32, [P, I32, DP]),
+    "model_destroy": (I32, [P]),
+}
+
+
+def dptr(a):
+    return a.ctypes.data_as(DP)
+
+
+def iptr(a):
+    return a.ctypes.data_as(IP)
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+class Binding:
+    """Callable wrapper of one library implementing the ABI under `prefix`."""
+
+    def __init__(self, path, prefix, extra=None):
+        self.path = path
+        self.prefix = prefix
+        self.lib = C.CDLL(path)
+        sigs = dict(SIGNATURES)
+        sigs.update(extra or {})
+        for name, (res, args) in sigs.items():
+            fn = getattr(self.lib, prefix + name)
+            fn.restype = res
+            fn.argtypes = args
+            setattr(self, name, fn)
+
+    def error(self):
+        msg = self.last_error()
+        return msg.decode() if msg else ""
+
+    def check(self, status, what=""):
+        if status == MFD_OK:
+            return
+        if status == MFD_ERR_NAN:
+            raise EngineError(NAN_MSG)
+        if status == MFD_ERR_INF:
+            raise EngineError(INF_MSG)
+        raise EngineError(f"{self.prefix}{what} failed ({status}): {self.error()}")
+
+
+ENGINE_EXTRA = {
+    "device_info": (I32, [C.POINTER(C.c_int64), C.POINTER(C.c_int64), IP, C.c_char_p, I32]),
+    "model_set_options": (I32, [P, I32, I32]),
+    "model_timing": (I32, [P, I32, DP, IP]),
+}
+
+_engine = None
+
+
+def engine():
+    """The HIP engine library; raises if it has not been built."""
+    global _engine
+    if _engine is None:
+        if not os.path.exists(ENGINE_LIB):
+            raise EngineError(
+                f"{ENGINE_LIB} is missing: build the HIP engine first "
+                "(python -c 'import __graft_entry__ as g; g.build()')")
+        _engine = Binding(ENGINE_LIB, "mfd_", ENGINE_EXTRA)
+    return _engine

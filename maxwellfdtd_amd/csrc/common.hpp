@@ -1,0 +1,139 @@
+// Device-side data layout of the MI355X MaxwellFDTD engine.
+//
+// One DevModel describes one solve resident in HBM.  All kernels take an array of
+// DevModel plus a packed tile work-list, so a single solve and a batch of uneven solves
+// run through the same launches.  Field arrays are SoA FP32, x fastest (the Julia
+// column-major order of Field1D/2D/3D, src/fields.jl:7-90); 1D uses Nx = Ny = 1 and
+// 2D uses Ny = 1.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mfd {
+
+enum { HX = 0, HY, HZ, DX, DY, DZ, EX, EY, EZ };
+enum { BC_PERIODIC = 1, BC_DIRICHLET = 2, BC_NEUMANN = 3 };
+enum { MAT_DISP = 1, MAT_KERR = 2, MAT_PLASMA = 4 };
+enum { SRC_SOFT = 0, SRC_HARD = 1, SRC_TFSF = 2 };
+
+// One CPML face layer: profiles from LeftPMLLayer/RightPMLLayer (src/pml.jl:57-127)
+// and the psi arrays of that face (src/pml.jl:221-304), FP32.
+struct DevLayer {
+    int present;
+    int ib;        // 0-based boundary index: left layer covers i <= ib, right i >= ib
+    int nb;
+    int pad;
+    const float *K, *A, *B;
+    float *psiE0, *psiE1;   // H half-step psi (x: Eyx,Ezx  y: Exy,Ezy  z: Exz,Eyz)
+    float *psiH0, *psiH1;   // E half-step psi (x: Hyx,Hzx  y: Hxy,Hzy  z: Hxz,Hyz)
+};
+
+// Material coefficient table entry (index 0 = vacuum) with the material's compact
+// state: only the cells of the material hold ADE / plasma state, addressed by the
+// per-cell slot index (the reference keeps full-grid arrays, src/materials.jl:424-474).
+struct DevMat {
+    float Mh, Me, Md1, Md2;     // src/materials.jl:393-400 (vacuum: src/models.jl:111-114)
+    int flags;                  // MAT_DISP | MAT_KERR | MAT_PLASMA
+    int nq;                     // ADE poles
+    float K2, K3;               // Kerr Pade: Mk2*Me^2, Mk3*Me^3 (src/models.jl:1140-1149)
+    const float *Aq, *Bq, *Cq;  // ade_coefficients (src/materials.jl:509-577)
+    float2 *P;                  // [nq][nc][cnt]: (P^n, P^{n-1})
+    int cnt;                    // state slots (cells of this material)
+    int rate_kind, rate_n, pad;
+    double Rava, ksi, Cprho0, Marho0, rate_c, rate_k, rho0;
+    float Ap, Bp;
+    const double *rate_x, *rate_y;
+    double2 *rhod;              // [cnt]: (rho, drho), FP64
+    float2 *Pp;                 // [nc][cnt]: (Pp^n, Pp^{n-1})
+    float *Pa;                  // [nc][cnt]
+    float *JE;                  // [cnt]
+};
+
+// Soft/hard point source or TFSF source of one model.
+struct DevSource {
+    int kind;                   // SRC_SOFT / SRC_HARD / SRC_TFSF
+    int comp;
+    int n;
+    float coeff;
+    const int *cells;           // 0-based linear cell indices [n]
+    const float *wave;          // [Nt][n]
+    // TFSF: 3D face scheme (1D/2D map onto faces 4,5 / 0,1,4,5, see engine.cpp)
+    int box[6];                 // 0-based X1, X2, Y1, Y2, Z1, Z2
+    int mode;                   // 0 tables, 1 aux 1D
+    int aux_n;
+    const float *tab[24];       // replay tables per face: H_u, H_v, E_u, E_v
+    float cm[3], ce[3], cd[3];  // Cm/d, Ce/d, EPS0*Ce/d per axis (src/sources.jl:725)
+    // auxiliary 1D run (FP64): fields [6][Nz] = Hx, Hy, Dx, Dy, Ex, Ey
+    double *aux;
+    double *auxpsi;             // [2 faces][4][nbmax]: dEx/dz, dEy/dz, dHx/dz, dHy/dz
+    int aux_nbmax;
+    int aux_ib[2], aux_nb[2], aux_present[2];
+    const double *auxK[2], *auxA[2], *auxB[2];
+    const int *aux_comp, *aux_iz;
+    const double *aux_coeff, *aux_wave;   // [aux_n][Nt]
+    double Mh0, Me0, Md10, Md20, idz;
+};
+
+// Field / spectral monitor of one model.
+struct DevMon {
+    int spectral, issum, n, nw;
+    int ncomp;
+    int comps[6];
+    const int *cells;
+    float *rec;                 // field monitor, per-point record [Nt][ncomp][n]
+    double *sum;                // field monitor with sum=true [Nt][ncomp]
+    float *samp;                // spectral: time block of samples [K][ncomp][n]
+    double2 *S;                 // spectral accumulators [ncomp][nw][n] (or [ncomp][nw])
+    const double *w;            // [nw]
+    double tshift;
+};
+
+struct DevModel {
+    int ndims, Nx, Ny, Nz;
+    long long N, sy, sz;        // cells, strides in y and z
+    float idx, idy, idz, dtf;   // 1/dx, 1/dy, 1/dz, dt (FP32)
+    double dt;
+    int bcs[6];
+    int Nt;
+    int int_flags;              // 1 Sa, 2 E2
+    float *F[9];
+    const unsigned char *id;    // material id per cell, nullptr if no material
+    const int *slot;            // state slot per cell, nullptr if no stateful material
+    const DevMat *mats;         // [nmat + 1]
+    DevLayer L[6];
+    float *Sa, *E2;
+    int *it;                    // device step counter: completed steps
+    int *flag;                  // non-finite Ex: 1 NaN, 2 Inf
+    int nsrc, nmon;
+    const DevSource *src;
+    const DevMon *mon;
+    int dft_k;                  // samples buffered in the current DFT time block
+    int dft_block;              // DFT time-block length
+};
+
+// Tile of the work-list: model index, x0, y0, (z0 | z1 << 16).
+typedef int4 Tile;
+
+__host__ __device__ inline int tile_z0(const Tile &t) { return t.w & 0xffff; }
+__host__ __device__ inline int tile_z1(const Tile &t) { return (t.w >> 16) & 0xffff; }
+
+// tile geometry per dimensionality
+constexpr int TX3 = 64, TY3 = 4;    // 3D: 64 x 4 columns, marching z
+constexpr int TX2 = 256;            // 2D: 256 x-columns, marching z
+constexpr int TX1 = 256;            // 1D: 256 cells
+
+// launchers (defined in the .hip files)
+void launch_h3d(const DevModel *m, const Tile *tiles, int ntiles, int grid, bool mats, hipStream_t s);
+void launch_e3d(const DevModel *m, const Tile *tiles, int ntiles, int grid, bool mats, hipStream_t s);
+void launch_h2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, bool mats, hipStream_t s);
+void launch_e2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, bool mats, hipStream_t s);
+void launch_h1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, bool mats, hipStream_t s);
+void launch_e1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, bool mats, hipStream_t s);
+void launch_point_sources(const DevModel *m, int nmodels, int slot, int maxn, hipStream_t s);
+void launch_tfsf(const DevModel *m, int nmodels, int slot, int ndims, int maxface, hipStream_t s);
+void launch_monitors(const DevModel *m, int nmodels, int maxn, hipStream_t s);
+void launch_dft_flush(const DevModel *m, int nmodels, int maxn, int nk, hipStream_t s);
+void launch_integrals(const DevModel *m, int nmodels, int grid, int ndims, hipStream_t s);
+void launch_finish(const DevModel *m, int nmodels, hipStream_t s);
+
+}  // namespace mfd

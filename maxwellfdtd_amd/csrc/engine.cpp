@@ -1,0 +1,1069 @@
+// Host engine: model upload to HBM, the per-step launch sequence, hipGraph replay and
+// the C ABI of include/maxwell_fdtd.h.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+namespace mfd {
+
+static thread_local std::string g_error;
+void set_error(const std::string &msg) { g_error = msg; }
+
+Arena::~Arena() { release(); }
+void Arena::release() {
+    for (void *p : ptrs) hipFree(p);
+    ptrs.clear();
+    bytes = 0;
+}
+
+bool Model::comp_present(int c) const {
+    if (d.ndims == 3) return true;
+    if (d.ndims == 2) return c == HY || c == DX || c == DZ || c == EX || c == EZ;
+    return c == HY || c == DX || c == EX;
+}
+
+static int monitor_comps(int ndims, int *out) {
+    static const int c3[6] = {HX, HY, HZ, EX, EY, EZ};
+    static const int c2[3] = {HY, EX, EZ};
+    static const int c1[2] = {HY, EX};
+    const int *c = ndims == 3 ? c3 : ndims == 2 ? c2 : c1;
+    const int n = ndims == 3 ? 6 : ndims == 2 ? 3 : 2;
+    for (int i = 0; i < n; i++) out[i] = c[i];
+    return n;
+}
+
+static int psi_len(const Model &m, int face, int nb) {
+    const int Nx = m.d.Nx, Ny = m.d.Ny, Nz = m.d.Nz;
+    if (face < 2) return nb * Ny * Nz;
+    if (face < 4) return Nx * nb * Nz;
+    return Nx * Ny * nb;
+}
+
+int64_t Model::footprint() const {
+    int64_t b = 0;
+    int nf = 0;
+    for (int c = 0; c < 9; c++) nf += comp_present(c);
+    b += (int64_t)nf * N * 4;
+    b += N;
+    bool stateful = false;
+    std::vector<long long> cnt(mats.size() + 1, 0);
+    if (has_geom)
+        for (long long i = 0; i < N; i++) cnt[geom[i]]++;
+    for (size_t k = 0; k < mats.size(); k++) {
+        const auto &md = mats[k].d;
+        const long long c = cnt[k + 1];
+        if (md.nq > 0 || md.isplasma) stateful = true;
+        b += (int64_t)md.nq * ncomp() * c * 8;
+        if (md.isplasma) b += c * (16 + ncomp() * 12 + 4);
+    }
+    if (stateful) b += N * 4;
+    for (int f = 0; f < 6; f++)
+        if (L[f].present) b += (int64_t)4 * psi_len(*this, f, L[f].nb) * 4 + 3 * L[f].nb * 4;
+    for (const auto &s : srcs) {
+        if (s.type == 0) b += (int64_t)s.cells.size() * (4 + 4LL * d.Nt);
+        else {
+            for (const auto &t : s.tables) b += t.size() * 4;
+            b += 6LL * d.Nz * 8 + (int64_t)s.aux_wave.size() * 8;
+        }
+    }
+    int cm[6];
+    const int nc = monitor_comps(d.ndims, cm);
+    for (const auto &mo : mons) {
+        const int64_t n = mo.cells.size();
+        if (!mo.spectral) b += mo.issum ? (int64_t)d.Nt * nc * 8 : (int64_t)d.Nt * nc * n * 4;
+        else b += (int64_t)dft_block * nc * n * 4 + (int64_t)nc * mo.w.size() * (mo.issum ? 1 : n) * 16;
+    }
+    if (int_flags & MFD_INT_SA) b += N * 4;
+    if (int_flags & MFD_INT_E2) b += N * 4;
+    return b;
+}
+
+template <class T>
+static hipError_t upload_vec(Arena &a, T **dst, const std::vector<T> &v) {
+    hipError_t e = a.alloc(dst, v.size());
+    if (e != hipSuccess) return e;
+    if (!v.empty()) e = hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    return e;
+}
+
+static std::vector<float> to_f32(const std::vector<double> &v) {
+    std::vector<float> r(v.size());
+    for (size_t i = 0; i < v.size(); i++) r[i] = (float)v[i];
+    return r;
+}
+
+#define UP(call)                                                                       \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            set_error(std::string("upload: ") + #call + ": " + hipGetErrorString(e_)); \
+            arena.release();                                                           \
+            return e_ == hipErrorOutOfMemory ? MFD_ERR_NOMEM : MFD_ERR_HIP;            \
+        }                                                                              \
+    } while (0)
+
+int Model::upload() {
+    if (uploaded) return MFD_OK;
+    DevModel m{};
+    m.ndims = d.ndims;
+    m.Nx = d.Nx; m.Ny = d.Ny; m.Nz = d.Nz;
+    m.N = N;
+    m.sy = d.Nx;
+    m.sz = (long long)d.Nx * d.Ny;
+    m.idx = (float)(1.0 / d.dx);
+    m.idy = (float)(1.0 / d.dy);
+    m.idz = (float)(1.0 / d.dz);
+    m.dtf = (float)d.dt;
+    m.dt = d.dt;
+    for (int i = 0; i < 6; i++) m.bcs[i] = d.bcs[i];
+    m.Nt = d.Nt;
+    m.int_flags = int_flags;
+    for (int c = 0; c < 9; c++) {
+        m.F[c] = nullptr;
+        if (!comp_present(c)) continue;
+        UP(arena.alloc(&m.F[c], N));
+        UP(hipMemset(m.F[c], 0, N * sizeof(float)));
+    }
+    // materials: id per cell (uint8) and compact state slots
+    const int nmat = (int)mats.size();
+    std::vector<long long> cnt(nmat + 1, 0);
+    bool stateful = false;
+    for (const auto &hm : mats)
+        if (hm.d.nq > 0 || hm.d.isplasma) stateful = true;
+    has_mats = has_geom;
+    for (long long i = 0; i < N; i++)
+        if (geom[i] > nmat) {
+            set_error("geometry id " + std::to_string(geom[i]) + " has no material");
+            return MFD_ERR_ARG;
+        }
+    {
+        // the id array always exists (all zeros without materials) so that a batch
+        // mixing models with and without materials can read it for every model
+        std::vector<unsigned char> ids(N);
+        std::vector<int> slot(stateful ? N : 0);
+        for (long long i = 0; i < N; i++) {
+            const int g = geom[i];
+            ids[i] = (unsigned char)g;
+            if (stateful) slot[i] = (int)cnt[g];
+            cnt[g]++;
+        }
+        unsigned char *did;
+        UP(upload_vec(arena, &did, ids));
+        m.id = did;
+        if (stateful) {
+            int *ds;
+            UP(upload_vec(arena, &ds, slot));
+            m.slot = ds;
+        }
+    }
+    const int nc = ncomp();
+    hmats.assign(nmat + 1, DevMat{});
+    {
+        DevMat &v = hmats[0];
+        v.Mh = (float)d.Mh; v.Me = (float)d.Me; v.Md1 = (float)d.Md1; v.Md2 = (float)d.Md2;
+    }
+    for (int k = 0; k < nmat; k++) {
+        const HostMat &hm = mats[k];
+        DevMat &v = hmats[k + 1];
+        const long long c = cnt[k + 1];
+        v.Mh = (float)hm.d.Mh; v.Me = (float)hm.d.Me;
+        v.Md1 = (float)hm.d.Md1; v.Md2 = (float)hm.d.Md2;
+        v.flags = (hm.d.nq > 0 ? MAT_DISP : 0) | (hm.d.iskerr ? MAT_KERR : 0) |
+                  (hm.d.isplasma ? MAT_PLASMA : 0);
+        v.nq = hm.d.nq;
+        v.cnt = (int)c;
+        v.K2 = (float)(hm.d.Mk2 * hm.d.Me * hm.d.Me);
+        v.K3 = (float)(hm.d.Mk3 * hm.d.Me * hm.d.Me * hm.d.Me);
+        if (hm.d.nq > 0) {
+            float *p;
+            UP(upload_vec(arena, &p, to_f32(hm.Aq))); v.Aq = p;
+            UP(upload_vec(arena, &p, to_f32(hm.Bq))); v.Bq = p;
+            UP(upload_vec(arena, &p, to_f32(hm.Cq))); v.Cq = p;
+            UP(arena.alloc(&v.P, (size_t)hm.d.nq * nc * c));
+            UP(hipMemset(v.P, 0, (size_t)hm.d.nq * nc * c * sizeof(float2)));
+        }
+        if (hm.d.isplasma) {
+            v.Rava = hm.d.Rava; v.ksi = hm.d.ksi; v.rho0 = hm.d.rho0;
+            v.Cprho0 = hm.d.Cp * hm.d.rho0;
+            v.Marho0 = hm.d.Ma * hm.d.rho0;
+            v.Ap = (float)hm.d.Ap; v.Bp = (float)hm.d.Bp;
+            v.rate_kind = hm.d.rate_kind; v.rate_c = hm.d.rate_c; v.rate_k = hm.d.rate_k;
+            v.rate_n = (int)hm.rx.size();
+            double *dp;
+            UP(upload_vec(arena, &dp, hm.rx)); v.rate_x = dp;
+            UP(upload_vec(arena, &dp, hm.ry)); v.rate_y = dp;
+            UP(arena.alloc(&v.rhod, c));
+            UP(hipMemset(v.rhod, 0, c * sizeof(double2)));
+            UP(arena.alloc(&v.Pp, (size_t)nc * c));
+            UP(hipMemset(v.Pp, 0, (size_t)nc * c * sizeof(float2)));
+            UP(arena.alloc(&v.Pa, (size_t)nc * c));
+            UP(hipMemset(v.Pa, 0, (size_t)nc * c * sizeof(float)));
+            UP(arena.alloc(&v.JE, c));
+            UP(hipMemset(v.JE, 0, c * sizeof(float)));
+        }
+    }
+    DevMat *dmats;
+    UP(upload_vec(arena, &dmats, hmats));
+    m.mats = dmats;
+    // CPML layers
+    for (int f = 0; f < 6; f++) {
+        DevLayer &l = m.L[f];
+        l = DevLayer{};
+        if (!L[f].present) continue;
+        l.present = 1;
+        l.ib = L[f].ib - 1;
+        l.nb = L[f].nb;
+        float *p;
+        UP(upload_vec(arena, &p, to_f32(L[f].K))); l.K = p;
+        UP(upload_vec(arena, &p, to_f32(L[f].A))); l.A = p;
+        UP(upload_vec(arena, &p, to_f32(L[f].B))); l.B = p;
+        const size_t n = psi_len(*this, f, l.nb);
+        float **dst[4] = {&l.psiE0, &l.psiE1, &l.psiH0, &l.psiH1};
+        for (int k = 0; k < 4; k++) {
+            UP(arena.alloc(dst[k], n));
+            UP(hipMemset(*dst[k], 0, n * sizeof(float)));
+        }
+    }
+    // sources
+    hsrc.assign(srcs.size(), DevSource{});
+    maxsrc_n = 1;
+    maxface = 1;
+    has_tfsf = false;
+    for (size_t s = 0; s < srcs.size(); s++) {
+        const HostSource &hs = srcs[s];
+        DevSource &v = hsrc[s];
+        if (hs.type == 0) {
+            v.kind = hs.kind == MFD_SRC_HARD ? SRC_HARD : SRC_SOFT;
+            v.comp = hs.comp;
+            v.n = (int)hs.cells.size();
+            v.coeff = (float)hs.coeff;
+            int *ip;
+            UP(upload_vec(arena, &ip, hs.cells)); v.cells = ip;
+            float *fp;
+            UP(upload_vec(arena, &fp, to_f32(hs.wave))); v.wave = fp;
+            maxsrc_n = std::max(maxsrc_n, v.n);
+            continue;
+        }
+        has_tfsf = true;
+        v.kind = SRC_TFSF;
+        v.mode = hs.mode == MFD_TFSF_AUX1D ? 1 : 0;
+        // face scheme of the 3D replay; 1D and 2D degenerate tangential ranges
+        int b[6];
+        if (d.ndims == 3) {
+            for (int i = 0; i < 6; i++) b[i] = hs.box[i] - 1;
+        } else if (d.ndims == 2) {
+            b[0] = hs.box[0] - 1; b[1] = hs.box[1] - 1; b[2] = 0; b[3] = 0;
+            b[4] = hs.box[4] - 1; b[5] = hs.box[5] - 1;
+        } else {
+            b[0] = 0; b[1] = 1; b[2] = 0; b[3] = 0;
+            b[4] = hs.box[4] - 1; b[5] = hs.box[5] - 1;
+        }
+        for (int i = 0; i < 6; i++) v.box[i] = b[i];
+        for (int f = 0; f < 6; f++) {
+            const int axis = f >> 1;
+            const int ua = axis == 0 ? 1 : 0, va = axis == 2 ? 1 : 2;
+            maxface = std::max(maxface, (b[2 * ua + 1] - b[2 * ua] + 1) * (b[2 * va + 1] - b[2 * va] + 1));
+        }
+        const double ds[3] = {d.dx, d.dy, d.dz};
+        for (int a = 0; a < 3; a++) {
+            const double Cm = d.dt / MFD_MU0, Ce = d.dt / MFD_EPS0;
+            v.cm[a] = (float)(Cm / ds[a]);
+            v.ce[a] = (float)(Ce / ds[a]);
+            v.cd[a] = (float)(MFD_EPS0 * Ce / ds[a]);
+        }
+        if (v.mode == 0) {
+            int slotmap[24];
+            for (int k = 0; k < 24; k++) slotmap[k] = -1;
+            if (d.ndims == 3) {
+                for (int k = 0; k < 24; k++) slotmap[k] = k;
+            } else if (d.ndims == 2) {
+                // incHy_1z, incEz_1z, incHy_2z, incEz_2z, incHy_x1, incEx_x1, incHy_x2, incEx_x2
+                slotmap[0] = 0; slotmap[3] = 1; slotmap[4] = 2; slotmap[7] = 3;
+                slotmap[17] = 4; slotmap[18] = 5; slotmap[21] = 6; slotmap[22] = 7;
+            } else {
+                // incHy_1, incEx_1, incHy_2, incEx_2
+                slotmap[17] = 0; slotmap[18] = 1; slotmap[21] = 2; slotmap[22] = 3;
+            }
+            for (int k = 0; k < 24; k++) {
+                v.tab[k] = nullptr;
+                if (slotmap[k] < 0) continue;
+                float *fp;
+                UP(upload_vec(arena, &fp, to_f32(hs.tables[slotmap[k]])));
+                v.tab[k] = fp;
+            }
+        } else {
+            const int Nz = d.Nz;
+            UP(arena.alloc(&v.aux, 6 * (size_t)Nz));
+            UP(hipMemset(v.aux, 0, 6 * (size_t)Nz * sizeof(double)));
+            int nbmax = 1;
+            for (int f = 0; f < 2; f++) {
+                const HostLayer &hl = L[4 + f];
+                v.aux_present[f] = hl.present ? 1 : 0;
+                v.aux_ib[f] = hl.ib - 1;
+                v.aux_nb[f] = hl.nb;
+                if (hl.present) {
+                    nbmax = std::max(nbmax, hl.nb);
+                    double *dp;
+                    UP(upload_vec(arena, &dp, hl.K)); v.auxK[f] = dp;
+                    UP(upload_vec(arena, &dp, hl.A)); v.auxA[f] = dp;
+                    UP(upload_vec(arena, &dp, hl.B)); v.auxB[f] = dp;
+                }
+            }
+            v.aux_nbmax = nbmax;
+            UP(arena.alloc(&v.auxpsi, 8 * (size_t)nbmax));
+            UP(hipMemset(v.auxpsi, 0, 8 * (size_t)nbmax * sizeof(double)));
+            v.aux_n = (int)hs.aux_comp.size();
+            int *ip;
+            UP(upload_vec(arena, &ip, hs.aux_comp)); v.aux_comp = ip;
+            UP(upload_vec(arena, &ip, hs.aux_iz)); v.aux_iz = ip;
+            double *dp;
+            UP(upload_vec(arena, &dp, hs.aux_coeff)); v.aux_coeff = dp;
+            UP(upload_vec(arena, &dp, hs.aux_wave)); v.aux_wave = dp;
+            v.Mh0 = d.Mh; v.Me0 = d.Me; v.Md10 = d.Md1; v.Md20 = d.Md2;
+            v.idz = 1.0 / d.dz;
+        }
+    }
+    DevSource *dsrc;
+    UP(upload_vec(arena, &dsrc, hsrc));
+    m.src = dsrc;
+    m.nsrc = (int)srcs.size();
+    // monitors
+    int comps[6];
+    ncomp_mon = monitor_comps(d.ndims, comps);
+    hmon.assign(mons.size(), DevMon{});
+    maxmon_n = 1;
+    has_spectral = false;
+    for (size_t q = 0; q < mons.size(); q++) {
+        const HostMon &hm = mons[q];
+        DevMon &v = hmon[q];
+        v.spectral = hm.spectral;
+        v.issum = hm.issum;
+        v.n = (int)hm.cells.size();
+        v.ncomp = ncomp_mon;
+        for (int c = 0; c < 6; c++) v.comps[c] = c < ncomp_mon ? comps[c] : 0;
+        int *ip;
+        UP(upload_vec(arena, &ip, hm.cells)); v.cells = ip;
+        maxmon_n = std::max(maxmon_n, v.n);
+        if (!hm.spectral) {
+            if (hm.issum) {
+                UP(arena.alloc(&v.sum, (size_t)d.Nt * ncomp_mon));
+                UP(hipMemset(v.sum, 0, (size_t)d.Nt * ncomp_mon * sizeof(double)));
+            } else {
+                UP(arena.alloc(&v.rec, (size_t)d.Nt * ncomp_mon * v.n));
+                UP(hipMemset(v.rec, 0, (size_t)d.Nt * ncomp_mon * v.n * sizeof(float)));
+            }
+        } else {
+            has_spectral = true;
+            v.nw = (int)hm.w.size();
+            v.tshift = hm.tshift;
+            double *dp;
+            UP(upload_vec(arena, &dp, hm.w)); v.w = dp;
+            UP(arena.alloc(&v.samp, (size_t)dft_block * ncomp_mon * v.n));
+            const size_t ns = (size_t)ncomp_mon * v.nw * (hm.issum ? 1 : v.n);
+            UP(arena.alloc(&v.S, ns));
+            UP(hipMemset(v.S, 0, ns * sizeof(double2)));
+        }
+    }
+    DevMon *dmon;
+    UP(upload_vec(arena, &dmon, hmon));
+    m.mon = dmon;
+    m.nmon = (int)mons.size();
+    m.dft_block = dft_block;
+    if (int_flags & MFD_INT_SA) {
+        UP(arena.alloc(&m.Sa, N));
+        UP(hipMemset(m.Sa, 0, N * sizeof(float)));
+    }
+    if (int_flags & MFD_INT_E2) {
+        UP(arena.alloc(&m.E2, N));
+        UP(hipMemset(m.E2, 0, N * sizeof(float)));
+    }
+    int *ctr;
+    UP(arena.alloc(&ctr, 2));
+    UP(hipMemset(ctr, 0, 2 * sizeof(int)));
+    m.it = ctr;
+    m.flag = ctr + 1;
+    dm = m;
+    it = 0;
+    dft_done = 0;
+    uploaded = true;
+    return MFD_OK;
+}
+
+int Model::read_field(int comp, double *out) {
+    if (comp < 0 || comp > 8 || !comp_present(comp)) {
+        set_error("component not present in this model");
+        return MFD_ERR_ARG;
+    }
+    if (!uploaded) {
+        std::fill(out, out + N, 0.0);
+        return MFD_OK;
+    }
+    std::vector<float> tmp(N);
+    MFD_HIP(hipDeviceSynchronize());
+    MFD_HIP(hipMemcpy(tmp.data(), dm.F[comp], N * sizeof(float), hipMemcpyDeviceToHost));
+    for (long long i = 0; i < N; i++) out[i] = tmp[i];
+    return MFD_OK;
+}
+
+int Model::write_field(int comp, const double *in) {
+    if (comp < 0 || comp > 8 || !comp_present(comp)) {
+        set_error("component not present in this model");
+        return MFD_ERR_ARG;
+    }
+    int st = upload();
+    if (st != MFD_OK) return st;
+    std::vector<float> tmp(N);
+    for (long long i = 0; i < N; i++) tmp[i] = (float)in[i];
+    MFD_HIP(hipDeviceSynchronize());
+    MFD_HIP(hipMemcpy(dm.F[comp], tmp.data(), N * sizeof(float), hipMemcpyHostToDevice));
+    return MFD_OK;
+}
+
+// ---- Runner --------------------------------------------------------------------------
+Runner::~Runner() { release(); }
+
+void Runner::release() {
+    if (gexec) { hipGraphExecDestroy(gexec); gexec = nullptr; }
+    if (d_models) { hipFree(d_models); d_models = nullptr; }
+    if (d_tiles) { hipFree(d_tiles); d_tiles = nullptr; }
+    for (auto e : ev) hipEventDestroy(e);
+    ev.clear();
+}
+
+static void make_tiles(const Model &m, int idx, int zchunk, std::vector<Tile> &out) {
+    const int Nx = m.d.Nx, Ny = m.d.Ny, Nz = m.d.Nz;
+    if (m.d.ndims == 3) {
+        for (int z0 = 0; z0 < Nz; z0 += zchunk)
+            for (int y0 = 0; y0 < Ny; y0 += TY3)
+                for (int x0 = 0; x0 < Nx; x0 += TX3)
+                    out.push_back(make_int4(idx, x0, y0, z0 | (std::min(Nz, z0 + zchunk) << 16)));
+    } else if (m.d.ndims == 2) {
+        for (int z0 = 0; z0 < Nz; z0 += zchunk)
+            for (int x0 = 0; x0 < Nx; x0 += TX2)
+                out.push_back(make_int4(idx, x0, 0, z0 | (std::min(Nz, z0 + zchunk) << 16)));
+    } else {
+        for (int z0 = 0; z0 < Nz; z0 += TX1)
+            out.push_back(make_int4(idx, 0, 0, z0 | (std::min(Nz, z0 + TX1) << 16)));
+    }
+}
+
+int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_t s) {
+    release();
+    models = ms;
+    opt = o;
+    stream = s;
+    if (ms.empty()) { set_error("no models"); return MFD_ERR_ARG; }
+    ndims = ms[0]->d.ndims;
+    int dev = 0;
+    MFD_HIP(hipGetDevice(&dev));
+    MFD_HIP(hipDeviceGetAttribute(&cu_count, hipDeviceAttributeMultiprocessorCount, dev));
+    std::vector<DevModel> hd;
+    std::vector<Tile> tiles;
+    any_mats = any_int = any_mon = any_spectral = false;
+    maxsrc = 0;
+    maxsrc_n = maxmon_n = maxface = 1;
+    dft_block = ms[0]->dft_block;
+    long long maxN = 1;
+    for (size_t k = 0; k < ms.size(); k++) {
+        Model *m = ms[k];
+        if (m->d.ndims != ndims) { set_error("a run mixes 1D/2D/3D models"); return MFD_ERR_ARG; }
+        int st = m->upload();
+        if (st != MFD_OK) return st;
+        hd.push_back(m->dm);
+        make_tiles(*m, (int)k, opt.zchunk, tiles);
+        any_mats |= m->has_mats;
+        any_int |= m->int_flags != 0;
+        any_mon |= !m->mons.empty();
+        any_spectral |= m->has_spectral;
+        maxsrc = std::max(maxsrc, (int)m->srcs.size());
+        maxsrc_n = std::max(maxsrc_n, m->maxsrc_n);
+        maxmon_n = std::max(maxmon_n, m->maxmon_n);
+        maxface = std::max(maxface, m->maxface);
+        maxN = std::max(maxN, m->N);
+        if (m->dft_block != dft_block) { set_error("DFT block mismatch"); return MFD_ERR_ARG; }
+    }
+    src_kind_mask.assign(maxsrc, 0);
+    for (Model *m : ms)
+        for (size_t s2 = 0; s2 < m->srcs.size(); s2++)
+            src_kind_mask[s2] |= m->srcs[s2].type == 0 ? 1 : 2;
+    int_grid = (int)std::min<long long>((maxN + 255) / 256, (long long)cu_count * 16);
+    MFD_HIP(hipMalloc(&d_models, hd.size() * sizeof(DevModel)));
+    MFD_HIP(hipMemcpy(d_models, hd.data(), hd.size() * sizeof(DevModel), hipMemcpyHostToDevice));
+    ntiles = (int)tiles.size();
+    MFD_HIP(hipMalloc(&d_tiles, tiles.size() * sizeof(Tile)));
+    MFD_HIP(hipMemcpy(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    return MFD_OK;
+}
+
+int Runner::launch_step(bool timing) {
+    const int nm = (int)models.size();
+    int grid = ntiles;
+    if (opt.persistent > 0) grid = std::min(ntiles, cu_count * opt.persistent);
+    if (timing && ev.empty()) {
+        ev.resize(4);
+        for (auto &e : ev) MFD_HIP(hipEventCreate(&e));
+    }
+    if (timing) MFD_HIP(hipEventRecord(ev[0], stream));
+    if (ndims == 3) launch_h3d(d_models, d_tiles, ntiles, grid, any_mats, stream);
+    else if (ndims == 2) launch_h2d(d_models, d_tiles, ntiles, grid, any_mats, stream);
+    else launch_h1d(d_models, d_tiles, ntiles, grid, any_mats, stream);
+    if (timing) { MFD_HIP(hipEventRecord(ev[1], stream)); MFD_HIP(hipEventRecord(ev[2], stream)); }
+    if (ndims == 3) launch_e3d(d_models, d_tiles, ntiles, grid, any_mats, stream);
+    else if (ndims == 2) launch_e2d(d_models, d_tiles, ntiles, grid, any_mats, stream);
+    else launch_e1d(d_models, d_tiles, ntiles, grid, any_mats, stream);
+    if (timing) MFD_HIP(hipEventRecord(ev[3], stream));
+    for (int s = 0; s < maxsrc; s++) {
+        if (src_kind_mask[s] & 1) launch_point_sources(d_models, nm, s, maxsrc_n, stream);
+        if (src_kind_mask[s] & 2) launch_tfsf(d_models, nm, s, ndims, maxface, stream);
+    }
+    if (any_mon) launch_monitors(d_models, nm, maxmon_n, stream);
+    if (any_int) launch_integrals(d_models, nm, int_grid, ndims, stream);
+    launch_finish(d_models, nm, stream);
+    MFD_HIP(hipGetLastError());
+    if (timing) {
+        MFD_HIP(hipEventSynchronize(ev[3]));
+        float a = 0, b = 0;
+        MFD_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+        MFD_HIP(hipEventElapsedTime(&b, ev[2], ev[3]));
+        t_h_ms += a; t_e_ms += b; n_h++; n_e++;
+    }
+    return MFD_OK;
+}
+
+int Runner::build_graph(int steps) {
+    if (gexec) { hipGraphExecDestroy(gexec); gexec = nullptr; }
+    hipGraph_t g;
+    MFD_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int st = MFD_OK;
+    for (int k = 0; k < steps && st == MFD_OK; k++) {
+        st = launch_step(false);
+        if (st == MFD_OK && any_spectral && (k + 1) % dft_block == 0)
+            launch_dft_flush(d_models, (int)models.size(), maxmon_n, dft_block, stream);
+    }
+    hipError_t e = hipStreamEndCapture(stream, &g);
+    if (st != MFD_OK) return st;
+    if (e != hipSuccess) { set_error(std::string("graph capture: ") + hipGetErrorString(e)); return MFD_ERR_HIP; }
+    e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) { set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return MFD_ERR_HIP; }
+    gsteps = steps;
+    return MFD_OK;
+}
+
+// One step of every model; models that already finished are skipped on the device.
+int Runner::run(int nsteps) {
+    int done = 0;
+    const bool timing = opt.kernel_timing;
+    int G = opt.graph_steps;
+    if (any_spectral && G > 0) G = dft_block;
+    auto step_eager = [&]() -> int {
+        int st = launch_step(timing);
+        if (st != MFD_OK) return st;
+        for (Model *m : models) if (m->it < m->d.Nt) m->it++;
+        if (any_spectral) {
+            bool due = false;
+            for (Model *m : models) due |= m->has_spectral && (m->it - m->dft_done) == dft_block;
+            if (due) {
+                launch_dft_flush(d_models, (int)models.size(), maxmon_n, dft_block, stream);
+                for (Model *m : models) if (m->has_spectral && m->it - m->dft_done == dft_block) m->dft_done = m->it;
+            }
+        }
+        return MFD_OK;
+    };
+    while (done < nsteps) {
+        bool aligned = true;
+        if (any_spectral)
+            for (Model *m : models) aligned &= (m->it - m->dft_done) % dft_block == 0;
+        if (G > 0 && !timing && aligned && nsteps - done >= G) {
+            if (!gexec || gsteps != G) {
+                int st = build_graph(G);
+                if (st != MFD_OK) return st;
+            }
+            MFD_HIP(hipGraphLaunch(gexec, stream));
+            for (Model *m : models) {
+                const int adv = std::min(G, m->d.Nt - m->it);
+                m->it += adv;
+                if (m->has_spectral) m->dft_done += (adv / dft_block) * dft_block;
+            }
+            done += G;
+        } else {
+            int st = step_eager();
+            if (st != MFD_OK) return st;
+            done += 1;
+        }
+    }
+    return check_flags();
+}
+
+int Runner::flush_dft_partial() {
+    if (!any_spectral) return MFD_OK;
+    for (size_t k = 0; k < models.size(); k++) {
+        Model *m = models[k];
+        const int nk = m->it - m->dft_done;
+        if (!m->has_spectral || nk <= 0) continue;
+        launch_dft_flush(d_models + k, 1, m->maxmon_n, nk, stream);
+        m->dft_done = m->it;
+    }
+    MFD_HIP(hipStreamSynchronize(stream));
+    return MFD_OK;
+}
+
+int Runner::check_flags() {
+    MFD_HIP(hipStreamSynchronize(stream));
+    for (Model *m : models) {
+        int f = 0;
+        MFD_HIP(hipMemcpy(&f, m->dm.flag, sizeof(int), hipMemcpyDeviceToHost));
+        if (f & 1) { set_error("Something went wrong. I found NaN field values."); return MFD_ERR_NAN; }
+        if (f & 2) { set_error("Something went wrong. I found Inf field values."); return MFD_ERR_INF; }
+    }
+    return MFD_OK;
+}
+
+}  // namespace mfd
+
+// =======================================================================================
+// C ABI
+// =======================================================================================
+using namespace mfd;
+
+struct mfd_model {
+    Model m;
+    Runner runner;
+    bool runner_ready = false;
+    RunOptions opt;
+    hipStream_t stream = nullptr;
+};
+
+static int check_device() {
+    static int ok = -1;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
+    if (ok >= 0) return ok;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+        set_error("no HIP device visible");
+        ok = MFD_ERR_NODEV;
+        return ok;
+    }
+    hipDeviceProp_t p;
+    int dev = 0;
+    hipGetDevice(&dev);
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess || std::strncmp(p.gcnArchName, "gfx950", 6) != 0) {
+        set_error(std::string("device is not gfx950: ") + p.gcnArchName);
+        ok = MFD_ERR_NODEV;
+        return ok;
+    }
+    ok = MFD_OK;
+    return ok;
+}
+
+extern "C" {
+
+int32_t mfd_version(void) { return MFD_ABI_VERSION; }
+const char *mfd_last_error(void) { return g_error.c_str(); }
+
+int32_t mfd_device_info(int64_t *free_bytes, int64_t *total_bytes, int32_t *cu_count, char *name,
+                        int32_t name_len) {
+    int st = check_device();
+    if (st != MFD_OK) return st;
+    size_t fr = 0, tot = 0;
+    MFD_HIP(hipMemGetInfo(&fr, &tot));
+    int dev = 0;
+    MFD_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    MFD_HIP(hipGetDeviceProperties(&p, dev));
+    if (free_bytes) *free_bytes = (int64_t)fr;
+    if (total_bytes) *total_bytes = (int64_t)tot;
+    if (cu_count) *cu_count = p.multiProcessorCount;
+    if (name && name_len > 0) {
+        std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    return MFD_OK;
+}
+
+int32_t mfd_model_create(const mfd_model_desc *desc, mfd_model **out) {
+    if (!desc || !out) { set_error("null argument"); return MFD_ERR_ARG; }
+    if (desc->ndims < 1 || desc->ndims > 3 || desc->Nz < 2 || desc->Nx < 1 || desc->Ny < 1 ||
+        (desc->ndims == 1 && (desc->Nx != 1 || desc->Ny != 1)) ||
+        (desc->ndims == 2 && (desc->Ny != 1 || desc->Nx < 2)) ||
+        (desc->ndims == 3 && (desc->Nx < 2 || desc->Ny < 2)) || desc->Nt < 1 ||
+        desc->Nz > 65535 || !(desc->dt > 0)) {
+        set_error("bad model descriptor");
+        return MFD_ERR_ARG;
+    }
+    for (int i = 0; i < 6; i++)
+        if (desc->bcs[i] < 1 || desc->bcs[i] > 3) {
+            set_error("Wrong 'bc' value. It can be either ':periodic', ':dirichlet', or ':neumann'.");
+            return MFD_ERR_ARG;
+        }
+    int st = check_device();
+    if (st != MFD_OK) return st;
+    mfd_model *h = new mfd_model();
+    h->m.d = *desc;
+    h->m.N = (long long)desc->Nx * desc->Ny * desc->Nz;
+    h->m.geom.assign(h->m.N, 0);
+    *out = h;
+    return MFD_OK;
+}
+
+#define NEED_HOST(h)                                                                    \
+    do {                                                                                \
+        if (!(h)) { set_error("null model"); return MFD_ERR_ARG; }                      \
+        if ((h)->m.uploaded) { set_error("model already uploaded"); return MFD_ERR_STATE; } \
+    } while (0)
+
+int32_t mfd_model_set_geometry(mfd_model *h, const int32_t *ids) {
+    NEED_HOST(h);
+    h->m.has_geom = false;
+    for (long long i = 0; i < h->m.N; i++) {
+        if (ids[i] < 0 || ids[i] > 255) { set_error("geometry ids must be in 0..255"); return MFD_ERR_ARG; }
+        h->m.geom[i] = ids[i];
+        if (ids[i] > 0) h->m.has_geom = true;
+    }
+    return MFD_OK;
+}
+
+int32_t mfd_model_add_material(mfd_model *h, const mfd_material_desc *mat) {
+    NEED_HOST(h);
+    if (!mat || mat->nq < 0 || h->m.mats.size() >= 255) { set_error("bad material"); return MFD_ERR_ARG; }
+    HostMat hm;
+    hm.d = *mat;
+    hm.Aq.assign(mat->Aq, mat->Aq + mat->nq);
+    hm.Bq.assign(mat->Bq, mat->Bq + mat->nq);
+    hm.Cq.assign(mat->Cq, mat->Cq + mat->nq);
+    if (mat->isplasma) {
+        if (mat->rate_kind == MFD_RATE_LOGLOG) {
+            if (mat->rate_n < 2) { set_error("loglog rate table needs >= 2 points"); return MFD_ERR_ARG; }
+            hm.rx.assign(mat->rate_x, mat->rate_x + mat->rate_n);
+            hm.ry.assign(mat->rate_y, mat->rate_y + mat->rate_n);
+        } else if (mat->rate_kind != MFD_RATE_POWER) {
+            set_error("plasma material needs a power-law or log-log ionization rate");
+            return MFD_ERR_ARG;
+        }
+    }
+    hm.d.Aq = hm.d.Bq = hm.d.Cq = nullptr;
+    hm.d.rate_x = hm.d.rate_y = nullptr;
+    h->m.mats.push_back(std::move(hm));
+    return MFD_OK;
+}
+
+int32_t mfd_model_set_pml(mfd_model *h, int32_t face, int32_t ib, int32_t nb, const double *K,
+                          const double *A, const double *B) {
+    NEED_HOST(h);
+    const auto &d = h->m.d;
+    const int n = face < 2 ? d.Nx : face < 4 ? d.Ny : d.Nz;
+    if (face < 0 || face > 5 || nb < 1 || ib < 1 || ib > n ||
+        (face % 2 == 0 && nb != ib) || (face % 2 == 1 && nb != n - ib + 1)) {
+        set_error("bad pml layer");
+        return MFD_ERR_ARG;
+    }
+    if ((d.ndims == 1 && face < 4) || (d.ndims == 2 && (face == 2 || face == 3))) {
+        set_error("pml face not present for this dimensionality");
+        return MFD_ERR_ARG;
+    }
+    HostLayer &l = h->m.L[face];
+    l.present = true;
+    l.ib = ib;
+    l.nb = nb;
+    l.K.assign(K, K + nb);
+    l.A.assign(A, A + nb);
+    l.B.assign(B, B + nb);
+    return MFD_OK;
+}
+
+int32_t mfd_model_add_source(mfd_model *h, int32_t kind, int32_t comp, double coeff, int32_t n,
+                             const int32_t *cells, const double *waveform) {
+    NEED_HOST(h);
+    if (n < 1) { set_error("I did not find any grid points which satisfy your source geometry."); return MFD_ERR_ARG; }
+    if (comp < 0 || comp > 8 || !h->m.comp_present(comp) || (kind != MFD_SRC_SOFT && kind != MFD_SRC_HARD)) {
+        set_error("bad source component or kind");
+        return MFD_ERR_ARG;
+    }
+    HostSource s;
+    s.type = 0;
+    s.kind = kind;
+    s.comp = comp;
+    s.coeff = coeff;
+    s.cells.assign(cells, cells + n);
+    for (int c : s.cells)
+        if (c < 0 || c >= h->m.N) { set_error("source cell out of range"); return MFD_ERR_ARG; }
+    s.wave.assign(waveform, waveform + (size_t)n * h->m.d.Nt);
+    h->m.srcs.push_back(std::move(s));
+    return MFD_OK;
+}
+
+static size_t tfsf_len(const mfd_model_desc &d, const int *b, int k) {
+    const size_t Nt = d.Nt;
+    const int Nxi = b[1] - b[0], Nyi = b[3] - b[2], Nzi = b[5] - b[4];
+    if (d.ndims == 1) return Nt;
+    if (d.ndims == 2) return (size_t)(k < 4 ? Nzi : Nxi) * Nt;
+    const int face = k / 4, j = k % 4;
+    const bool big = j == 0 || j == 3;
+    size_t a, c;
+    if (face < 2) { a = big ? Nyi + 1 : Nyi; c = big ? Nzi : Nzi + 1; }
+    else if (face < 4) { a = big ? Nxi + 1 : Nxi; c = big ? Nzi : Nzi + 1; }
+    else { a = big ? Nxi + 1 : Nxi; c = big ? Nyi : Nyi + 1; }
+    return a * c * Nt;
+}
+
+int32_t mfd_model_add_tfsf(mfd_model *h, const mfd_tfsf_desc *t) {
+    NEED_HOST(h);
+    const auto &d = h->m.d;
+    if (!t) { set_error("null tfsf"); return MFD_ERR_ARG; }
+    const int Ns[3] = {d.Nx, d.Ny, d.Nz};
+    for (int a = 0; a < 3; a++) {
+        if (d.ndims == 1 && a < 2) continue;
+        if (d.ndims == 2 && a == 1) continue;
+        const int i1 = t->box[2 * a], i2 = t->box[2 * a + 1];
+        if (!(i1 >= 2 && i1 < i2 && i2 <= Ns[a])) {
+            set_error("TFSF box must lie strictly inside the grid: 2 <= i1 < i2 <= N");
+            return MFD_ERR_ARG;
+        }
+    }
+    HostSource s;
+    s.type = 1;
+    for (int i = 0; i < 6; i++) s.box[i] = t->box[i];
+    s.mode = t->mode;
+    if (t->mode == MFD_TFSF_TABLES) {
+        const int need = d.ndims == 1 ? 4 : d.ndims == 2 ? 8 : 24;
+        if (t->ntables != need || !t->tables) { set_error("wrong number of TFSF tables"); return MFD_ERR_ARG; }
+        for (int k = 0; k < need; k++) {
+            const size_t len = tfsf_len(d, t->box, k);
+            s.tables.emplace_back(t->tables[k], t->tables[k] + len);
+        }
+    } else if (t->mode == MFD_TFSF_AUX1D) {
+        if (t->aux_n < 1) { set_error("plane-wave TFSF needs an auxiliary source"); return MFD_ERR_ARG; }
+        for (int k = 0; k < t->aux_n; k++) {
+            const int c = t->aux_comp[k];
+            if (!(c == MFD_EX || c == MFD_EY || c == MFD_DX || c == MFD_DY || c == MFD_HX || c == MFD_HY) ||
+                t->aux_iz[k] < 0 || t->aux_iz[k] >= d.Nz) {
+                set_error("bad auxiliary TFSF source");
+                return MFD_ERR_ARG;
+            }
+        }
+        s.aux_comp.assign(t->aux_comp, t->aux_comp + t->aux_n);
+        s.aux_iz.assign(t->aux_iz, t->aux_iz + t->aux_n);
+        s.aux_coeff.assign(t->aux_coeff, t->aux_coeff + t->aux_n);
+        s.aux_wave.assign(t->aux_wave, t->aux_wave + (size_t)t->aux_n * d.Nt);
+    } else {
+        set_error("bad TFSF mode");
+        return MFD_ERR_ARG;
+    }
+    h->m.srcs.push_back(std::move(s));
+    return MFD_OK;
+}
+
+int32_t mfd_model_add_field_monitor(mfd_model *h, int32_t n, const int32_t *cells, int32_t issum) {
+    NEED_HOST(h);
+    if (n < 1) { set_error("I did not find any grid points which satisfy your monitor geometry."); return MFD_ERR_ARG; }
+    HostMon m;
+    m.spectral = false;
+    m.issum = issum != 0;
+    m.cells.assign(cells, cells + n);
+    for (int c : m.cells)
+        if (c < 0 || c >= h->m.N) { set_error("monitor cell out of range"); return MFD_ERR_ARG; }
+    h->m.mons.push_back(std::move(m));
+    return MFD_OK;
+}
+
+int32_t mfd_model_add_spectral_monitor(mfd_model *h, int32_t n, const int32_t *cells, int32_t nw,
+                                       const double *w, double tshift, int32_t issum) {
+    NEED_HOST(h);
+    if (n < 1) { set_error("I did not find any grid points which satisfy your monitor geometry."); return MFD_ERR_ARG; }
+    if (nw < 1) { set_error("spectral monitor needs at least one frequency"); return MFD_ERR_ARG; }
+    HostMon m;
+    m.spectral = true;
+    m.issum = issum != 0;
+    m.cells.assign(cells, cells + n);
+    for (int c : m.cells)
+        if (c < 0 || c >= h->m.N) { set_error("monitor cell out of range"); return MFD_ERR_ARG; }
+    m.w.assign(w, w + nw);
+    m.tshift = tshift;
+    h->m.mons.push_back(std::move(m));
+    return MFD_OK;
+}
+
+int32_t mfd_model_enable_integrals(mfd_model *h, int32_t flags) {
+    NEED_HOST(h);
+    h->m.int_flags = flags & (MFD_INT_SA | MFD_INT_E2);
+    return MFD_OK;
+}
+
+int32_t mfd_model_footprint(mfd_model *h, int64_t *bytes) {
+    if (!h || !bytes) { set_error("null argument"); return MFD_ERR_ARG; }
+    *bytes = h->m.uploaded ? (int64_t)h->m.arena.bytes : h->m.footprint();
+    return MFD_OK;
+}
+
+int32_t mfd_model_upload(mfd_model *h) {
+    if (!h) { set_error("null model"); return MFD_ERR_ARG; }
+    return h->m.upload();
+}
+
+static int ensure_runner(mfd_model *h) {
+    if (h->runner_ready) return MFD_OK;
+    if (!h->stream) MFD_HIP(hipStreamCreate(&h->stream));
+    int st = h->runner.init({&h->m}, h->opt, h->stream);
+    if (st != MFD_OK) return st;
+    h->runner_ready = true;
+    return MFD_OK;
+}
+
+int32_t mfd_model_step(mfd_model *h, int32_t nsteps) {
+    if (!h || nsteps < 0) { set_error("bad arguments"); return MFD_ERR_ARG; }
+    if (h->m.it + nsteps > h->m.d.Nt) { set_error("model already ran all Nt steps"); return MFD_ERR_STATE; }
+    int st = ensure_runner(h);
+    if (st != MFD_OK) return st;
+    return h->runner.run(nsteps);
+}
+
+int32_t mfd_model_get_it(mfd_model *h, int32_t *it) {
+    if (!h || !it) { set_error("null argument"); return MFD_ERR_ARG; }
+    *it = h->m.it;
+    return MFD_OK;
+}
+
+int32_t mfd_model_get_field(mfd_model *h, int32_t comp, double *out) {
+    if (!h || !out) { set_error("null argument"); return MFD_ERR_ARG; }
+    return h->m.read_field(comp, out);
+}
+
+int32_t mfd_model_set_field(mfd_model *h, int32_t comp, const double *in) {
+    if (!h || !in) { set_error("null argument"); return MFD_ERR_ARG; }
+    return h->m.write_field(comp, in);
+}
+
+int32_t mfd_model_get_monitor(mfd_model *h, int32_t imon, int32_t comp, double *out) {
+    if (!h || !out || imon < 0 || imon >= (int)h->m.mons.size()) { set_error("bad monitor"); return MFD_ERR_ARG; }
+    Model &m = h->m;
+    const HostMon &hm = m.mons[imon];
+    int comps[6];
+    const int nc = monitor_comps(m.d.ndims, comps);
+    int ci = -1;
+    for (int c = 0; c < nc; c++) if (comps[c] == comp) ci = c;
+    if (ci < 0) { set_error("component not recorded by monitors of this model"); return MFD_ERR_ARG; }
+    const size_t n = hm.cells.size(), Nt = m.d.Nt;
+    if (!m.uploaded) {
+        const size_t len = hm.spectral ? 2 * hm.w.size() * (hm.issum ? 1 : n) : (hm.issum ? Nt : n * Nt);
+        std::fill(out, out + len, 0.0);
+        return MFD_OK;
+    }
+    if (h->runner_ready) {
+        int st = h->runner.flush_dft_partial();
+        if (st != MFD_OK) return st;
+    }
+    MFD_HIP(hipDeviceSynchronize());
+    const DevMon &dmn = m.hmon[imon];
+    if (!hm.spectral) {
+        if (hm.issum) {
+            std::vector<double> tmp(Nt * nc);
+            MFD_HIP(hipMemcpy(tmp.data(), dmn.sum, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+            for (size_t t = 0; t < Nt; t++) out[t] = tmp[t * nc + ci];
+        } else {
+            std::vector<float> tmp(Nt * nc * n);
+            MFD_HIP(hipMemcpy(tmp.data(), dmn.rec, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+            for (size_t t = 0; t < Nt; t++)
+                for (size_t k = 0; k < n; k++) out[k + n * t] = tmp[(t * nc + ci) * n + k];
+        }
+    } else {
+        const size_t nw = hm.w.size(), np = hm.issum ? 1 : n;
+        std::vector<double2> tmp(nc * nw * np);
+        MFD_HIP(hipMemcpy(tmp.data(), dmn.S, tmp.size() * sizeof(double2), hipMemcpyDeviceToHost));
+        for (size_t w = 0; w < nw; w++)
+            for (size_t k = 0; k < np; k++) {
+                const double2 v = tmp[((size_t)ci * nw + w) * np + k];
+                out[2 * (k + np * w)] = v.x;
+                out[2 * (k + np * w) + 1] = v.y;
+            }
+    }
+    return MFD_OK;
+}
+
+int32_t mfd_model_get_output(mfd_model *h, int32_t which, double *out) {
+    if (!h || !out) { set_error("null argument"); return MFD_ERR_ARG; }
+    Model &m = h->m;
+    const long long N = m.N;
+    if (!m.uploaded) { std::fill(out, out + N, 0.0); return MFD_OK; }
+    MFD_HIP(hipDeviceSynchronize());
+    if (which == MFD_OUT_SA || which == MFD_OUT_E2) {
+        const float *src = which == MFD_OUT_SA ? m.dm.Sa : m.dm.E2;
+        if (!src) { set_error("integral not enabled"); return MFD_ERR_STATE; }
+        std::vector<float> tmp(N);
+        MFD_HIP(hipMemcpy(tmp.data(), src, N * sizeof(float), hipMemcpyDeviceToHost));
+        for (long long i = 0; i < N; i++) out[i] = tmp[i];
+        return MFD_OK;
+    }
+    if (which != MFD_OUT_JE && which != MFD_OUT_RHO) { set_error("bad output"); return MFD_ERR_ARG; }
+    std::fill(out, out + N, 0.0);
+    for (size_t k = 0; k < m.mats.size(); k++) {
+        if (!m.mats[k].d.isplasma) continue;
+        const DevMat &v = m.hmats[k + 1];
+        const int cnt = v.cnt;
+        if (cnt == 0) continue;
+        std::vector<double> vals(cnt);
+        if (which == MFD_OUT_JE) {
+            std::vector<float> tmp(cnt);
+            MFD_HIP(hipMemcpy(tmp.data(), v.JE, cnt * sizeof(float), hipMemcpyDeviceToHost));
+            for (int i = 0; i < cnt; i++) vals[i] = tmp[i];
+        } else {
+            std::vector<double2> tmp(cnt);
+            MFD_HIP(hipMemcpy(tmp.data(), v.rhod, cnt * sizeof(double2), hipMemcpyDeviceToHost));
+            for (int i = 0; i < cnt; i++) vals[i] = tmp[i].x * m.mats[k].d.rho0;
+        }
+        int s = 0;
+        for (long long i = 0; i < N; i++)
+            if (m.geom[i] == (int)k + 1) out[i] += vals[s++];
+    }
+    return MFD_OK;
+}
+
+int32_t mfd_model_destroy(mfd_model *h) {
+    if (!h) return MFD_OK;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    h->runner.release();
+    h->m.arena.release();
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return MFD_OK;
+}
+
+// Engine options (not part of the reference interface): 0 graph steps per replay,
+// 1 z planes per tile, 2 persistent workgroups per CU (0 = one per tile),
+// 3 kernel timing (hipEvent pairs around the H/E launches), 4 DFT time-block length.
+int32_t mfd_model_set_options(mfd_model *h, int32_t opt, int32_t value) {
+    if (!h) { set_error("null model"); return MFD_ERR_ARG; }
+    switch (opt) {
+        case 0: h->opt.graph_steps = std::max(0, value); break;
+        case 1:
+            if (value < 1 || value > 65535) { set_error("bad z chunk"); return MFD_ERR_ARG; }
+            h->opt.zchunk = value;
+            break;
+        case 2: h->opt.persistent = std::max(0, value); break;
+        case 3: h->opt.kernel_timing = value != 0; break;
+        case 4:
+            if (h->m.uploaded || value < 1 || value > 64) { set_error("DFT block must be 1..64, set before upload"); return MFD_ERR_ARG; }
+            h->m.dft_block = value;
+            break;
+        default: set_error("unknown option"); return MFD_ERR_ARG;
+    }
+    if (h->runner_ready) {
+        h->runner.release();
+        h->runner_ready = false;
+    }
+    return MFD_OK;
+}
+
+// Accumulated hipEvent timing of the H (which = 0) or E (which = 1) launches.
+int32_t mfd_model_timing(mfd_model *h, int32_t which, double *total_ms, int32_t *count) {
+    if (!h || !total_ms || !count) { set_error("null argument"); return MFD_ERR_ARG; }
+    *total_ms = which == 0 ? h->runner.t_h_ms : h->runner.t_e_ms;
+    *count = (int32_t)(which == 0 ? h->runner.n_h : h->runner.n_e);
+    if (which == 2) { h->runner.t_h_ms = h->runner.t_e_ms = 0; h->runner.n_h = h->runner.n_e = 0; }
+    return MFD_OK;
+}
+
+}  // extern "C"

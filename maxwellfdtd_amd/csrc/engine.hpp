@@ -1,0 +1,150 @@
+// Host-side engine objects behind the C ABI: a Model owns its host description and,
+// once uploaded, its device state; a Runner advances a set of device-resident models
+// (one model, a batch, or the server's active set) through the shared launches.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/maxwell_fdtd.h"
+#include "common.hpp"
+
+namespace mfd {
+
+void set_error(const std::string &msg);
+
+#define MFD_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            ::mfd::set_error(std::string(#call) + ": " + hipGetErrorString(e_));       \
+            return MFD_ERR_HIP;                                                        \
+        }                                                                              \
+    } while (0)
+
+struct HostLayer {
+    bool present = false;
+    int ib = 0, nb = 0;
+    std::vector<double> K, A, B;
+};
+
+struct HostMat {
+    mfd_material_desc d{};
+    std::vector<double> Aq, Bq, Cq, rx, ry;
+};
+
+struct HostSource {
+    int type = 0;   // 0 point, 1 tfsf
+    int kind = 0, comp = 0;
+    double coeff = 0;
+    std::vector<int> cells;
+    std::vector<double> wave;
+    int box[6] = {0, 0, 0, 0, 0, 0};
+    int mode = 0;
+    std::vector<std::vector<double>> tables;
+    std::vector<int> aux_comp, aux_iz;
+    std::vector<double> aux_coeff, aux_wave;
+};
+
+struct HostMon {
+    bool spectral = false, issum = false;
+    std::vector<int> cells;
+    std::vector<double> w;
+    double tshift = 0;
+};
+
+// device allocations of one model
+struct Arena {
+    std::vector<void *> ptrs;
+    size_t bytes = 0;
+    ~Arena();
+    template <class T>
+    hipError_t alloc(T **p, size_t n) {
+        void *q = nullptr;
+        size_t b = n * sizeof(T);
+        if (b == 0) b = 16;
+        hipError_t e = hipMalloc(&q, b);
+        if (e != hipSuccess) return e;
+        ptrs.push_back(q);
+        bytes += b;
+        *p = (T *)q;
+        return hipSuccess;
+    }
+    void release();
+};
+
+struct Model {
+    mfd_model_desc d{};
+    long long N = 0;
+    std::vector<int32_t> geom;
+    bool has_geom = false;
+    std::vector<HostMat> mats;
+    HostLayer L[6];
+    std::vector<HostSource> srcs;
+    std::vector<HostMon> mons;
+    int int_flags = 0;
+
+    // device state
+    bool uploaded = false;
+    Arena arena;
+    DevModel dm{};              // host copy of the device descriptor
+    std::vector<DevMat> hmats;  // host copies of the material table
+    std::vector<DevSource> hsrc;
+    std::vector<DevMon> hmon;
+    int it = 0;                 // completed steps (host mirror of *dm.it)
+    int dft_done = 0;           // steps already contracted into the DFT accumulators
+    int dft_block = 32;
+    int ncomp_mon = 0;
+    int maxsrc_n = 0, maxmon_n = 0, maxface = 0;
+    bool has_mats = false, has_tfsf = false, has_spectral = false;
+
+    int ncomp() const { return d.ndims == 3 ? 3 : d.ndims == 2 ? 2 : 1; }
+    bool comp_present(int c) const;
+    int64_t footprint() const;
+    int upload();
+    int read_field(int comp, double *out);
+    int write_field(int comp, const double *in);
+};
+
+struct RunOptions {
+    int graph_steps = 16;       // steps captured per hipGraph replay (0 = eager launches)
+    int zchunk = 32;            // z planes per tile (3D/2D)
+    int persistent = 0;         // 0: one workgroup per tile; >0: that many workgroups per CU
+    bool kernel_timing = false; // hipEvent pairs around the H/E launches (eager mode)
+};
+
+// Advances a fixed set of uploaded models together.
+struct Runner {
+    std::vector<Model *> models;
+    DevModel *d_models = nullptr;
+    Tile *d_tiles = nullptr;
+    int ntiles = 0;
+    int ndims = 3;
+    bool any_mats = false, any_int = false, any_mon = false, any_spectral = false;
+    int maxsrc = 0;
+    std::vector<int> src_kind_mask;   // per slot: 1 point, 2 tfsf
+    int maxsrc_n = 1, maxmon_n = 1, maxface = 1, int_grid = 1;
+    int dft_block = 32;
+    RunOptions opt;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int gsteps = 0;
+    int cu_count = 256;
+    std::vector<hipEvent_t> ev;
+    double t_h_ms = 0, t_e_ms = 0;
+    long long n_h = 0, n_e = 0;
+
+    ~Runner();
+    int init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_t s);
+    void release();
+    int launch_step(bool timing);     // one step of every model, on `stream`
+    int build_graph(int steps);
+    int run(int nsteps);              // nsteps steps (models that finish are skipped)
+    int flush_dft_partial();
+    int check_flags();
+    int collect_timing();
+};
+
+}  // namespace mfd

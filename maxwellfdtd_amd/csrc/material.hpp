@@ -1,0 +1,131 @@
+// Per-cell material update of the E half-step, shared by the 1D/2D/3D kernels:
+// ADE poles, plasma (FP64 density and rate), Kerr Pade and E = (D - sum P) * Me.
+// Statement order follows update_E_kernel! 3D (src/models.jl:1043-1154).
+#pragma once
+#include "common.hpp"
+
+namespace mfd {
+
+constexpr double DBL_EPS_ = 2.220446049250313e-16;   // eps(Float64), src/models.jl:1106
+
+// ionrate(I) in FP64: power law sigma*I^K, or the log-log table of IonizationRate
+// (examples/5_nonlinear_propagation/IonizationRate.jl:30-52).
+__device__ inline double ionrate(const DevMat &mt, double I) {
+    if (!(I > 0.0)) return 0.0;
+    if (mt.rate_kind == 1) return mt.rate_c * pow(I, mt.rate_k);
+    if (mt.rate_kind == 2) {
+        const double *x = mt.rate_x, *y = mt.rate_y;
+        const int n = mt.rate_n;
+        const double xl = log10(I);
+        int i;
+        if (xl <= x[0]) i = 0;
+        else if (xl >= x[n - 1]) i = n - 2;
+        else {
+            int lo = 0, hi = n - 1;
+            while (lo < hi) {
+                int mid = (lo + hi) >> 1;
+                if (x[mid] >= xl) hi = mid; else lo = mid + 1;
+            }
+            i = lo - 1;
+        }
+        const double dydx = (y[i + 1] - y[i]) / (x[i + 1] - x[i]);
+        return exp10(y[i] + dydx * (xl - x[i]));
+    }
+    return 0.0;
+}
+
+// d[c]: updated D; e[c]: on entry E^n (read only when the material has ADE/plasma
+// state), on exit E^{n+1}.  s is the cell's state slot within its material.
+template <int NC>
+__device__ __forceinline__ void material_e(const DevMat &mt, int s, const float (&d)[NC],
+                                           float (&e)[NC], double dt, float dtf) {
+    const int flags = mt.flags;
+    float sumP[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) sumP[c] = 0.0f;
+    if (flags & MAT_DISP) {
+        const int nq = mt.nq, cnt = mt.cnt;
+        for (int q = 0; q < nq; q++) {
+            const float A = mt.Aq[q], B = mt.Bq[q], Cc = mt.Cq[q];
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                float2 *pp = mt.P + (size_t)(q * NC + c) * cnt + s;
+                const float2 p = *pp;
+                const float pn = A * p.x + B * p.y + Cc * e[c];
+                *pp = make_float2(pn, p.x);
+                sumP[c] += pn;
+            }
+        }
+    }
+    if (flags & MAT_PLASMA) {
+        const int cnt = mt.cnt;
+        double E2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < NC; c++) E2 += (double)e[c] * (double)e[c];
+        const double II = mt.ksi * E2;
+        double2 rd = mt.rhod[s];
+        const float cp = (float)(mt.Cprho0 * rd.x);
+        float J[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            float2 *pp = mt.Pp + (size_t)c * cnt + s;
+            const float2 p = *pp;
+            const float pn = mt.Ap * p.x + mt.Bp * p.y + cp * e[c];
+            *pp = make_float2(pn, p.x);
+            sumP[c] += pn;
+            J[c] = (pn - p.y) / (2.0f * dtf);
+        }
+        const double invE2 = (E2 >= DBL_EPS_) ? 1.0 / E2 : 0.0;
+        const float ca = (float)(mt.Marho0 * rd.y * invE2);
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            float *pa = mt.Pa + (size_t)c * cnt + s;
+            const float v = *pa + ca * e[c];
+            *pa = v;
+            sumP[c] += v;
+        }
+        const double R1 = ionrate(mt, II);
+        const double R2 = mt.Rava * E2;
+        double rho;
+        if (R2 == 0.0) {
+            rho = 1.0 - (1.0 - rd.x) * exp(-R1 * dt);
+        } else {
+            const double R12 = R1 - R2;
+            rho = R1 / R12 - (R1 / R12 - rd.x) * exp(-R12 * dt);
+        }
+        rd.x = rho;
+        rd.y = R1 * (1.0 - rho);
+        mt.rhod[s] = rd;
+        float je = 0.0f;
+#pragma unroll
+        for (int c = 0; c < NC; c++) je += J[c] * e[c];
+        mt.JE[s] += je * dtf;
+    }
+    const float Me = mt.Me;
+    if (flags & MAT_KERR) {
+        const float K2 = mt.K2, K3 = mt.K3;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const float u = d[c] - sumP[c];
+            e[c] = (1.0f + K2 * u + 2.0f * K3 * (u * u)) /
+                   (1.0f + 2.0f * K2 * u + 3.0f * K3 * (u * u)) * u * Me;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < NC; c++) e[c] = (d[c] - sumP[c]) * Me;
+    }
+}
+
+// Non-finite guard on Ex (src/models.jl:202-209): one atomic per wave that saw one.
+__device__ __forceinline__ void check_finite(float ex, int *flag, bool active) {
+    const bool bad = active && !isfinite(ex);
+    if (__any(bad)) {
+        if (bad) atomicOr(flag, isnan(ex) ? 1 : 2);
+    }
+}
+
+__device__ __forceinline__ float bcval(int bc, float periodic, float own) {
+    return bc == BC_PERIODIC ? periodic : (bc == BC_DIRICHLET ? 0.0f : own);
+}
+
+}  // namespace mfd

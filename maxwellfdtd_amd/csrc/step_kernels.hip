@@ -1,0 +1,437 @@
+// Per-step kernels around the H/E half-steps: soft/hard sources, TFSF (auxiliary 1D
+// run + face corrections), field/spectral monitors, the time-blocked DFT, integral
+// variables and the device step counter.  Every kernel takes the DevModel array of a
+// batch; blockIdx.y (or blockIdx.x for one-block-per-model kernels) selects the model,
+// and a model that finished its Nt steps is skipped.
+#include "common.hpp"
+#include "material.hpp"
+
+namespace mfd {
+
+constexpr double MU0_ = 1.25663706212e-6;
+
+// ---- soft / hard sources: add_source_kernel! (src/sources.jl:77-119,190-232) ---------
+__global__ __launch_bounds__(256) void k_point_sources(const DevModel *__restrict__ models,
+                                                       int slot) {
+    const DevModel &m = models[blockIdx.y];
+    if (slot >= m.nsrc) return;
+    const DevSource &S = m.src[slot];
+    if (S.kind == SRC_TFSF) return;
+    const int it0 = *m.it;
+    if (it0 >= m.Nt) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    bool wroteEx = false;
+    float v = 0.0f;
+    if (k < S.n) {
+        float *F = m.F[S.comp];
+        const int c = S.cells[k];
+        const float w = S.wave[(size_t)it0 * S.n + k];
+        v = (S.kind == SRC_SOFT) ? F[c] + S.coeff * w : w;
+        F[c] = v;
+        wroteEx = S.comp == EX;
+    }
+    check_finite(v, m.flag, wroteEx);
+}
+
+void launch_point_sources(const DevModel *m, int nmodels, int slot, int maxn, hipStream_t s) {
+    const int nb = (maxn + 255) / 256;
+    hipLaunchKernelGGL(k_point_sources, dim3(nb, nmodels), dim3(256), 0, s, m, slot);
+}
+
+// ---- TFSF auxiliary run --------------------------------------------------------------
+// One step of the x/y-uniform record run in FP64 (the transverse differences of the 3D
+// update vanish, so update_H_kernel!/update_E_kernel! 3D reduce to these z recursions,
+// src/models.jl:735-1162): H half, E half, then the aux soft sources.
+__global__ __launch_bounds__(256) void k_tfsf_aux(const DevModel *__restrict__ models,
+                                                  int slot) {
+    const DevModel &m = models[blockIdx.x];
+    if (slot >= m.nsrc) return;
+    const DevSource &S = m.src[slot];
+    if (S.kind != SRC_TFSF || S.mode != 1) return;
+    const int it0 = *m.it;
+    if (it0 >= m.Nt) return;
+    const int Nz = m.Nz;
+    double *Hx = S.aux, *Hy = S.aux + Nz, *Dx = S.aux + 2 * Nz, *Dy = S.aux + 3 * Nz;
+    double *Ex = S.aux + 4 * Nz, *Ey = S.aux + 5 * Nz;
+    const int nbm = S.aux_nbmax;
+    const double Mh = S.Mh0, Me = S.Me0, Md1 = S.Md10, Md2 = S.Md20, dz = 1.0 / S.idz;
+    const int bcl = m.bcs[4], bcr = m.bcs[5];
+    for (int iz = threadIdx.x; iz < Nz; iz += blockDim.x) {
+        double ex1, ey1;
+        if (iz + 1 < Nz) { ex1 = Ex[iz + 1]; ey1 = Ey[iz + 1]; }
+        else if (bcr == BC_PERIODIC) { ex1 = Ex[0]; ey1 = Ey[0]; }
+        else if (bcr == BC_DIRICHLET) { ex1 = 0.0; ey1 = 0.0; }
+        else { ex1 = Ex[iz]; ey1 = Ey[iz]; }
+        double dExz = (ex1 - Ex[iz]) / dz;
+        double dEyz = (ey1 - Ey[iz]) / dz;
+        double hx = Hx[iz], hy = Hy[iz];
+        for (int f = 0; f < 2; f++) {
+            if (!S.aux_present[f]) continue;
+            const bool in = f == 0 ? iz <= S.aux_ib[0] : iz >= S.aux_ib[1];
+            if (!in) continue;
+            const int p = f == 0 ? iz : iz - S.aux_ib[1];
+            double *pEx = S.auxpsi + (size_t)(f * 4 + 0) * nbm;
+            double *pEy = S.auxpsi + (size_t)(f * 4 + 1) * nbm;
+            const double A = S.auxA[f][p], B = S.auxB[f][p], K = S.auxK[f][p];
+            pEx[p] = B * pEx[p] + A * dExz;
+            pEy[p] = B * pEy[p] + A * dEyz;
+            hx += Mh * pEy[p];
+            hy -= Mh * pEx[p];
+            dExz = dExz / K;
+            dEyz = dEyz / K;
+        }
+        hx -= Mh * (0.0 - dEyz);
+        hy -= Mh * (dExz - 0.0);
+        Hx[iz] = hx;
+        Hy[iz] = hy;
+    }
+    __syncthreads();
+    for (int iz = threadIdx.x; iz < Nz; iz += blockDim.x) {
+        double hx0, hy0;
+        if (iz > 0) { hx0 = Hx[iz - 1]; hy0 = Hy[iz - 1]; }
+        else if (bcl == BC_PERIODIC) { hx0 = Hx[Nz - 1]; hy0 = Hy[Nz - 1]; }
+        else if (bcl == BC_DIRICHLET) { hx0 = 0.0; hy0 = 0.0; }
+        else { hx0 = Hx[iz]; hy0 = Hy[iz]; }
+        double dHxz = (Hx[iz] - hx0) / dz;
+        double dHyz = (Hy[iz] - hy0) / dz;
+        double dx = Dx[iz], dy = Dy[iz];
+        for (int f = 0; f < 2; f++) {
+            if (!S.aux_present[f]) continue;
+            const bool in = f == 0 ? iz <= S.aux_ib[0] : iz >= S.aux_ib[1];
+            if (!in) continue;
+            const int p = f == 0 ? iz : iz - S.aux_ib[1];
+            double *pHx = S.auxpsi + (size_t)(f * 4 + 2) * nbm;
+            double *pHy = S.auxpsi + (size_t)(f * 4 + 3) * nbm;
+            const double A = S.auxA[f][p], B = S.auxB[f][p], K = S.auxK[f][p];
+            pHx[p] = B * pHx[p] + A * dHxz;
+            pHy[p] = B * pHy[p] + A * dHyz;
+            dx -= Md2 * pHy[p];
+            dy += Md2 * pHx[p];
+            dHxz = dHxz / K;
+            dHyz = dHyz / K;
+        }
+        dx = Md1 * dx + Md2 * (0.0 - dHyz);
+        dy = Md1 * dy + Md2 * (dHxz - 0.0);
+        Dx[iz] = dx;
+        Dy[iz] = dy;
+        Ex[iz] = dx * Me;
+        Ey[iz] = dy * Me;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < S.aux_n; k++) {
+            const int iz = S.aux_iz[k];
+            const double v = S.aux_coeff[k] * S.aux_wave[(size_t)k * m.Nt + it0];
+            double *dst;
+            switch (S.aux_comp[k]) {
+                case HX: dst = Hx; break;
+                case HY: dst = Hy; break;
+                case DX: dst = Dx; break;
+                case DY: dst = Dy; break;
+                case EX: dst = Ex; break;
+                default: dst = Ey; break;
+            }
+            dst[iz] = dst[iz] + v;
+        }
+    }
+}
+
+// ---- TFSF face corrections: add_source!(::TFSFSource3D) (src/sources.jl:713-776) -------
+// Face f = 2*axis + side.  (u, v) are the tangential axes (x face: y,z; y face: x,z;
+// z face: x,y); the face tables are H_u, H_v, E_u, E_v.  1D and 2D models use faces
+// 4,5 (resp. 0,1,4,5) with degenerate tangential ranges.
+__device__ inline double aux_value(const DevSource &S, int Nz, int comp, int z) {
+    switch (comp) {
+        case HX: return S.aux[z];
+        case HY: return S.aux[Nz + z];
+        case EX: return S.aux[4 * Nz + z];
+        case EY: return S.aux[5 * Nz + z];
+        default: return 0.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tfsf_face(const DevModel *__restrict__ models,
+                                                   int slot, int f) {
+    const DevModel &m = models[blockIdx.y];
+    if (slot >= m.nsrc) return;
+    const DevSource &S = m.src[slot];
+    if (S.kind != SRC_TFSF) return;
+    const int it0 = *m.it;
+    if (it0 >= m.Nt) return;
+    const int axis = f >> 1, side = f & 1;
+    const int ua = axis == 0 ? 1 : 0, va = axis == 2 ? 1 : 2;
+    const int U1 = S.box[2 * ua], U2 = S.box[2 * ua + 1];
+    const int V1 = S.box[2 * va], V2 = S.box[2 * va + 1];
+    const int N1 = S.box[2 * axis], N2 = S.box[2 * axis + 1];
+    const int nu = U2 - U1 + 1, nv = V2 - V1 + 1;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nu * nv) return;
+    const int u = U1 + e % nu, v = V1 + e / nu;
+    const int nH = side ? N2 : N1 - 1, nE = side ? N2 : N1;
+    const float sg = side ? -1.0f : 1.0f;
+    const float s0 = (axis == 1 ? 1.0f : -1.0f) * sg;
+    int cHu, cHv, cEu, cEv, cDu, cDv;
+    if (axis == 0) { cHu = HY; cHv = HZ; cEu = EY; cEv = EZ; cDu = DY; cDv = DZ; }
+    else if (axis == 1) { cHu = HX; cHv = HZ; cEu = EX; cEv = EZ; cDu = DX; cDv = DZ; }
+    else { cHu = HX; cHv = HY; cEu = EX; cEv = EY; cDu = DX; cDv = DY; }
+    const int tabc[4] = {cHu, cHv, cEu, cEv};
+    const int Nu = U2 - U1, Nv = V2 - V1;
+    auto inc = [&](int k) -> float {
+        if (S.mode == 1) {
+            int z;
+            if (axis == 2) z = side ? N2 : (k < 2 ? N1 - 1 : N1);
+            else z = v;
+            return (float)aux_value(S, m.Nz, tabc[k], z);
+        }
+        const float *T = S.tab[4 * f + k];
+        if (!T) return 0.0f;
+        const int na = (k == 0 || k == 3) ? Nu + 1 : Nu;
+        const int nb = (k == 0 || k == 3) ? Nv : Nv + 1;
+        return T[(u - U1) + (size_t)na * ((v - V1) + (size_t)nb * it0)];
+    };
+    auto cell = [&](int n) -> long long {
+        int ix, iy, iz;
+        if (axis == 0) { ix = n; iy = u; iz = v; }
+        else if (axis == 1) { ix = u; iy = n; iz = v; }
+        else { ix = u; iy = v; iz = n; }
+        return ix + m.sy * iy + m.sz * iz;
+    };
+    const float cm = S.cm[axis], ce = S.ce[axis], cd = S.cd[axis];
+    // In the auxiliary (normal-incidence) mode the Hz and Ez incident fields are
+    // identically zero; their corrections add exact zeros and are skipped.
+    auto live = [&](int k) { return S.mode == 0 || (tabc[k] != HZ && tabc[k] != EZ); };
+    if (u <= U2 && v <= V2 - 1 && live(3)) {     // H_u += (s0) Cm/d * inc E_v
+        const float iv = inc(3);
+        float *F = m.F[cHu];
+        const long long c = cell(nH);
+        F[c] = F[c] + (s0 * cm) * iv;
+    }
+    if (u <= U2 - 1 && v <= V2 && live(2)) {     // H_v += (-s0) Cm/d * inc E_u
+        const float iv = inc(2);
+        float *F = m.F[cHv];
+        const long long c = cell(nH);
+        F[c] = F[c] + (-s0 * cm) * iv;
+    }
+    if (u <= U2 - 1 && v <= V2 && live(1)) {     // E_u, D_u += (-s0) Ce/d * inc H_v
+        const float iv = inc(1);
+        const long long c = cell(nE);
+        float *E = m.F[cEu];
+        float *D = m.F[cDu];
+        E[c] = E[c] + (-s0 * ce) * iv;
+        D[c] = D[c] + (-s0 * cd) * iv;
+    }
+    if (u <= U2 && v <= V2 - 1 && live(0)) {     // E_v, D_v += (s0) Ce/d * inc H_u
+        const float iv = inc(0);
+        const long long c = cell(nE);
+        float *E = m.F[cEv];
+        float *D = m.F[cDv];
+        E[c] = E[c] + (s0 * ce) * iv;
+        D[c] = D[c] + (s0 * cd) * iv;
+    }
+}
+
+void launch_tfsf(const DevModel *m, int nmodels, int slot, int ndims, int maxface,
+                 hipStream_t s) {
+    hipLaunchKernelGGL(k_tfsf_aux, dim3(nmodels), dim3(256), 0, s, m, slot);
+    const int nb = (maxface + 255) / 256;
+    for (int f = 0; f < 6; f++) {
+        if (ndims == 1 && f < 4) continue;
+        if (ndims == 2 && (f == 2 || f == 3)) continue;
+        hipLaunchKernelGGL(k_tfsf_face, dim3(nb, nmodels), dim3(256), 0, s, m, slot, f);
+    }
+}
+
+// ---- monitors: update_monitor! (src/monitors.jl:62-66,110-116,164-176,350-359) --------
+__device__ double block_sum(double v, double *sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (t < o) sh[t] += sh[t + o];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_monitors(const DevModel *__restrict__ models) {
+    __shared__ double sh[256];
+    const DevModel &m = models[blockIdx.y];
+    const int it0 = *m.it;
+    if (it0 >= m.Nt) return;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * blockDim.x;
+    for (int q = 0; q < m.nmon; q++) {
+        const DevMon &M = m.mon[q];
+        if (!M.spectral && M.issum) {
+            if (blockIdx.x != 0) continue;
+            for (int c = 0; c < M.ncomp; c++) {
+                const float *F = m.F[M.comps[c]];
+                double acc = 0.0;
+                for (int k = threadIdx.x; k < M.n; k += blockDim.x) acc += (double)F[M.cells[k]];
+                const double tot = block_sum(acc, sh);
+                if (threadIdx.x == 0) M.sum[(size_t)it0 * M.ncomp + c] = tot;
+            }
+        } else if (!M.spectral) {
+            for (int k = gid; k < M.n; k += stride)
+                for (int c = 0; c < M.ncomp; c++)
+                    M.rec[((size_t)it0 * M.ncomp + c) * M.n + k] = m.F[M.comps[c]][M.cells[k]];
+        } else {
+            const int kk = it0 % m.dft_block;
+            for (int k = gid; k < M.n; k += stride)
+                for (int c = 0; c < M.ncomp; c++)
+                    M.samp[((size_t)kk * M.ncomp + c) * M.n + k] = m.F[M.comps[c]][M.cells[k]];
+        }
+    }
+}
+
+void launch_monitors(const DevModel *m, int nmodels, int maxn, hipStream_t s) {
+    int nb = (maxn + 255) / 256;
+    if (nb < 1) nb = 1;
+    if (nb > 1024) nb = 1024;
+    hipLaunchKernelGGL(k_monitors, dim3(nb, nmodels), dim3(256), 0, s, m);
+}
+
+// ---- time-blocked DFT (src/dft.jl:27-51): S += sum_k F_k * 2 exp(-i w (t_k - tshift)) dt
+// The nk buffered samples of the last time block are contracted against an FP64 phase
+// table staged in LDS; runs after the step counter advanced past the block.
+constexpr int WCH = 32;
+constexpr int KMAX = 64;
+
+__global__ __launch_bounds__(256) void k_dft_flush(const DevModel *__restrict__ models, int nk) {
+    __shared__ double2 ph[KMAX * WCH];
+    __shared__ double sh[256];
+    const DevModel &m = models[blockIdx.y];
+    const int itn = *m.it;
+    const int base = itn - nk;
+    if (base < 0 || nk <= 0) return;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * blockDim.x;
+    const double dt = m.dt;
+    for (int q = 0; q < m.nmon; q++) {
+        const DevMon &M = m.mon[q];
+        if (!M.spectral) continue;
+        for (int w0 = 0; w0 < M.nw; w0 += WCH) {
+            const int nwc = min(WCH, M.nw - w0);
+            __syncthreads();
+            for (int e = threadIdx.x; e < nk * nwc; e += blockDim.x) {
+                const int k = e / nwc, w = e % nwc;
+                const double t = (double)(base + k) * dt;
+                const double th = M.w[w0 + w] * (t - M.tshift);
+                double sn, cs;
+                sincos(th, &sn, &cs);
+                ph[k * WCH + w] = make_double2(2.0 * cs * dt, -2.0 * sn * dt);
+            }
+            __syncthreads();
+            if (!M.issum) {
+                for (int j = gid; j < M.n; j += stride) {
+                    for (int c = 0; c < M.ncomp; c++) {
+                        double re[WCH], im[WCH];
+                        for (int w = 0; w < WCH; w++) { re[w] = 0.0; im[w] = 0.0; }
+                        for (int k = 0; k < nk; k++) {
+                            const int kk = (base + k) % m.dft_block;
+                            const double F = (double)M.samp[((size_t)kk * M.ncomp + c) * M.n + j];
+#pragma unroll
+                            for (int w = 0; w < WCH; w++) {
+                                const double2 p = ph[k * WCH + w];
+                                re[w] += F * p.x;
+                                im[w] += F * p.y;
+                            }
+                        }
+                        for (int w = 0; w < nwc; w++) {
+                            double2 *Sp = M.S + ((size_t)c * M.nw + (w0 + w)) * M.n + j;
+                            double2 sv = *Sp;
+                            sv.x += re[w];
+                            sv.y += im[w];
+                            *Sp = sv;
+                        }
+                    }
+                }
+            } else if (blockIdx.x == 0) {
+                for (int c = 0; c < M.ncomp; c++) {
+                    for (int k = 0; k < nk; k++) {
+                        const int kk = (base + k) % m.dft_block;
+                        double acc = 0.0;
+                        for (int j = threadIdx.x; j < M.n; j += blockDim.x)
+                            acc += (double)M.samp[((size_t)kk * M.ncomp + c) * M.n + j];
+                        const double Fs = block_sum(acc, sh);
+                        if (threadIdx.x < nwc) {
+                            const double2 p = ph[k * WCH + threadIdx.x];
+                            double2 *Sp = M.S + (size_t)c * M.nw + w0 + threadIdx.x;
+                            double2 sv = *Sp;
+                            sv.x += Fs * p.x;
+                            sv.y += Fs * p.y;
+                            *Sp = sv;
+                        }
+                        __syncthreads();
+                    }
+                }
+            }
+        }
+    }
+}
+
+void launch_dft_flush(const DevModel *m, int nmodels, int maxn, int nk, hipStream_t s) {
+    int nb = (maxn + 255) / 256;
+    if (nb < 1) nb = 1;
+    if (nb > 2048) nb = 2048;
+    hipLaunchKernelGGL(k_dft_flush, dim3(nb, nmodels), dim3(256), 0, s, m, nk);
+}
+
+// ---- integral variables: update_integral_variables! (src/outputs.jl:182-191,318-327,
+// 475-485): Sa += |E x H| dt, E2 += |E|^2 dt over all cells, after the sources.
+__global__ __launch_bounds__(256) void k_integrals(const DevModel *__restrict__ models,
+                                                   int ndims) {
+    const DevModel &m = models[blockIdx.y];
+    if (*m.it >= m.Nt || m.int_flags == 0) return;
+    const long long N = m.N;
+    const float dt = m.dtf;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    float *Sa = m.Sa, *E2 = m.E2;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N; i += stride) {
+        if (ndims == 3) {
+            const float hx = m.F[HX][i], hy = m.F[HY][i], hz = m.F[HZ][i];
+            const float ex = m.F[EX][i], ey = m.F[EY][i], ez = m.F[EZ][i];
+            if (Sa) {
+                const float a = ey * hz - ez * hy, b = ez * hx - ex * hz, c = ex * hy - ey * hx;
+                Sa[i] += sqrtf(a * a + b * b + c * c) * dt;
+            }
+            if (E2) E2[i] += (ex * ex + ey * ey + ez * ez) * dt;
+        } else if (ndims == 2) {
+            const float hy = m.F[HY][i], ex = m.F[EX][i], ez = m.F[EZ][i];
+            if (Sa) {
+                const float a = -ez * hy, b = ex * hy;
+                Sa[i] += sqrtf(a * a + b * b) * dt;
+            }
+            if (E2) E2[i] += (ex * ex + ez * ez) * dt;
+        } else {
+            const float hy = m.F[HY][i], ex = m.F[EX][i];
+            if (Sa) {
+                const float a = ex * hy;
+                Sa[i] += sqrtf(a * a) * dt;
+            }
+            if (E2) E2[i] += ex * ex * dt;
+        }
+    }
+}
+
+void launch_integrals(const DevModel *m, int nmodels, int grid, int ndims, hipStream_t s) {
+    hipLaunchKernelGGL(k_integrals, dim3(grid, nmodels), dim3(256), 0, s, m, ndims);
+}
+
+// ---- step counter ----------------------------------------------------------------------
+__global__ void k_finish(const DevModel *__restrict__ models, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) {
+        const DevModel &m = models[k];
+        const int it = *m.it;
+        if (it < m.Nt) *m.it = it + 1;
+    }
+}
+
+void launch_finish(const DevModel *m, int nmodels, hipStream_t s) {
+    hipLaunchKernelGGL(k_finish, dim3((nmodels + 255) / 256), dim3(256), 0, s, m, nmodels);
+}
+
+}  // namespace mfd

@@ -1,0 +1,258 @@
+"""Model builders shared by the CPU (oracle) and GPU (engine) tests.
+
+The first group translates the reference's own test files (test/1d_test.jl,
+test/2d_test.jl, test/3d_test.jl, test/*_test_losses.jl) into this package's API.
+The second group are small seeded models that switch on every feature of the per-step
+update (CPML on all faces, materials, ADE poles, Kerr, plasma, TFSF, monitors,
+integrals, every boundary condition) for parity checks of the engine against the oracle.
+"""
+import numpy as np
+
+from maxwellfdtd_amd import (C0, EPS0, CPML, DebyeSusceptibility, DrudeSusceptibility,
+                             FieldMonitor, Grid, HardSource, IonizationRate,
+                             LorentzSusceptibility, Material, Model, Plasma, PowerLawRate,
+                             SoftSource, SpectralMonitor, TFSFSource, susceptibility)
+
+
+def gauss_waveform(lam0=2e-6, tau0=20e-15, dt0=None):
+    w0 = 2 * np.pi * C0 / lam0
+    dt0 = 3 * tau0 if dt0 is None else dt0
+    return lambda t: np.exp(-(t - dt0) ** 2 / tau0 ** 2) * np.cos(w0 * (t - dt0))
+
+
+# ---- translated reference tests ------------------------------------------------------
+def ref_1d_test():
+    """test/1d_test.jl:1-22 -> (model, Eth, rtol)."""
+    grid = Grid(zmin=-5e-6, zmax=55e-6, Nz=2001)
+    w = gauss_waveform()
+    src = HardSource(geometry=lambda z: np.abs(z) < grid.dz / 2,
+                     waveform=lambda z, t: w(t) + 0 * z, component="Ex")
+    model = Model(grid, src, tmax=150e-15, pml=4e-6)
+    Eth = w(model.t[-1] - grid.z / C0)
+    return model, Eth, 1e-2
+
+
+def ref_2d_test():
+    """test/2d_test.jl:1-28."""
+    grid = Grid(xmin=-55e-6, xmax=55e-6, Nx=11, zmin=-5e-6, zmax=55e-6, Nz=1101)
+    w = gauss_waveform()
+    a0 = 10e-6
+
+    def waveform(x, z, t):
+        return np.exp(-0.5 * (x / a0) ** 2) * w(t)
+
+    src = HardSource(geometry=lambda x, z: np.abs(z) < grid.dz / 2 + 0 * x,
+                     waveform=waveform, component="Ex")
+    model = Model(grid, src, tmax=150e-15, pml=4e-6)
+    X, Z = np.meshgrid(grid.x, grid.z, indexing="ij")
+    Eth = np.exp(-0.5 * (X / a0) ** 2) * w(model.t[-1] - Z / C0)
+    return model, Eth, 5e-2
+
+
+def ref_3d_test():
+    """test/3d_test.jl:1-30."""
+    grid = Grid(xmin=-55e-6, xmax=55e-6, Nx=11, ymin=-55e-6, ymax=55e-6, Ny=11,
+                zmin=-5e-6, zmax=55e-6, Nz=801)
+    w = gauss_waveform()
+    a0 = 10e-6
+
+    def waveform(x, y, z, t):
+        return np.exp(-0.5 * (np.sqrt(x ** 2 + y ** 2) / a0) ** 2) * w(t)
+
+    src = HardSource(geometry=lambda x, y, z: np.abs(z) < grid.dz / 2 + 0 * x + 0 * y,
+                     waveform=waveform, component="Ex")
+    model = Model(grid, src, tmax=150e-15, pml=4e-6)
+    X, Y, Z = np.meshgrid(grid.x, grid.y, grid.z, indexing="ij")
+    Eth = np.exp(-0.5 * (np.sqrt(X ** 2 + Y ** 2) / a0) ** 2) * w(model.t[-1] - Z / C0)
+    return model, Eth, 1e-1
+
+
+def _loss_material(kind, geometry):
+    lam0 = 2e-6
+    w0 = 2 * np.pi * C0 / lam0
+    if kind == "sigma":
+        sigma, er = 300.0, 1.5 ** 2
+        ei = sigma / (EPS0 * w0)
+        ni = np.sqrt((np.sqrt(er ** 2 + ei ** 2) - er) / 2)
+        ki = ni * w0 / C0
+        return Material(geometry=geometry, eps=er, sigma=sigma), ki, 300e-15
+    wq = 2 * np.pi * C0 / 1e-6
+    chi = LorentzSusceptibility(deps=2.0, w0=wq, delta=0.01 * wq)
+    n0 = np.sqrt(1 + susceptibility(chi, w0))
+    ki = np.imag(n0) * w0 / C0
+    return Material(geometry=geometry, chi=chi), ki, 600e-15
+
+
+def ref_losses(ndims, kind):
+    """test/{1d,2d,3d}_test_losses.jl: (model, ki, z) for the Beer-Lambert check on Sa."""
+    w = gauss_waveform()
+    if ndims == 1:
+        grid = Grid(zmin=-15e-6, zmax=25e-6, Nz=2001)
+        src = SoftSource(geometry=lambda z: np.abs(z - -10e-6) <= grid.dz / 2,
+                         waveform=lambda z, t: w(t) + 0 * z, component="Ex")
+        geom = lambda z: z >= 0
+    elif ndims == 2:
+        grid = Grid(xmin=-25e-6, xmax=25e-6, Nx=61, zmin=-15e-6, zmax=25e-6, Nz=601)
+        src = SoftSource(geometry=lambda x, z: np.abs(z - -10e-6) <= grid.dz / 2 + 0 * x,
+                         waveform=lambda x, z, t: w(t) + 0 * x, component="Ex")
+        geom = lambda x, z: (z >= 0) + 0 * x > 0
+    else:
+        grid = Grid(xmin=-25e-6, xmax=25e-6, Nx=11, ymin=-25e-6, ymax=25e-6, Ny=11,
+                    zmin=-15e-6, zmax=25e-6, Nz=601)
+        src = SoftSource(geometry=lambda x, y, z: np.abs(z - -10e-6) <= grid.dz / 2 + 0 * x + 0 * y,
+                         waveform=lambda x, y, z, t: w(t) + 0 * x, component="Ex")
+        geom = lambda x, y, z: (z >= 0) + 0 * x + 0 * y > 0
+    mat, ki, tmax = _loss_material(kind, geom)
+    pml = 5e-6
+    model = Model(grid, src, tmax=tmax, pml=pml, material=mat)
+    return model, ki
+
+
+def beer_lambert_ok(Sa, z, ki, rtol=1e-2):
+    """test_intensity_losses_*D (test/1d_test_losses.jl:1-17): Sa normalised on
+    z in [0, 20 um] against exp(-2 ki z)."""
+    Sa = np.asarray(Sa, dtype=float)
+    mask = (z < 0) | (z > 20e-6)
+    shape = [1] * Sa.ndim
+    shape[-1] = len(z)
+    m = np.broadcast_to(mask.reshape(shape), Sa.shape)
+    S = np.where(m, 0.0, Sa)
+    S = S / S.max()
+    I = np.where(m, 0.0, np.broadcast_to(np.exp(-2 * ki * z).reshape(shape), Sa.shape))
+    den = max(np.linalg.norm(S), np.linalg.norm(I))
+    return np.linalg.norm(S - I) <= rtol * den
+
+
+# ---- seeded all-feature models ---------------------------------------------------------
+def spheres_3d(n=32, steps=120, seed=7, tfsf=True, soft=True, monitors=True, nspheres=5,
+               bc="periodic", pml=4):
+    """n^3 grid, CPML `pml` cells on every face, seeded dielectric spheres, a TFSF plane
+    wave and a soft point source."""
+    rng = np.random.default_rng(seed)
+    d = 40e-9
+    L = (n - 1) * d
+    grid = Grid(xmin=0, xmax=L, Nx=n, ymin=0, ymax=L, Ny=n, zmin=0, zmax=L, Nz=n)
+    mats = []
+    for _ in range(nspheres):
+        c = rng.uniform(0.3 * L, 0.7 * L, 3)
+        r = rng.uniform(2, 5) * d
+        eps = rng.uniform(1.5, 4.0)
+        mats.append(Material(geometry=lambda x, y, z, c=c, r=r:
+                             (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2 <= r ** 2,
+                             eps=eps))
+    w = gauss_waveform(lam0=800e-9, tau0=5e-15, dt0=12e-15)
+    srcs = []
+    if tfsf:
+        b0, b1 = (pml + 3) * d, L - (pml + 3) * d
+        srcs.append(TFSFSource(box=(b0, b1, b0, b1, b0, b1), waveform=w, component="Ex",
+                               z0=(pml + 1) * d))
+    if soft:
+        srcs.append(SoftSource(geometry=(0.45 * L, 0.5 * L, 0.55 * L),
+                               waveform=lambda x, y, z, t: 1e3 * w(t) + 0 * x, component="Ey"))
+    dt = 0.99 / C0 / np.sqrt(3 / d ** 2)
+    model = Model(grid, srcs, tmax=(steps - 1) * dt, CN=0.99, pml=pml * d, material=mats, bc=bc)
+    mons = None
+    if monitors:
+        w0 = 2 * np.pi * C0 / 800e-9
+        mons = [FieldMonitor((0.5 * L, 0.5 * L, 0.7 * L)),
+                FieldMonitor(lambda x, y, z: (np.abs(z - 0.7 * L) < d / 2) & (np.abs(x - 0.5 * L) < 2.5 * d)
+                             & (np.abs(y - 0.5 * L) < 2.5 * d), sum=True),
+                SpectralMonitor(lambda x, y, z: (np.abs(z - 0.75 * L) < d / 2) & (x > 0.3 * L) & (x < 0.7 * L)
+                                & (y > 0.4 * L) & (y < 0.6 * L), w=w0 * np.linspace(0.8, 1.2, 5)),
+                SpectralMonitor((0.5 * L, 0.5 * L, 0.3 * L), w=w0 * np.linspace(0.9, 1.1, 3), sum=True)]
+    return model, mons
+
+
+def dispersive_3d(n=28, steps=150, seed=11):
+    """Drude + 2 Lorentz metal sphere over a Debye substrate, TFSF illumination."""
+    rng = np.random.default_rng(seed)
+    d = 5e-9
+    L = (n - 1) * d
+    grid = Grid(xmin=0, xmax=L, Nx=n, ymin=0, ymax=L, Ny=n, zmin=0, zmax=L, Nz=n)
+    metal_chi = [DrudeSusceptibility(wp=1.37e16, gamma=1e14),
+                 LorentzSusceptibility(deps=1.09, w0=4.2e15, delta=0.9e15),
+                 LorentzSusceptibility(deps=0.5, w0=6.0e15, delta=1.2e15)]
+    c = np.array([0.5, 0.5, 0.55]) * L + rng.uniform(-1, 1, 3) * d
+    metal = Material(geometry=lambda x, y, z: (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2
+                     <= (6 * d) ** 2, eps=1.0, chi=metal_chi)
+    sub = Material(geometry=lambda x, y, z: (z < 0.3 * L) + 0 * x + 0 * y > 0, eps=2.0,
+                   chi=DebyeSusceptibility(deps=2.0, tau=1e-14))
+    w = gauss_waveform(lam0=700e-9, tau0=2e-15, dt0=6e-15)
+    src = TFSFSource(box=(6 * d, L - 6 * d, 6 * d, L - 6 * d, 6 * d, L - 6 * d), waveform=w,
+                     component="Ey", z0=5 * d)
+    dt = 0.99 / C0 / np.sqrt(3 / d ** 2)
+    model = Model(grid, src, tmax=(steps - 1) * dt, CN=0.99, pml=4 * d, material=[sub, metal])
+    return model
+
+
+def nonlinear_3d(n=24, steps=150, I0=1e17, rate="power"):
+    """Kerr + plasma medium under an intense TFSF pulse (800 nm)."""
+    d = 40e-9
+    L = (n - 1) * d
+    grid = Grid(xmin=0, xmax=L, Nx=n, ymin=0, ymax=L, Ny=n, zmin=0, zmax=L, Nz=n)
+    n0, n2 = 1.45, 2.5e-20
+    chi3 = 4 / 3 * n0 ** 2 * EPS0 * C0 * n2
+    w0 = 2 * np.pi * C0 / 800e-9
+    if rate == "power":
+        ionrate = PowerLawRate(sigma=1e-90, K=6)
+    else:
+        x = np.linspace(13, 19, 61)
+        ionrate = IonizationRate(x=x, y=6 * x - 90.0 + 0.01 * np.sin(x))
+    plasma = Plasma(ionrate=ionrate, rho0=2.1e28, nuc=1e15, frequency=w0, Uiev=9.0, mr=0.64)
+    med = Material(geometry=lambda x, y, z: (z > 0.35 * L) + 0 * x + 0 * y > 0, eps=n0 ** 2,
+                   chi3=chi3, plasma=plasma)
+    E0 = np.sqrt(I0 / (n0 * EPS0 * C0 / 2))
+    tau0 = 4e-15
+    w = lambda t: E0 * np.exp(-(t - 10e-15) ** 2 / tau0 ** 2) * np.cos(w0 * (t - 10e-15))
+    src = TFSFSource(box=(6 * d, L - 6 * d, 6 * d, L - 6 * d, 6 * d, L - 6 * d), waveform=w,
+                     component="Ex", z0=5 * d)
+    dt = 0.99 / C0 / np.sqrt(3 / d ** 2)
+    model = Model(grid, src, tmax=(steps - 1) * dt, CN=0.99, pml=4 * d, material=med)
+    return model
+
+
+def lorentz_2d(nx=96, nz=80, steps=200, seed=3, tfsf=True):
+    """2D (x, z) Lorentz rods and a dielectric slab, TFSF plane wave + soft point source."""
+    rng = np.random.default_rng(seed)
+    d = 40e-9
+    grid = Grid(xmin=0, xmax=(nx - 1) * d, Nx=nx, zmin=0, zmax=(nz - 1) * d, Nz=nz)
+    Lx, Lz = (nx - 1) * d, (nz - 1) * d
+    mats = [Material(geometry=lambda x, z: (z > 0.6 * Lz) & (z < 0.7 * Lz),
+                     eps=float(rng.uniform(1, 12)))]
+    for _ in range(3):
+        c = rng.uniform(0.3, 0.7, 2) * np.array([Lx, Lz])
+        mats.append(Material(geometry=lambda x, z, c=c: (x - c[0]) ** 2 + (z - c[1]) ** 2 <= (4 * d) ** 2,
+                             eps=float(rng.uniform(1, 4)),
+                             chi=LorentzSusceptibility(deps=float(rng.uniform(0.5, 3)),
+                                                       w0=2 * np.pi * C0 / 600e-9, delta=1e14)))
+    w = gauss_waveform(lam0=800e-9, tau0=5e-15, dt0=12e-15)
+    srcs = [SoftSource(geometry=(0.3 * Lx, 0.4 * Lz), waveform=lambda x, z, t: 100 * w(t) + 0 * x,
+                       component="Ez")]
+    if tfsf:
+        srcs.append(TFSFSource(box=(8 * d, Lx - 8 * d, 8 * d, Lz - 8 * d), waveform=w,
+                               component="Ex", z0=6 * d))
+    dt = 0.99 / C0 / np.sqrt(2 / d ** 2)
+    model = Model(grid, srcs, tmax=(steps - 1) * dt, CN=0.99, pml=5 * d, material=mats)
+    w0 = 2 * np.pi * C0 / 800e-9
+    mons = [FieldMonitor((0.5 * Lx, 0.8 * Lz)),
+            SpectralMonitor(lambda x, z: (np.abs(z - 0.8 * Lz) < d / 2) & (x > 0.2 * Lx) & (x < 0.8 * Lx),
+                            w=w0 * np.linspace(0.8, 1.2, 7))]
+    return model, mons
+
+
+def slab_1d(nz=600, steps=900, dispersive=True):
+    """1D: CPML, a Lorentz + conductive slab and a soft source, plus monitors."""
+    d = 40e-9
+    grid = Grid(zmin=0, zmax=(nz - 1) * d, Nz=nz)
+    L = (nz - 1) * d
+    chi = LorentzSusceptibility(deps=1.5, w0=2 * np.pi * C0 / 600e-9, delta=3e13) if dispersive else None
+    mats = [Material(geometry=lambda z: (z > 0.5 * L) & (z < 0.65 * L), eps=2.25, sigma=50.0, chi=chi)]
+    w = gauss_waveform(lam0=800e-9, tau0=10e-15, dt0=30e-15)
+    src = SoftSource(geometry=lambda z: np.abs(z - 0.125 * L) < d / 2, waveform=lambda z, t: w(t) + 0 * z,
+                     component="Ex")
+    dt = 0.99 / C0 * d
+    model = Model(grid, src, tmax=(steps - 1) * dt, CN=0.99, pml=10 * d, material=mats)
+    w0 = 2 * np.pi * C0 / 800e-9
+    mons = [FieldMonitor((0.875 * L,)),
+            SpectralMonitor((0.875 * L,), w=2 * np.pi * C0 / np.linspace(600e-9, 1000e-9, 32))]
+    return model, mons

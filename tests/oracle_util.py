@@ -1,0 +1,66 @@
+"""Test helpers: bind the oracle (test infrastructure) through the same ABI call
+sequence the engine receives."""
+import os
+import subprocess
+
+import numpy as np
+
+from maxwellfdtd_amd import _lib
+from maxwellfdtd_amd.models import Handle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+
+_ORACLES = {}
+
+
+def oracle(precision="f64"):
+    """ctypes binding of oracle/liboracle_<precision>.so (built on demand with make)."""
+    if precision not in _ORACLES:
+        path = os.path.join(ORACLE_DIR, f"liboracle_{precision}.so")
+        src = os.path.join(ORACLE_DIR, "fdtd_oracle.c")
+        if not os.path.exists(path) or os.path.getmtime(path) < os.path.getmtime(src):
+            subprocess.run(["make", "-C", ORACLE_DIR, "-s"], check=True)
+        import ctypes as C
+        extra = {"set_threads": (C.c_int32, [C.c_int32]), "real_size": (C.c_int32, [])}
+        _ORACLES[precision] = _lib.Binding(path, "ofd_", extra)
+    return _ORACLES[precision]
+
+
+def run_oracle(model, monitors=None, integrals=None, precision="f64", steps=None):
+    """Run `model` on the oracle; returns the Handle (caller reads fields)."""
+    desc = model.describe(monitors=monitors, integrals=integrals)
+    h = Handle(desc, oracle(precision))
+    h.step(model.Nt if steps is None else steps)
+    return h
+
+
+def rel_l2(a, b):
+    a = np.asarray(a).ravel()
+    b = np.asarray(b).ravel()
+    if not (np.iscomplexobj(a) or np.iscomplexobj(b)):
+        a = a.astype(float)
+        b = b.astype(float)
+    den = max(np.linalg.norm(a), np.linalg.norm(b))
+    return 0.0 if den == 0 else float(np.linalg.norm(a - b) / den)
+
+
+def isapprox(a, b, rtol):
+    """Julia isapprox for arrays: norm(a-b) <= rtol*max(norm(a), norm(b))."""
+    return rel_l2(a, b) <= rtol
+
+
+FIELD_GROUPS = {1: {"H": ("Hy",), "D": ("Dx",), "E": ("Ex",)},
+                2: {"H": ("Hy",), "D": ("Dx", "Dz"), "E": ("Ex", "Ez")},
+                3: {"H": ("Hx", "Hy", "Hz"), "D": ("Dx", "Dy", "Dz"), "E": ("Ex", "Ey", "Ez")}}
+
+
+def field_errors(ha, hb, ndims):
+    """Relative L2 error of the vector fields H, D, E (all components together) of two
+    handles: ||a - b|| / max(||a||, ||b||)."""
+    out = {}
+    for g, comps in FIELD_GROUPS[ndims].items():
+        a = np.concatenate([ha.field(c) for c in comps])
+        b = np.concatenate([hb.field(c) for c in comps])
+        out[g] = rel_l2(a, b)
+    return out

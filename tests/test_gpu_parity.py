@@ -1,0 +1,158 @@
+"""GPU parity: the MI355X engine against the FP64 oracle on identical inputs, through the
+same C-ABI call sequence.  Bar: relative L2 <= 1e-4 on the H, D and E vector fields, on
+monitors, on DFT spectra and on integral outputs (the FP32-vs-FP64 tolerance of the
+per-step update), plus the reference's own analytic checks."""
+import numpy as np
+import pytest
+
+import cases
+from maxwellfdtd_amd import _lib, solve
+from maxwellfdtd_amd.models import Handle, MONITOR_COMPONENTS
+from oracle_util import field_errors, oracle, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+def run_pair(model, mons=None, integrals=None, steps=None, options=None):
+    desc = model.describe(monitors=mons, integrals=integrals)
+    n = model.Nt if steps is None else steps
+    hg = Handle(desc, _lib.engine())
+    for k, v in (options or {}).items():
+        hg.lib.check(hg.lib.model_set_options(hg.h, k, v), "set_options")
+    hg.step(n)
+    ho = Handle(desc, oracle("f64"))
+    ho.step(n)
+    return hg, ho, desc
+
+
+def assert_parity(hg, ho, desc, model, tol=TOL):
+    errs = field_errors(hg, ho, model.grid.ndims)
+    assert max(errs.values()) <= tol, errs
+    if desc.integrals & 1:
+        assert rel_l2(hg.output(0), ho.output(0)) <= tol
+    if desc.integrals & 2:
+        assert rel_l2(hg.output(1), ho.output(1)) <= tol
+    if model.isplasma:
+        assert rel_l2(hg.output(2), ho.output(2)) <= tol * 10, "JE"
+        assert rel_l2(hg.output(3), ho.output(3)) <= tol, "rho"
+    for i, mon in enumerate(desc.monitors):
+        names = MONITOR_COMPONENTS[model.grid.ndims]
+        a = np.concatenate([np.ravel(hg.monitor(i, c)) for c in names])
+        b = np.concatenate([np.ravel(ho.monitor(i, c)) for c in names])
+        assert rel_l2(a, b) <= tol, (i, mon["kind"], rel_l2(a, b))
+    return errs
+
+
+def test_engine_loads():
+    lib = _lib.engine()
+    assert lib.version() == 1
+
+
+def test_ref_1d_propagation():
+    model, Eth, rtol = cases.ref_1d_test()
+    hg, ho, desc = run_pair(model)
+    assert rel_l2(hg.field("Ex"), Eth) <= rtol
+    assert_parity(hg, ho, desc, model)
+
+
+def test_ref_2d_propagation():
+    model, Eth, rtol = cases.ref_2d_test()
+    hg, ho, desc = run_pair(model)
+    assert rel_l2(hg.field("Ex"), Eth.ravel(order="F")) <= rtol
+    assert_parity(hg, ho, desc, model)
+
+
+def test_ref_3d_propagation():
+    model, Eth, rtol = cases.ref_3d_test()
+    hg, ho, desc = run_pair(model)
+    assert rel_l2(hg.field("Ex"), Eth.ravel(order="F")) <= rtol
+    assert_parity(hg, ho, desc, model)
+
+
+@pytest.mark.parametrize("ndims", [1, 2, 3])
+@pytest.mark.parametrize("kind", ["sigma", "lorentz"])
+def test_ref_losses(ndims, kind):
+    model, ki = cases.ref_losses(ndims, kind)
+    sol = solve(model)
+    assert cases.beer_lambert_ok(sol.Sa, model.grid.z, ki)
+
+
+def test_spheres_3d_all_features():
+    model, mons = cases.spheres_3d()
+    hg, ho, desc = run_pair(model, mons)
+    assert_parity(hg, ho, desc, model)
+
+
+def test_spheres_3d_boundary_conditions():
+    bc = ("dirichlet", "neumann", "neumann", "periodic", "periodic", "dirichlet")
+    model, mons = cases.spheres_3d(n=24, steps=80, bc=bc, monitors=False, pml=0)
+    hg, ho, desc = run_pair(model)
+    assert_parity(hg, ho, desc, model)
+
+
+def test_dispersive_3d():
+    model = cases.dispersive_3d()
+    hg, ho, desc = run_pair(model)
+    assert_parity(hg, ho, desc, model)
+
+
+@pytest.mark.parametrize("rate", ["power", "table"])
+def test_nonlinear_3d(rate):
+    model = cases.nonlinear_3d(rate=rate)
+    hg, ho, desc = run_pair(model)
+    assert_parity(hg, ho, desc, model)
+
+
+def test_lorentz_2d():
+    model, mons = cases.lorentz_2d()
+    hg, ho, desc = run_pair(model, mons)
+    assert_parity(hg, ho, desc, model)
+
+
+def test_slab_1d():
+    model, mons = cases.slab_1d()
+    hg, ho, desc = run_pair(model, mons)
+    assert_parity(hg, ho, desc, model)
+
+
+def test_graph_and_eager_agree_bitwise():
+    model, mons = cases.spheres_3d(n=24, steps=70)
+    desc = model.describe(monitors=mons)
+    outs = []
+    for g in (0, 16):
+        h = Handle(desc, _lib.engine())
+        h.lib.check(h.lib.model_set_options(h.h, 0, g), "set_options")
+        h.step(model.Nt)
+        outs.append([h.field(c) for c in ("Hx", "Ey", "Dz")] + [np.ravel(h.monitor(2, "Ex"))])
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
+def test_chunked_stepping_is_bitwise_identical():
+    model, mons = cases.spheres_3d(n=24, steps=90)
+    desc = model.describe(monitors=mons)
+    h1 = Handle(desc, _lib.engine())
+    h1.step(model.Nt)
+    h2 = Handle(desc, _lib.engine())
+    for n in (1, 7, 33, 2, 19, 28):
+        h2.step(n)
+        h2.monitor(2, "Ex")      # forces a partial DFT flush mid-run
+    for c in ("Hy", "Ez", "Dx"):
+        assert np.array_equal(h1.field(c), h2.field(c))
+    for i in range(4):
+        for c in ("Ex", "Hz"):
+            a, b = h1.monitor(i, c), h2.monitor(i, c)
+            assert rel_l2(a, b) <= 1e-12
+
+
+def test_nonfinite_is_reported():
+    from maxwellfdtd_amd import Grid, Model, SoftSource
+    grid = Grid(zmin=0, zmax=1e-6, Nz=101)
+    src = SoftSource(geometry=(0.5e-6,), waveform=lambda z, t: 1.0 + 0 * t, component="Ex")
+    model = Model(grid, src, tmax=4000 * 1.5 * grid.dz / 3e8, CN=1.5, pml=0)
+    desc = model.describe()
+    h = Handle(desc, _lib.engine())
+    with pytest.raises(_lib.EngineError, match="I found (NaN|Inf) field values"):
+        h.step(model.Nt)
