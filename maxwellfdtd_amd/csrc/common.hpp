@@ -11,6 +11,17 @@
 
 namespace mfd {
 
+// Global-address-space views of device pointers.  Pointers that kernels read out of the
+// DevModel structs are generic (flat) to the compiler; casting them to address space 1
+// makes every access a global_load/global_store instead of a flat access (which would
+// also count against the scalar-load counter and serialise with it).
+template <class T>
+using gp = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gp<T> G(T *p) { return (gp<T>)p; }
+template <class T>
+__device__ __forceinline__ gp<const T> G(const T *p) { return (gp<const T>)p; }
+
 enum { HX = 0, HY, HZ, DX, DY, DZ, EX, EY, EZ };
 enum { BC_PERIODIC = 1, BC_DIRICHLET = 2, BC_NEUMANN = 3 };
 enum { MAT_DISP = 1, MAT_KERR = 2, MAT_PLASMA = 4 };
@@ -97,9 +108,10 @@ struct DevModel {
     int Nt;
     int int_flags;              // 1 Sa, 2 E2
     float *F[9];
-    const unsigned char *id;    // material id per cell, nullptr if no material
+    const unsigned char *id;    // material id per cell (bit 7: cell written by a source)
     const int *slot;            // state slot per cell, nullptr if no stateful material
     const DevMat *mats;         // [nmat + 1]
+    int nmat;
     DevLayer L[6];
     float *Sa, *E2;
     int *it;                    // device step counter: completed steps
@@ -107,9 +119,14 @@ struct DevModel {
     int nsrc, nmon;
     const DevSource *src;
     const DevMon *mon;
-    int dft_k;                  // samples buffered in the current DFT time block
     int dft_block;              // DFT time-block length
+    const int *touched;         // cells written by a source (integral fixup list)
+    int ntouched;
 };
+
+constexpr unsigned char ID_MASK = 0x7f, ID_TOUCHED = 0x80;
+// E-kernel material modes: vacuum only, coefficient table only, full (ADE/Kerr/plasma)
+enum { MODE_VAC = 0, MODE_SIMPLE = 1, MODE_FULL = 2 };
 
 // Tile of the work-list: model index, x0, y0, (z0 | z1 << 16).
 typedef int4 Tile;
@@ -118,22 +135,27 @@ __host__ __device__ inline int tile_z0(const Tile &t) { return t.w & 0xffff; }
 __host__ __device__ inline int tile_z1(const Tile &t) { return (t.w >> 16) & 0xffff; }
 
 // tile geometry per dimensionality
-constexpr int TX3 = 64, TY3 = 4;    // 3D: 64 x 4 columns, marching z
+constexpr int TX3 = 64, TY3 = 4;    // 3D scalar kernels: 64 x 4 columns, marching z
+constexpr int TX3V = 256;           // 3D vector kernels: 256 x (4 per lane) x TY rows
 constexpr int TX2 = 256;            // 2D: 256 x-columns, marching z
 constexpr int TX1 = 256;            // 1D: 256 cells
 
 // launchers (defined in the .hip files)
-void launch_h3d(const DevModel *m, const Tile *tiles, int ntiles, int grid, bool mats, hipStream_t s);
-void launch_e3d(const DevModel *m, const Tile *tiles, int ntiles, int grid, bool mats, hipStream_t s);
-void launch_h2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, bool mats, hipStream_t s);
-void launch_e2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, bool mats, hipStream_t s);
-void launch_h1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, bool mats, hipStream_t s);
-void launch_e1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, bool mats, hipStream_t s);
+void launch_h3d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, hipStream_t s);
+void launch_e3d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, hipStream_t s);
+void launch_h3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int ty, hipStream_t s);
+void launch_e3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int ty, hipStream_t s);
+void launch_h3d_v4(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int ty, hipStream_t s);
+void launch_e3d_v4(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int ty, hipStream_t s);
+void launch_h2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, hipStream_t s);
+void launch_e2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, hipStream_t s);
+void launch_h1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, hipStream_t s);
+void launch_e1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, hipStream_t s);
+void launch_integrals_touched(const DevModel *m, int nmodels, int maxn, int ndims, hipStream_t s);
 void launch_point_sources(const DevModel *m, int nmodels, int slot, int maxn, hipStream_t s);
 void launch_tfsf(const DevModel *m, int nmodels, int slot, int ndims, int maxface, hipStream_t s);
 void launch_monitors(const DevModel *m, int nmodels, int maxn, hipStream_t s);
 void launch_dft_flush(const DevModel *m, int nmodels, int maxn, int nk, hipStream_t s);
-void launch_integrals(const DevModel *m, int nmodels, int grid, int ndims, hipStream_t s);
 void launch_finish(const DevModel *m, int nmodels, hipStream_t s);
 
 }  // namespace mfd

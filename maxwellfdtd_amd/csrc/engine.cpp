@@ -96,6 +96,49 @@ static std::vector<float> to_f32(const std::vector<double> &v) {
     return r;
 }
 
+// 0-based face box of the 3D face scheme; 1D and 2D boxes are degenerate in the absent
+// tangential axes (see the TFSF kernels).
+static void tfsf_face_box(const mfd_model_desc &d, const int *box, int *b) {
+    if (d.ndims == 3) {
+        for (int i = 0; i < 6; i++) b[i] = box[i] - 1;
+    } else if (d.ndims == 2) {
+        b[0] = box[0] - 1; b[1] = box[1] - 1; b[2] = 0; b[3] = 0;
+        b[4] = box[4] - 1; b[5] = box[5] - 1;
+    } else {
+        b[0] = 0; b[1] = 1; b[2] = 0; b[3] = 0;
+        b[4] = box[4] - 1; b[5] = box[5] - 1;
+    }
+}
+
+// Cells a source writes after the E half-step: their integral variables are accumulated
+// after all sources (the E kernels skip them).
+static void mark_touched(const Model &m, std::vector<unsigned char> &mark) {
+    const auto &d = m.d;
+    for (const auto &s : m.srcs) {
+        if (s.type == 0) {
+            for (int c : s.cells) mark[c] = 1;
+            continue;
+        }
+        int b[6];
+        tfsf_face_box(d, s.box, b);
+        for (int axis = 0; axis < 3; axis++) {
+            if ((d.ndims == 1 && axis < 2) || (d.ndims == 2 && axis == 1)) continue;
+            const int ua = axis == 0 ? 1 : 0, va = axis == 2 ? 1 : 2;
+            const int planes[4] = {b[2 * axis] - 1, b[2 * axis], b[2 * axis + 1], b[2 * axis + 1]};
+            for (int n : planes)
+                for (int u = b[2 * ua]; u <= b[2 * ua + 1]; u++)
+                    for (int v = b[2 * va]; v <= b[2 * va + 1]; v++) {
+                        int c3[3];
+                        c3[axis] = n; c3[ua] = u; c3[va] = v;
+                        if (c3[0] < 0 || c3[0] >= d.Nx || c3[1] < 0 || c3[1] >= d.Ny ||
+                            c3[2] < 0 || c3[2] >= d.Nz)
+                            continue;
+                        mark[c3[0] + (long long)d.Nx * (c3[1] + (long long)d.Ny * c3[2])] = 1;
+                    }
+        }
+    }
+}
+
 #define UP(call)                                                                       \
     do {                                                                               \
         hipError_t e_ = (call);                                                        \
@@ -122,11 +165,25 @@ int Model::upload() {
     for (int i = 0; i < 6; i++) m.bcs[i] = d.bcs[i];
     m.Nt = d.Nt;
     m.int_flags = int_flags;
-    for (int c = 0; c < 9; c++) {
-        m.F[c] = nullptr;
-        if (!comp_present(c)) continue;
-        UP(arena.alloc(&m.F[c], N));
-        UP(hipMemset(m.F[c], 0, N * sizeof(float)));
+    // per-cell FP32 arrays (fields, Sa, E2) come from one pool; array k starts k*stagger
+    // bytes past a 256-byte boundary so that equal cell indices of different arrays do
+    // not all land on the same HBM channel and bank
+    {
+        float **slots[11];
+        int nslot = 0;
+        for (int c = 0; c < 9; c++) {
+            m.F[c] = nullptr;
+            if (comp_present(c)) slots[nslot++] = &m.F[c];
+        }
+        if (int_flags & MFD_INT_SA) slots[nslot++] = &m.Sa;
+        if (int_flags & MFD_INT_E2) slots[nslot++] = &m.E2;
+        const size_t arr = ((size_t)N * sizeof(float) + 255) / 256 * 256;
+        const size_t st = ((size_t)std::max(0, stagger) + 255) / 256 * 256;
+        const size_t total = nslot * arr + (size_t)nslot * st;
+        char *pool;
+        UP(arena.alloc(&pool, total));
+        UP(hipMemset(pool, 0, total));
+        for (int k = 0; k < nslot; k++) *slots[k] = (float *)(pool + k * (arr + st));
     }
     // materials: id per cell (uint8) and compact state slots
     const int nmat = (int)mats.size();
@@ -145,15 +202,23 @@ int Model::upload() {
         // mixing models with and without materials can read it for every model
         std::vector<unsigned char> ids(N);
         std::vector<int> slot(stateful ? N : 0);
+        std::vector<unsigned char> mark(N, 0);
+        mark_touched(*this, mark);
+        touched.clear();
         for (long long i = 0; i < N; i++) {
             const int g = geom[i];
-            ids[i] = (unsigned char)g;
+            ids[i] = (unsigned char)(g | (mark[i] ? ID_TOUCHED : 0));
+            if (mark[i]) touched.push_back((int)i);
             if (stateful) slot[i] = (int)cnt[g];
             cnt[g]++;
         }
         unsigned char *did;
         UP(upload_vec(arena, &did, ids));
         m.id = did;
+        int *dt_;
+        UP(upload_vec(arena, &dt_, touched));
+        m.touched = dt_;
+        m.ntouched = (int)touched.size();
         if (stateful) {
             int *ds;
             UP(upload_vec(arena, &ds, slot));
@@ -209,6 +274,7 @@ int Model::upload() {
     DevMat *dmats;
     UP(upload_vec(arena, &dmats, hmats));
     m.mats = dmats;
+    m.nmat = nmat;
     // CPML layers
     for (int f = 0; f < 6; f++) {
         DevLayer &l = m.L[f];
@@ -253,15 +319,7 @@ int Model::upload() {
         v.mode = hs.mode == MFD_TFSF_AUX1D ? 1 : 0;
         // face scheme of the 3D replay; 1D and 2D degenerate tangential ranges
         int b[6];
-        if (d.ndims == 3) {
-            for (int i = 0; i < 6; i++) b[i] = hs.box[i] - 1;
-        } else if (d.ndims == 2) {
-            b[0] = hs.box[0] - 1; b[1] = hs.box[1] - 1; b[2] = 0; b[3] = 0;
-            b[4] = hs.box[4] - 1; b[5] = hs.box[5] - 1;
-        } else {
-            b[0] = 0; b[1] = 1; b[2] = 0; b[3] = 0;
-            b[4] = hs.box[4] - 1; b[5] = hs.box[5] - 1;
-        }
+        tfsf_face_box(d, hs.box, b);
         for (int i = 0; i < 6; i++) v.box[i] = b[i];
         for (int f = 0; f < 6; f++) {
             const int axis = f >> 1;
@@ -373,14 +431,7 @@ int Model::upload() {
     m.mon = dmon;
     m.nmon = (int)mons.size();
     m.dft_block = dft_block;
-    if (int_flags & MFD_INT_SA) {
-        UP(arena.alloc(&m.Sa, N));
-        UP(hipMemset(m.Sa, 0, N * sizeof(float)));
-    }
-    if (int_flags & MFD_INT_E2) {
-        UP(arena.alloc(&m.E2, N));
-        UP(hipMemset(m.E2, 0, N * sizeof(float)));
-    }
+
     int *ctr;
     UP(arena.alloc(&ctr, 2));
     UP(hipMemset(ctr, 0, 2 * sizeof(int)));
@@ -434,9 +485,15 @@ void Runner::release() {
     ev.clear();
 }
 
-static void make_tiles(const Model &m, int idx, int zchunk, std::vector<Tile> &out) {
+static void make_tiles(const Model &m, int idx, int zchunk, int ty, int kform, std::vector<Tile> &out) {
     const int Nx = m.d.Nx, Ny = m.d.Ny, Nz = m.d.Nz;
-    if (m.d.ndims == 3) {
+    if (m.d.ndims == 3 && kform > 0) {
+        const int tx = kform == 1 ? TX3V : 64;
+        for (int z0 = 0; z0 < Nz; z0 += zchunk)
+            for (int y0 = 0; y0 < Ny; y0 += ty)
+                for (int x0 = 0; x0 < Nx; x0 += tx)
+                    out.push_back(make_int4(idx, x0, y0, z0 | (std::min(Nz, z0 + zchunk) << 16)));
+    } else if (m.d.ndims == 3) {
         for (int z0 = 0; z0 < Nz; z0 += zchunk)
             for (int y0 = 0; y0 < Ny; y0 += TY3)
                 for (int x0 = 0; x0 < Nx; x0 += TX3)
@@ -463,19 +520,28 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     MFD_HIP(hipDeviceGetAttribute(&cu_count, hipDeviceAttributeMultiprocessorCount, dev));
     std::vector<DevModel> hd;
     std::vector<Tile> tiles;
-    any_mats = any_int = any_mon = any_spectral = false;
+    any_mats = any_full = any_int = any_mon = any_spectral = false;
+    maxtouched = 1;
     maxsrc = 0;
     maxsrc_n = maxmon_n = maxface = 1;
     dft_block = ms[0]->dft_block;
     long long maxN = 1;
+    kform = ndims == 3 ? opt.vec : 0;
+    for (Model *m : ms) {
+        if (kform == 1 && m->d.Nx % 4 != 0) kform = 2;
+        if (kform == 2 && m->N >= (1LL << 30)) kform = 0;   // 32-bit cell offsets
+    }
     for (size_t k = 0; k < ms.size(); k++) {
         Model *m = ms[k];
         if (m->d.ndims != ndims) { set_error("a run mixes 1D/2D/3D models"); return MFD_ERR_ARG; }
         int st = m->upload();
         if (st != MFD_OK) return st;
         hd.push_back(m->dm);
-        make_tiles(*m, (int)k, opt.zchunk, tiles);
+        make_tiles(*m, (int)k, opt.zchunk, opt.ty, kform, tiles);
         any_mats |= m->has_mats;
+        for (const auto &hm : m->mats)
+            any_full |= hm.d.nq > 0 || hm.d.iskerr || hm.d.isplasma;
+        maxtouched = std::max(maxtouched, (int)m->touched.size());
         any_int |= m->int_flags != 0;
         any_mon |= !m->mons.empty();
         any_spectral |= m->has_spectral;
@@ -493,9 +559,19 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     int_grid = (int)std::min<long long>((maxN + 255) / 256, (long long)cu_count * 16);
     MFD_HIP(hipMalloc(&d_models, hd.size() * sizeof(DevModel)));
     MFD_HIP(hipMemcpy(d_models, hd.data(), hd.size() * sizeof(DevModel), hipMemcpyHostToDevice));
-    ntiles = (int)tiles.size();
-    MFD_HIP(hipMalloc(&d_tiles, tiles.size() * sizeof(Tile)));
-    MFD_HIP(hipMemcpy(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs, so block b
+    // runs on XCD b % 8.  Give every XCD one contiguous chunk of the tile list, so tiles
+    // that share halo rows and planes share that XCD's L2.
+    const int nxcd = 8;
+    const int per = ((int)tiles.size() + nxcd - 1) / nxcd;
+    std::vector<Tile> perm((size_t)per * nxcd, make_int4(-1, 0, 0, 0));
+    for (int b = 0; b < per * nxcd; b++) {
+        const int src = (b % nxcd) * per + b / nxcd;
+        if (src < (int)tiles.size()) perm[b] = tiles[src];
+    }
+    ntiles = (int)perm.size();
+    MFD_HIP(hipMalloc(&d_tiles, perm.size() * sizeof(Tile)));
+    MFD_HIP(hipMemcpy(d_tiles, perm.data(), perm.size() * sizeof(Tile), hipMemcpyHostToDevice));
     return MFD_OK;
 }
 
@@ -508,20 +584,25 @@ int Runner::launch_step(bool timing) {
         for (auto &e : ev) MFD_HIP(hipEventCreate(&e));
     }
     if (timing) MFD_HIP(hipEventRecord(ev[0], stream));
-    if (ndims == 3) launch_h3d(d_models, d_tiles, ntiles, grid, any_mats, stream);
-    else if (ndims == 2) launch_h2d(d_models, d_tiles, ntiles, grid, any_mats, stream);
-    else launch_h1d(d_models, d_tiles, ntiles, grid, any_mats, stream);
+    const int mode = any_full ? MODE_FULL : any_mats ? MODE_SIMPLE : MODE_VAC;
+    if (ndims == 3 && kform == 1) launch_h3d_v4(d_models, d_tiles, ntiles, grid, mode, opt.ty, stream);
+    else if (ndims == 3 && kform == 2) launch_h3d_p(d_models, d_tiles, ntiles, grid, mode, opt.ty, stream);
+    else if (ndims == 3) launch_h3d(d_models, d_tiles, ntiles, grid, mode, stream);
+    else if (ndims == 2) launch_h2d(d_models, d_tiles, ntiles, grid, mode, stream);
+    else launch_h1d(d_models, d_tiles, ntiles, grid, mode, stream);
     if (timing) { MFD_HIP(hipEventRecord(ev[1], stream)); MFD_HIP(hipEventRecord(ev[2], stream)); }
-    if (ndims == 3) launch_e3d(d_models, d_tiles, ntiles, grid, any_mats, stream);
-    else if (ndims == 2) launch_e2d(d_models, d_tiles, ntiles, grid, any_mats, stream);
-    else launch_e1d(d_models, d_tiles, ntiles, grid, any_mats, stream);
+    if (ndims == 3 && kform == 1) launch_e3d_v4(d_models, d_tiles, ntiles, grid, mode, any_int, opt.ty, stream);
+    else if (ndims == 3 && kform == 2) launch_e3d_p(d_models, d_tiles, ntiles, grid, mode, any_int, opt.ty, stream);
+    else if (ndims == 3) launch_e3d(d_models, d_tiles, ntiles, grid, mode, any_int, stream);
+    else if (ndims == 2) launch_e2d(d_models, d_tiles, ntiles, grid, mode, any_int, stream);
+    else launch_e1d(d_models, d_tiles, ntiles, grid, mode, any_int, stream);
     if (timing) MFD_HIP(hipEventRecord(ev[3], stream));
     for (int s = 0; s < maxsrc; s++) {
         if (src_kind_mask[s] & 1) launch_point_sources(d_models, nm, s, maxsrc_n, stream);
         if (src_kind_mask[s] & 2) launch_tfsf(d_models, nm, s, ndims, maxface, stream);
     }
     if (any_mon) launch_monitors(d_models, nm, maxmon_n, stream);
-    if (any_int) launch_integrals(d_models, nm, int_grid, ndims, stream);
+    if (any_int) launch_integrals_touched(d_models, nm, maxtouched, ndims, stream);
     launch_finish(d_models, nm, stream);
     MFD_HIP(hipGetLastError());
     if (timing) {
@@ -720,7 +801,7 @@ int32_t mfd_model_set_geometry(mfd_model *h, const int32_t *ids) {
     NEED_HOST(h);
     h->m.has_geom = false;
     for (long long i = 0; i < h->m.N; i++) {
-        if (ids[i] < 0 || ids[i] > 255) { set_error("geometry ids must be in 0..255"); return MFD_ERR_ARG; }
+        if (ids[i] < 0 || ids[i] > 127) { set_error("geometry ids must be in 0..127"); return MFD_ERR_ARG; }
         h->m.geom[i] = ids[i];
         if (ids[i] > 0) h->m.has_geom = true;
     }
@@ -729,7 +810,8 @@ int32_t mfd_model_set_geometry(mfd_model *h, const int32_t *ids) {
 
 int32_t mfd_model_add_material(mfd_model *h, const mfd_material_desc *mat) {
     NEED_HOST(h);
-    if (!mat || mat->nq < 0 || h->m.mats.size() >= 255) { set_error("bad material"); return MFD_ERR_ARG; }
+    if (!mat || mat->nq < 0) { set_error("bad material"); return MFD_ERR_ARG; }
+    if (h->m.mats.size() >= 127) { set_error("at most 127 materials per model"); return MFD_ERR_ARG; }
     HostMat hm;
     hm.d = *mat;
     hm.Aq.assign(mat->Aq, mat->Aq + mat->nq);
@@ -1033,7 +1115,9 @@ int32_t mfd_model_destroy(mfd_model *h) {
 
 // Engine options (not part of the reference interface): 0 graph steps per replay,
 // 1 z planes per tile, 2 persistent workgroups per CU (0 = one per tile),
-// 3 kernel timing (hipEvent pairs around the H/E launches), 4 DFT time-block length.
+// 3 kernel timing (hipEvent pairs around the H/E launches), 4 DFT time-block length,
+// 5 rows per workgroup of the 3D tiled kernels, 6 3D kernel form (0 scalar, 1 vector,
+// 2 pipelined), 7 per-array stagger of the field pool (bytes).
 int32_t mfd_model_set_options(mfd_model *h, int32_t opt, int32_t value) {
     if (!h) { set_error("null model"); return MFD_ERR_ARG; }
     switch (opt) {
@@ -1047,6 +1131,18 @@ int32_t mfd_model_set_options(mfd_model *h, int32_t opt, int32_t value) {
         case 4:
             if (h->m.uploaded || value < 1 || value > 64) { set_error("DFT block must be 1..64, set before upload"); return MFD_ERR_ARG; }
             h->m.dft_block = value;
+            break;
+        case 5:
+            if (value != 1 && value != 2 && value != 4 && value != 8) { set_error("rows per workgroup must be 1, 2, 4 or 8"); return MFD_ERR_ARG; }
+            h->opt.ty = value;
+            break;
+        case 6:
+            if (value < 0 || value > 2) { set_error("3D kernel form must be 0, 1 or 2"); return MFD_ERR_ARG; }
+            h->opt.vec = value;
+            break;
+        case 7:
+            if (h->m.uploaded || value < 0) { set_error("stagger must be >= 0, set before upload"); return MFD_ERR_ARG; }
+            h->m.stagger = value;
             break;
         default: set_error("unknown option"); return MFD_ERR_ARG;
     }

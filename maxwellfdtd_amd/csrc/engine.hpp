@@ -96,8 +96,10 @@ struct Model {
     int it = 0;                 // completed steps (host mirror of *dm.it)
     int dft_done = 0;           // steps already contracted into the DFT accumulators
     int dft_block = 32;
+    int stagger = 0;            // byte offset added per per-cell array in the field pool
     int ncomp_mon = 0;
     int maxsrc_n = 0, maxmon_n = 0, maxface = 0;
+    std::vector<int> touched;   // cells written by sources (sorted)
     bool has_mats = false, has_tfsf = false, has_spectral = false;
 
     int ncomp() const { return d.ndims == 3 ? 3 : d.ndims == 2 ? 2 : 1; }
@@ -110,7 +112,9 @@ struct Model {
 
 struct RunOptions {
     int graph_steps = 16;       // steps captured per hipGraph replay (0 = eager launches)
-    int zchunk = 32;            // z planes per tile (3D/2D)
+    int zchunk = 16;            // z planes per tile (3D/2D)
+    int ty = 4;                 // rows per workgroup of the 3D vector kernels (1, 2, 4, 8)
+    int vec = 2;                // 3D kernel form: 0 scalar, 1 vector (Nx % 4 == 0), 2 pipelined
     int persistent = 0;         // 0: one workgroup per tile; >0: that many workgroups per CU
     bool kernel_timing = false; // hipEvent pairs around the H/E launches (eager mode)
 };
@@ -122,7 +126,9 @@ struct Runner {
     Tile *d_tiles = nullptr;
     int ntiles = 0;
     int ndims = 3;
-    bool any_mats = false, any_int = false, any_mon = false, any_spectral = false;
+    bool any_mats = false, any_full = false, any_int = false, any_mon = false, any_spectral = false;
+    int maxtouched = 1;
+    int kform = 0;              // 3D kernel form in use
     int maxsrc = 0;
     std::vector<int> src_kind_mask;   // per slot: 1 point, 2 tfsf
     int maxsrc_n = 1, maxmon_n = 1, maxface = 1, int_grid = 1;

@@ -6,11 +6,12 @@
 
 namespace mfd {
 
-template <bool MATS>
+template <int MODE>
 __global__ __launch_bounds__(256) void k_h1d(const DevModel *__restrict__ models,
                                              const Tile *__restrict__ tiles, int ntiles) {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile tl = tiles[t];
+        if (tl.x < 0) continue;
         const DevModel &m = models[tl.x];
         if (*m.it >= m.Nt) continue;
         const int iz = tile_z0(tl) + threadIdx.x;
@@ -20,7 +21,7 @@ __global__ __launch_bounds__(256) void k_h1d(const DevModel *__restrict__ models
         float *__restrict__ Hy = m.F[HY];
         const int bcz = m.bcs[5];
         float Mh = m.mats[0].Mh;
-        if (MATS) Mh = m.mats[m.id[iz]].Mh;
+        if (MODE != MODE_VAC) Mh = m.mats[m.id[iz] & ID_MASK].Mh;
         const float ex = Ex[iz];
         float ex_zp;
         if (iz + 1 < Nz) ex_zp = Ex[iz + 1];
@@ -54,11 +55,12 @@ __global__ __launch_bounds__(256) void k_h1d(const DevModel *__restrict__ models
     }
 }
 
-template <bool MATS>
+template <int MODE, bool INTEG>
 __global__ __launch_bounds__(256) void k_e1d(const DevModel *__restrict__ models,
                                              const Tile *__restrict__ tiles, int ntiles) {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile tl = tiles[t];
+        if (tl.x < 0) continue;
         const DevModel &m = models[tl.x];
         if (*m.it >= m.Nt) continue;
         const int iz = tile_z0(tl) + threadIdx.x;
@@ -68,9 +70,9 @@ __global__ __launch_bounds__(256) void k_e1d(const DevModel *__restrict__ models
         float *__restrict__ Dx = m.F[DX];
         float *__restrict__ Ex = m.F[EX];
         const int bcz = m.bcs[4];
-        int g = 0;
-        if (MATS) g = m.id[iz];
-        const DevMat &mt = m.mats[g];
+        int idb = 0;
+        if (MODE != MODE_VAC || INTEG) idb = m.id[iz];
+        const DevMat &mt = m.mats[MODE != MODE_VAC ? (idb & ID_MASK) : 0];
         const float Md1 = mt.Md1, Md2 = mt.Md2;
         const float hy = Hy[iz];
         float hy_zm;
@@ -103,29 +105,40 @@ __global__ __launch_bounds__(256) void k_e1d(const DevModel *__restrict__ models
         d[0] = Md1 * d[0] + Md2 * (0.0f - dHyz);
         Dx[iz] = d[0];
         float e[1] = {0.0f};
-        if (MATS && (mt.flags & (MAT_DISP | MAT_PLASMA))) {
+        if (MODE == MODE_FULL && (mt.flags & (MAT_DISP | MAT_PLASMA))) {
             e[0] = Ex[iz];
             material_e<1>(mt, m.slot[iz], d, e, m.dt, m.dtf);
-        } else if (MATS && (mt.flags & MAT_KERR)) {
+        } else if (MODE == MODE_FULL && (mt.flags & MAT_KERR)) {
             material_e<1>(mt, 0, d, e, m.dt, m.dtf);
         } else {
             e[0] = d[0] * mt.Me;
         }
         Ex[iz] = e[0];
         check_finite(e[0], m.flag, true);
+        if (INTEG && !(idb & ID_TOUCHED)) {   // src/outputs.jl:186-189
+            const float dt = m.dtf;
+            if (m.Sa) {
+                const float a = e[0] * hy;
+                m.Sa[iz] += sqrtf(a * a) * dt;
+            }
+            if (m.E2) m.E2[iz] += e[0] * e[0] * dt;
+        }
     }
 }
 
-void launch_h1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, bool mats,
+void launch_h1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode,
                 hipStream_t s) {
-    if (mats) hipLaunchKernelGGL(k_h1d<true>, dim3(grid), dim3(TX1), 0, s, m, tiles, ntiles);
-    else hipLaunchKernelGGL(k_h1d<false>, dim3(grid), dim3(TX1), 0, s, m, tiles, ntiles);
+    if (mode == MODE_VAC) hipLaunchKernelGGL(k_h1d<MODE_VAC>, dim3(grid), dim3(TX1), 0, s, m, tiles, ntiles);
+    else hipLaunchKernelGGL(k_h1d<MODE_SIMPLE>, dim3(grid), dim3(TX1), 0, s, m, tiles, ntiles);
 }
 
-void launch_e1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, bool mats,
+void launch_e1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ,
                 hipStream_t s) {
-    if (mats) hipLaunchKernelGGL(k_e1d<true>, dim3(grid), dim3(TX1), 0, s, m, tiles, ntiles);
-    else hipLaunchKernelGGL(k_e1d<false>, dim3(grid), dim3(TX1), 0, s, m, tiles, ntiles);
+#define E1(MD, IN) hipLaunchKernelGGL((k_e1d<MD, IN>), dim3(grid), dim3(TX1), 0, s, m, tiles, ntiles)
+    if (mode == MODE_VAC) { if (integ) E1(MODE_VAC, true); else E1(MODE_VAC, false); }
+    else if (mode == MODE_SIMPLE) { if (integ) E1(MODE_SIMPLE, true); else E1(MODE_SIMPLE, false); }
+    else { if (integ) E1(MODE_FULL, true); else E1(MODE_FULL, false); }
+#undef E1
 }
 
 }  // namespace mfd

@@ -7,11 +7,12 @@
 
 namespace mfd {
 
-template <bool MATS>
+template <int MODE>
 __global__ __launch_bounds__(256) void k_h2d(const DevModel *__restrict__ models,
                                              const Tile *__restrict__ tiles, int ntiles) {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile tl = tiles[t];
+        if (tl.x < 0) continue;
         const DevModel &m = models[tl.x];
         if (*m.it >= m.Nt) continue;
         const int ix = tl.y + threadIdx.x;
@@ -41,7 +42,7 @@ __global__ __launch_bounds__(256) void k_h2d(const DevModel *__restrict__ models
             else if (bcx == BC_DIRICHLET) ez_xp = 0.0f;
             else ez_xp = ez;
             float Mh = Mh0;
-            if (MATS) Mh = m.mats[m.id[i]].Mh;
+            if (MODE != MODE_VAC) Mh = m.mats[m.id[i] & ID_MASK].Mh;
             float dExz = (ex_zp - ex) * idz;
             float dEzx = (ez_xp - ez) * idx;
             float hy = Hy[i];
@@ -96,11 +97,12 @@ __global__ __launch_bounds__(256) void k_h2d(const DevModel *__restrict__ models
     }
 }
 
-template <bool MATS>
+template <int MODE, bool INTEG>
 __global__ __launch_bounds__(256) void k_e2d(const DevModel *__restrict__ models,
                                              const Tile *__restrict__ tiles, int ntiles) {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile tl = tiles[t];
+        if (tl.x < 0) continue;
         const DevModel &m = models[tl.x];
         if (*m.it >= m.Nt) continue;
         const int ix = tl.y + threadIdx.x;
@@ -131,9 +133,9 @@ __global__ __launch_bounds__(256) void k_e2d(const DevModel *__restrict__ models
             if (!xfirst || bcx == BC_PERIODIC) hy_xm = Hy[i + xoff];
             else if (bcx == BC_DIRICHLET) hy_xm = 0.0f;
             else hy_xm = hy;
-            int g = 0;
-            if (MATS) g = m.id[i];
-            const DevMat &mt = MATS ? m.mats[g] : vac;
+            int idb = 0;
+            if (MODE != MODE_VAC || INTEG) idb = m.id[i];
+            const DevMat &mt = MODE != MODE_VAC ? m.mats[idb & ID_MASK] : vac;
             const float Md1 = mt.Md1, Md2 = mt.Md2;
             float dHyx = (hy - hy_xm) * idx;
             float dHyz = (hy - hy_zm) * idz;
@@ -186,31 +188,42 @@ __global__ __launch_bounds__(256) void k_e2d(const DevModel *__restrict__ models
             d[1] = Md1 * d[1] + Md2 * (dHyx - 0.0f);
             Dx[i] = d[0]; Dz[i] = d[1];
             float e[2] = {0.0f, 0.0f};
-            if (MATS && (mt.flags & (MAT_DISP | MAT_PLASMA))) {
+            if (MODE == MODE_FULL && (mt.flags & (MAT_DISP | MAT_PLASMA))) {
                 e[0] = Ex[i]; e[1] = Ez[i];
                 material_e<2>(mt, m.slot[i], d, e, m.dt, m.dtf);
-            } else if (MATS && (mt.flags & MAT_KERR)) {
+            } else if (MODE == MODE_FULL && (mt.flags & MAT_KERR)) {
                 material_e<2>(mt, 0, d, e, m.dt, m.dtf);
             } else {
                 e[0] = d[0] * mt.Me; e[1] = d[1] * mt.Me;
             }
             Ex[i] = e[0]; Ez[i] = e[1];
             check_finite(e[0], m.flag, true);
+            if (INTEG && !(idb & ID_TOUCHED)) {   // src/outputs.jl:322-325
+                const float dt = m.dtf;
+                if (m.Sa) {
+                    const float a = -e[1] * hy, b = e[0] * hy;
+                    m.Sa[i] += sqrtf(a * a + b * b) * dt;
+                }
+                if (m.E2) m.E2[i] += (e[0] * e[0] + e[1] * e[1]) * dt;
+            }
             hy_zm = hy;
         }
     }
 }
 
-void launch_h2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, bool mats,
+void launch_h2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode,
                 hipStream_t s) {
-    if (mats) hipLaunchKernelGGL(k_h2d<true>, dim3(grid), dim3(TX2), 0, s, m, tiles, ntiles);
-    else hipLaunchKernelGGL(k_h2d<false>, dim3(grid), dim3(TX2), 0, s, m, tiles, ntiles);
+    if (mode == MODE_VAC) hipLaunchKernelGGL(k_h2d<MODE_VAC>, dim3(grid), dim3(TX2), 0, s, m, tiles, ntiles);
+    else hipLaunchKernelGGL(k_h2d<MODE_SIMPLE>, dim3(grid), dim3(TX2), 0, s, m, tiles, ntiles);
 }
 
-void launch_e2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, bool mats,
+void launch_e2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ,
                 hipStream_t s) {
-    if (mats) hipLaunchKernelGGL(k_e2d<true>, dim3(grid), dim3(TX2), 0, s, m, tiles, ntiles);
-    else hipLaunchKernelGGL(k_e2d<false>, dim3(grid), dim3(TX2), 0, s, m, tiles, ntiles);
+#define E2(MD, IN) hipLaunchKernelGGL((k_e2d<MD, IN>), dim3(grid), dim3(TX2), 0, s, m, tiles, ntiles)
+    if (mode == MODE_VAC) { if (integ) E2(MODE_VAC, true); else E2(MODE_VAC, false); }
+    else if (mode == MODE_SIMPLE) { if (integ) E2(MODE_SIMPLE, true); else E2(MODE_SIMPLE, false); }
+    else { if (integ) E2(MODE_FULL, true); else E2(MODE_FULL, false); }
+#undef E2
 }
 
 }  // namespace mfd

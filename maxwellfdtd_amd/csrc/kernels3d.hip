@@ -13,11 +13,12 @@
 
 namespace mfd {
 
-template <bool MATS>
+template <int MODE>
 __global__ __launch_bounds__(256) void k_h3d(const DevModel *__restrict__ models,
                                              const Tile *__restrict__ tiles, int ntiles) {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile tl = tiles[t];
+        if (tl.x < 0) continue;
         const DevModel &m = models[tl.x];
         if (*m.it >= m.Nt) continue;
         const int ix = tl.y + threadIdx.x, iy = tl.z + threadIdx.y;
@@ -63,7 +64,7 @@ __global__ __launch_bounds__(256) void k_h3d(const DevModel *__restrict__ models
             else if (bcy == BC_DIRICHLET) { ex_yp = 0.0f; ez_yp = 0.0f; }
             else { ex_yp = ex; ez_yp = ez; }
             float Mh = Mh0;
-            if (MATS) Mh = m.mats[m.id[i]].Mh;
+            if (MODE != MODE_VAC) Mh = m.mats[m.id[i] & ID_MASK].Mh;
             float dExy = (ex_yp - ex) * idy;
             float dExz = (ex_zp - ex) * idz;
             float dEyx = (ey_xp - ey) * idx;
@@ -153,11 +154,13 @@ __global__ __launch_bounds__(256) void k_h3d(const DevModel *__restrict__ models
     }
 }
 
-template <bool MATS>
+template <int MODE, bool INTEG>
 __global__ __launch_bounds__(256) void k_e3d(const DevModel *__restrict__ models,
                                              const Tile *__restrict__ tiles, int ntiles) {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile tl = tiles[t];
+        if (tl.x < 0) continue;
+        if (tl.x < 0) continue;
         const DevModel &m = models[tl.x];
         if (*m.it >= m.Nt) continue;
         const int ix = tl.y + threadIdx.x, iy = tl.z + threadIdx.y;
@@ -204,9 +207,10 @@ __global__ __launch_bounds__(256) void k_e3d(const DevModel *__restrict__ models
             if (!yfirst || bcy == BC_PERIODIC) { hx_ym = Hx[i + yoff]; hz_ym = Hz[i + yoff]; }
             else if (bcy == BC_DIRICHLET) { hx_ym = 0.0f; hz_ym = 0.0f; }
             else { hx_ym = hx; hz_ym = hz; }
-            int g = 0;
-            if (MATS) g = m.id[i];
-            const DevMat &mt = MATS ? m.mats[g] : vac;
+            int idb = 0;
+            if (MODE != MODE_VAC || INTEG) idb = m.id[i];
+            const int g = idb & ID_MASK;
+            const DevMat &mt = MODE != MODE_VAC ? m.mats[g] : vac;
             const float Md1 = mt.Md1, Md2 = mt.Md2;
             float dHxy = (hx - hx_ym) * idy;
             float dHxz = (hx - hx_zm) * idz;
@@ -293,10 +297,10 @@ __global__ __launch_bounds__(256) void k_e3d(const DevModel *__restrict__ models
             d[2] = Md1 * d[2] + Md2 * (dHyx - dHxy);
             Dx[i] = d[0]; Dy[i] = d[1]; Dz[i] = d[2];
             float e[3] = {0.0f, 0.0f, 0.0f};
-            if (MATS && (mt.flags & (MAT_DISP | MAT_PLASMA))) {
+            if (MODE == MODE_FULL && (mt.flags & (MAT_DISP | MAT_PLASMA))) {
                 e[0] = Ex[i]; e[1] = Ey[i]; e[2] = Ez[i];
                 material_e<3>(mt, m.slot[i], d, e, m.dt, m.dtf);
-            } else if (MATS && (mt.flags & MAT_KERR)) {
+            } else if (MODE == MODE_FULL && (mt.flags & MAT_KERR)) {
                 material_e<3>(mt, 0, d, e, m.dt, m.dtf);
             } else {
                 const float Me = mt.Me;
@@ -304,23 +308,37 @@ __global__ __launch_bounds__(256) void k_e3d(const DevModel *__restrict__ models
             }
             Ex[i] = e[0]; Ey[i] = e[1]; Ez[i] = e[2];
             check_finite(e[0], m.flag, true);
+            // integral variables of the step (src/outputs.jl:480-483); cells that a source
+            // rewrites afterwards are accumulated by the post-source fixup instead
+            if (INTEG && !(idb & ID_TOUCHED)) {
+                const float dt = m.dtf;
+                if (m.Sa) {
+                    const float a = e[1] * hz - e[2] * hy, b = e[2] * hx - e[0] * hz;
+                    const float c = e[0] * hy - e[1] * hx;
+                    m.Sa[i] += sqrtf(a * a + b * b + c * c) * dt;
+                }
+                if (m.E2) m.E2[i] += (e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) * dt;
+            }
             hx_zm = hx; hy_zm = hy;
         }
     }
 }
 
-void launch_h3d(const DevModel *m, const Tile *tiles, int ntiles, int grid, bool mats,
+void launch_h3d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode,
                 hipStream_t s) {
     dim3 block(TX3, TY3);
-    if (mats) hipLaunchKernelGGL(k_h3d<true>, dim3(grid), block, 0, s, m, tiles, ntiles);
-    else hipLaunchKernelGGL(k_h3d<false>, dim3(grid), block, 0, s, m, tiles, ntiles);
+    if (mode == MODE_VAC) hipLaunchKernelGGL(k_h3d<MODE_VAC>, dim3(grid), block, 0, s, m, tiles, ntiles);
+    else hipLaunchKernelGGL(k_h3d<MODE_SIMPLE>, dim3(grid), block, 0, s, m, tiles, ntiles);
 }
 
-void launch_e3d(const DevModel *m, const Tile *tiles, int ntiles, int grid, bool mats,
+void launch_e3d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ,
                 hipStream_t s) {
     dim3 block(TX3, TY3);
-    if (mats) hipLaunchKernelGGL(k_e3d<true>, dim3(grid), block, 0, s, m, tiles, ntiles);
-    else hipLaunchKernelGGL(k_e3d<false>, dim3(grid), block, 0, s, m, tiles, ntiles);
+#define E3(MD, IN) hipLaunchKernelGGL((k_e3d<MD, IN>), dim3(grid), block, 0, s, m, tiles, ntiles)
+    if (mode == MODE_VAC) { if (integ) E3(MODE_VAC, true); else E3(MODE_VAC, false); }
+    else if (mode == MODE_SIMPLE) { if (integ) E3(MODE_SIMPLE, true); else E3(MODE_SIMPLE, false); }
+    else { if (integ) E3(MODE_FULL, true); else E3(MODE_FULL, false); }
+#undef E3
 }
 
 }  // namespace mfd

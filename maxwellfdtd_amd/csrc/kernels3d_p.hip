@@ -1,0 +1,359 @@
+// 3D Yee H and E half-steps, pipelined single-cell-per-lane form
+// (update_H_kernel! / update_E_kernel! 3D, src/models.jl:735-1175).
+//
+// A tile is 64 x-cells by TY rows by a z range; each lane owns one (x, y) column and
+// marches up z.  Every load a plane needs (own fields, the y-neighbour row, the z+1
+// plane of the H half, material ids, integral accumulators, and the edge lane's x
+// neighbour) is issued one plane ahead of that plane's compute and stores, so two planes
+// of loads are in flight per wave while registers stay few enough for a high wave count.
+// In-wave x neighbours come through a cross-lane shuffle.  Material coefficients come
+// from an LDS table.  All field pointers are accessed in the global address space.
+#include "common.hpp"
+#include "material.hpp"
+
+namespace mfd {
+
+typedef __attribute__((address_space(1))) float gf1;
+typedef __attribute__((address_space(1))) const float gcf1;
+typedef const __attribute__((address_space(1))) unsigned char gcu1;
+
+struct LayP {
+    int present, ib, nb;
+    const float *K, *A, *B;
+    float *p0, *p1;
+};
+__device__ __forceinline__ LayP layp(const DevLayer &l, bool h) {
+    LayP r;
+    r.present = l.present; r.ib = l.ib; r.nb = l.nb;
+    r.K = l.K; r.A = l.A; r.B = l.B;
+    r.p0 = h ? l.psiE0 : l.psiH0;
+    r.p1 = h ? l.psiE1 : l.psiH1;
+    return r;
+}
+
+// one CPML slab update of a cell: psi = B psi + A d for the derivative pair, returns psi
+__device__ __forceinline__ void slab2(const LayP &L, int p, unsigned j, float &d0, float &d1,
+                                      float &a0, float &a1) {
+    const float B = G(L.B)[p], A = G(L.A)[p], K = G(L.K)[p];
+    a0 = B * G(L.p0)[j] + A * d0;
+    a1 = B * G(L.p1)[j] + A * d1;
+    G(L.p0)[j] = a0;
+    G(L.p1)[j] = a1;
+    d0 = d0 / K;
+    d1 = d1 / K;
+}
+
+// ---------------------------------- H half ---------------------------------------------
+struct HP {
+    float ez, exn, eyn, exy, ezy, hx, hy, hz, eyE, ezE;
+    unsigned char g;
+};
+
+template <int MODE, int TY>
+__global__ __launch_bounds__(64 * TY) void k_h3d_p(const DevModel *__restrict__ models,
+                                                   const Tile *__restrict__ tiles, int ntiles) {
+    __shared__ float mh_tab[128];
+    const int lane = threadIdx.x, ty = threadIdx.y;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const Tile tl = tiles[t];
+        if (tl.x < 0) continue;
+        const DevModel &m = models[tl.x];
+        if (*m.it >= m.Nt) continue;
+        const int nm = MODE == MODE_VAC ? 0 : m.nmat;
+        for (int k = lane + 64 * ty; k <= nm; k += 64 * TY) mh_tab[k] = m.mats[k].Mh;
+        __syncthreads();
+        const int Nx = m.Nx, Ny = m.Ny, Nz = m.Nz;
+        const int ix = tl.y + lane, iy = tl.z + ty;
+        if (ix < Nx && iy < Ny) {
+            const int z0 = tile_z0(tl), z1 = tile_z1(tl);
+            gcf1 *Ex = G((const float *)m.F[EX]), *Ey = G((const float *)m.F[EY]);
+            gcf1 *Ez = G((const float *)m.F[EZ]);
+            gf1 *Hx = G(m.F[HX]), *Hy = G(m.F[HY]), *Hz = G(m.F[HZ]);
+            gcu1 *id = G(m.id);
+            const unsigned sz = (unsigned)m.sz, col = (unsigned)(ix + m.sy * iy);
+            const float idx = m.idx, idy = m.idy, idz = m.idz;
+            const int bcx = m.bcs[1], bcy = m.bcs[3], bcz = m.bcs[5];
+            const bool yin = iy + 1 < Ny;
+            const bool yload = yin || bcy == BC_PERIODIC;
+            const unsigned yoff = yin ? (unsigned)m.sy : (unsigned)(-(long long)(Ny - 1) * m.sy);
+            const bool edge = lane == 63 || ix + 1 >= Nx;
+            const bool eload = edge && (ix + 1 < Nx || bcx == BC_PERIODIC);
+            const unsigned xoff = ix + 1 < Nx ? 1u : (unsigned)(1 - Nx);
+            LayP L[6];
+#pragma unroll
+            for (int f = 0; f < 6; f++) L[f] = layp(m.L[f], true);
+            const bool xs0 = L[0].present && ix <= L[0].ib, xs1 = L[1].present && ix >= L[1].ib;
+            const bool ys0 = L[2].present && iy <= L[2].ib, ys1 = L[3].present && iy >= L[3].ib;
+            auto load = [&](int iz, HP &P) {
+                const unsigned i = col + sz * iz;
+                P.ez = Ez[i];
+                if (iz + 1 < Nz) { P.exn = Ex[i + sz]; P.eyn = Ey[i + sz]; }
+                else if (bcz == BC_PERIODIC) { P.exn = Ex[col]; P.eyn = Ey[col]; }
+                if (yload) { P.exy = Ex[i + yoff]; P.ezy = Ez[i + yoff]; }
+                P.hx = Hx[i]; P.hy = Hy[i]; P.hz = Hz[i];
+                if (eload) { P.eyE = Ey[i + xoff]; P.ezE = Ez[i + xoff]; }
+                if (MODE != MODE_VAC) P.g = id[i];
+            };
+            float ex = Ex[col + sz * z0], ey = Ey[col + sz * z0];
+            auto compute = [&](int iz, HP &P) {
+                const unsigned i = col + sz * iz;
+                if (iz + 1 >= Nz && bcz != BC_PERIODIC) {
+                    P.exn = bcz == BC_DIRICHLET ? 0.0f : ex;
+                    P.eyn = bcz == BC_DIRICHLET ? 0.0f : ey;
+                }
+                if (!yload) {
+                    P.exy = bcy == BC_DIRICHLET ? 0.0f : ex;
+                    P.ezy = bcy == BC_DIRICHLET ? 0.0f : P.ez;
+                }
+                float eyx = __shfl_down(ey, 1), ezx = __shfl_down(P.ez, 1);
+                if (edge) {
+                    if (eload) { eyx = P.eyE; ezx = P.ezE; }
+                    else if (bcx == BC_DIRICHLET) { eyx = 0.0f; ezx = 0.0f; }
+                    else { eyx = ey; ezx = P.ez; }
+                }
+                const float Mh = MODE != MODE_VAC ? mh_tab[P.g & ID_MASK] : mh_tab[0];
+                float dExy = (P.exy - ex) * idy;
+                float dExz = (P.exn - ex) * idz;
+                float dEyx = (eyx - ey) * idx;
+                float dEyz = (P.eyn - ey) * idz;
+                float dEzx = (ezx - P.ez) * idx;
+                float dEzy = (P.ezy - P.ez) * idy;
+                float hx = P.hx, hy = P.hy, hz = P.hz;
+                float a0, a1;
+                if (xs0) {   // src/models.jl:805-815
+                    slab2(L[0], ix, ix + (unsigned)L[0].nb * (iy + (unsigned)Ny * iz), dEyx, dEzx, a0, a1);
+                    hy += Mh * a1; hz -= Mh * a0;
+                }
+                if (xs1) {   // src/models.jl:816-826
+                    const int p = ix - L[1].ib;
+                    slab2(L[1], p, p + (unsigned)L[1].nb * (iy + (unsigned)Ny * iz), dEyx, dEzx, a0, a1);
+                    hy += Mh * a1; hz -= Mh * a0;
+                }
+                if (ys0) {   // src/models.jl:827-837
+                    slab2(L[2], iy, ix + (unsigned)Nx * (iy + (unsigned)L[2].nb * iz), dExy, dEzy, a0, a1);
+                    hx -= Mh * a1; hz += Mh * a0;
+                }
+                if (ys1) {   // src/models.jl:838-848
+                    const int p = iy - L[3].ib;
+                    slab2(L[3], p, ix + (unsigned)Nx * (p + (unsigned)L[3].nb * iz), dExy, dEzy, a0, a1);
+                    hx -= Mh * a1; hz += Mh * a0;
+                }
+                if (L[4].present && iz <= L[4].ib) {   // src/models.jl:849-859
+                    slab2(L[4], iz, col + sz * iz, dExz, dEyz, a0, a1);
+                    hx += Mh * a1; hy -= Mh * a0;
+                }
+                if (L[5].present && iz >= L[5].ib) {   // src/models.jl:860-871
+                    const int p = iz - L[5].ib;
+                    slab2(L[5], p, col + sz * p, dExz, dEyz, a0, a1);
+                    hx += Mh * a1; hy -= Mh * a0;
+                }
+                hx -= Mh * (dEzy - dEyz);
+                hy -= Mh * (dExz - dEzx);
+                hz -= Mh * (dEyx - dExy);
+                Hx[i] = hx; Hy[i] = hy; Hz[i] = hz;
+                ex = P.exn; ey = P.eyn;
+            };
+            HP A, B;
+            load(z0, A);
+            for (int iz = z0; iz < z1; iz += 2) {
+                if (iz + 1 < z1) load(iz + 1, B);
+                compute(iz, A);
+                if (iz + 1 >= z1) break;
+                if (iz + 2 < z1) load(iz + 2, A);
+                compute(iz + 1, B);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------- E half ---------------------------------------------
+struct EP {
+    float hx, hy, hz, hxy, hzy, d0, d1, d2, sa, e2, hyE, hzE;
+    unsigned char g;
+};
+
+template <int MODE, bool INTEG, int TY>
+__global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ models,
+                                                   const Tile *__restrict__ tiles, int ntiles) {
+    __shared__ float4 ctab[128];    // (Md1, Md2, Me, has ADE/Kerr/plasma)
+    const int lane = threadIdx.x, ty = threadIdx.y;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const Tile tl = tiles[t];
+        if (tl.x < 0) continue;
+        const DevModel &m = models[tl.x];
+        if (*m.it >= m.Nt) continue;
+        const int nm = MODE == MODE_VAC ? 0 : m.nmat;
+        for (int k = lane + 64 * ty; k <= nm; k += 64 * TY) {
+            const DevMat &mt = m.mats[k];
+            ctab[k] = make_float4(mt.Md1, mt.Md2, mt.Me, mt.flags ? 1.0f : 0.0f);
+        }
+        __syncthreads();
+        const int Nx = m.Nx, Ny = m.Ny, Nz = m.Nz;
+        const int ix = tl.y + lane, iy = tl.z + ty;
+        if (ix < Nx && iy < Ny) {
+            const int z0 = tile_z0(tl), z1 = tile_z1(tl);
+            gcf1 *Hx = G((const float *)m.F[HX]), *Hy = G((const float *)m.F[HY]);
+            gcf1 *Hz = G((const float *)m.F[HZ]);
+            gf1 *Dx = G(m.F[DX]), *Dy = G(m.F[DY]), *Dz = G(m.F[DZ]);
+            gf1 *Ex = G(m.F[EX]), *Ey = G(m.F[EY]), *Ez = G(m.F[EZ]);
+            gf1 *Sa = G(m.Sa), *E2 = G(m.E2);
+            const bool doSa = INTEG && m.Sa, doE2 = INTEG && m.E2;
+            gcu1 *id = G(m.id);
+            const unsigned sz = (unsigned)m.sz, col = (unsigned)(ix + m.sy * iy);
+            const float idx = m.idx, idy = m.idy, idz = m.idz, dtf = m.dtf;
+            const int bcx = m.bcs[0], bcy = m.bcs[2], bcz = m.bcs[4];
+            const bool yin = iy > 0;
+            const bool yload = yin || bcy == BC_PERIODIC;
+            const unsigned yoff = yin ? (unsigned)(-m.sy) : (unsigned)((Ny - 1) * m.sy);
+            const bool edge = lane == 0 || ix == 0;
+            const bool eload = edge && (ix > 0 || bcx == BC_PERIODIC);
+            const unsigned xoff = ix > 0 ? (unsigned)-1 : (unsigned)(Nx - 1);
+            LayP L[6];
+#pragma unroll
+            for (int f = 0; f < 6; f++) L[f] = layp(m.L[f], false);
+            const bool xs0 = L[0].present && ix <= L[0].ib, xs1 = L[1].present && ix >= L[1].ib;
+            const bool ys0 = L[2].present && iy <= L[2].ib, ys1 = L[3].present && iy >= L[3].ib;
+            auto load = [&](int iz, EP &P) {
+                const unsigned i = col + sz * iz;
+                P.hx = Hx[i]; P.hy = Hy[i]; P.hz = Hz[i];
+                if (yload) { P.hxy = Hx[i + yoff]; P.hzy = Hz[i + yoff]; }
+                P.d0 = Dx[i]; P.d1 = Dy[i]; P.d2 = Dz[i];
+                if (eload) { P.hyE = Hy[i + xoff]; P.hzE = Hz[i + xoff]; }
+                if (MODE != MODE_VAC || INTEG) P.g = id[i];
+                if (doSa) P.sa = Sa[i];
+                if (doE2) P.e2 = E2[i];
+            };
+            float hxm = 0.0f, hym = 0.0f;     // H(z-1)
+            bool zm_own = false;
+            if (z0 > 0) { hxm = Hx[col + sz * (z0 - 1)]; hym = Hy[col + sz * (z0 - 1)]; }
+            else if (bcz == BC_PERIODIC) { hxm = Hx[col + sz * (Nz - 1)]; hym = Hy[col + sz * (Nz - 1)]; }
+            else if (bcz == BC_NEUMANN) zm_own = true;
+            auto compute = [&](int iz, EP &P) {
+                const unsigned i = col + sz * iz;
+                if (zm_own) { hxm = P.hx; hym = P.hy; zm_own = false; }
+                if (!yload) {
+                    P.hxy = bcy == BC_DIRICHLET ? 0.0f : P.hx;
+                    P.hzy = bcy == BC_DIRICHLET ? 0.0f : P.hz;
+                }
+                float hyx = __shfl_up(P.hy, 1), hzx = __shfl_up(P.hz, 1);
+                if (edge) {
+                    if (eload) { hyx = P.hyE; hzx = P.hzE; }
+                    else if (bcx == BC_DIRICHLET) { hyx = 0.0f; hzx = 0.0f; }
+                    else { hyx = P.hy; hzx = P.hz; }
+                }
+                const int g = P.g & ID_MASK;
+                const float4 cf = MODE != MODE_VAC ? ctab[g] : ctab[0];
+                const float Md1 = cf.x, Md2 = cf.y;
+                float dHxy = (P.hx - P.hxy) * idy;
+                float dHxz = (P.hx - hxm) * idz;
+                float dHyx = (P.hy - hyx) * idx;
+                float dHyz = (P.hy - hym) * idz;
+                float dHzx = (P.hz - hzx) * idx;
+                float dHzy = (P.hz - P.hzy) * idy;
+                float d0 = P.d0, d1 = P.d1, d2 = P.d2;
+                float a0, a1;
+                if (xs0) {   // src/models.jl:976-986
+                    slab2(L[0], ix, ix + (unsigned)L[0].nb * (iy + (unsigned)Ny * iz), dHyx, dHzx, a0, a1);
+                    d1 -= Md2 * a1; d2 += Md2 * a0;
+                }
+                if (xs1) {   // src/models.jl:987-997
+                    const int p = ix - L[1].ib;
+                    slab2(L[1], p, p + (unsigned)L[1].nb * (iy + (unsigned)Ny * iz), dHyx, dHzx, a0, a1);
+                    d1 -= Md2 * a1; d2 += Md2 * a0;
+                }
+                if (ys0) {   // src/models.jl:998-1006
+                    slab2(L[2], iy, ix + (unsigned)Nx * (iy + (unsigned)L[2].nb * iz), dHxy, dHzy, a0, a1);
+                    d0 += Md2 * a1; d2 -= Md2 * a0;
+                }
+                if (ys1) {   // src/models.jl:1007-1015
+                    const int p = iy - L[3].ib;
+                    slab2(L[3], p, ix + (unsigned)Nx * (p + (unsigned)L[3].nb * iz), dHxy, dHzy, a0, a1);
+                    d0 += Md2 * a1; d2 -= Md2 * a0;
+                }
+                if (L[4].present && iz <= L[4].ib) {   // src/models.jl:1016-1025
+                    slab2(L[4], iz, col + sz * iz, dHxz, dHyz, a0, a1);
+                    d0 -= Md2 * a1; d1 += Md2 * a0;
+                }
+                if (L[5].present && iz >= L[5].ib) {   // src/models.jl:1026-1035
+                    const int p = iz - L[5].ib;
+                    slab2(L[5], p, col + sz * p, dHxz, dHyz, a0, a1);
+                    d0 -= Md2 * a1; d1 += Md2 * a0;
+                }
+                d0 = Md1 * d0 + Md2 * (dHzy - dHyz);
+                d1 = Md1 * d1 + Md2 * (dHxz - dHzx);
+                d2 = Md1 * d2 + Md2 * (dHyx - dHxy);
+                float e[3];
+                e[0] = d0 * cf.z; e[1] = d1 * cf.z; e[2] = d2 * cf.z;
+                if (MODE == MODE_FULL && cf.w != 0.0f) {
+                    const DevMat &mt = m.mats[g];
+                    const float dd[3] = {d0, d1, d2};
+                    int s = 0;
+                    if (mt.flags & (MAT_DISP | MAT_PLASMA)) {
+                        e[0] = Ex[i]; e[1] = Ey[i]; e[2] = Ez[i];
+                        s = G(m.slot)[i];
+                    }
+                    material_e<3>(mt, s, dd, e, m.dt, dtf);
+                }
+                Dx[i] = d0; Dy[i] = d1; Dz[i] = d2;
+                Ex[i] = e[0]; Ey[i] = e[1]; Ez[i] = e[2];
+                check_finite(e[0], m.flag, true);
+                if (INTEG && !(P.g & ID_TOUCHED)) {   // src/outputs.jl:480-483
+                    if (doSa) {
+                        const float a = e[1] * P.hz - e[2] * P.hy, b = e[2] * P.hx - e[0] * P.hz;
+                        const float cc = e[0] * P.hy - e[1] * P.hx;
+                        Sa[i] = P.sa + sqrtf(a * a + b * b + cc * cc) * dtf;
+                    }
+                    if (doE2) E2[i] = P.e2 + (e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) * dtf;
+                }
+                hxm = P.hx; hym = P.hy;
+            };
+            EP A, B;
+            load(z0, A);
+            for (int iz = z0; iz < z1; iz += 2) {
+                if (iz + 1 < z1) load(iz + 1, B);
+                compute(iz, A);
+                if (iz + 1 >= z1) break;
+                if (iz + 2 < z1) load(iz + 2, A);
+                compute(iz + 1, B);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int TY>
+static void lh(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, hipStream_t s) {
+    dim3 block(64, TY);
+    if (mode == MODE_VAC) hipLaunchKernelGGL((k_h3d_p<MODE_VAC, TY>), dim3(grid), block, 0, s, m, tiles, ntiles);
+    else hipLaunchKernelGGL((k_h3d_p<MODE_SIMPLE, TY>), dim3(grid), block, 0, s, m, tiles, ntiles);
+}
+
+template <int TY>
+static void le(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ,
+               hipStream_t s) {
+    dim3 block(64, TY);
+#define EP_(MD, IN) hipLaunchKernelGGL((k_e3d_p<MD, IN, TY>), dim3(grid), block, 0, s, m, tiles, ntiles)
+    if (mode == MODE_VAC) { if (integ) EP_(MODE_VAC, true); else EP_(MODE_VAC, false); }
+    else if (mode == MODE_SIMPLE) { if (integ) EP_(MODE_SIMPLE, true); else EP_(MODE_SIMPLE, false); }
+    else { if (integ) EP_(MODE_FULL, true); else EP_(MODE_FULL, false); }
+#undef EP_
+}
+
+void launch_h3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int ty,
+                  hipStream_t s) {
+    if (ty == 1) lh<1>(m, tiles, ntiles, grid, mode, s);
+    else if (ty == 2) lh<2>(m, tiles, ntiles, grid, mode, s);
+    else if (ty == 4) lh<4>(m, tiles, ntiles, grid, mode, s);
+    else lh<8>(m, tiles, ntiles, grid, mode, s);
+}
+
+void launch_e3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ,
+                  int ty, hipStream_t s) {
+    if (ty == 1) le<1>(m, tiles, ntiles, grid, mode, integ, s);
+    else if (ty == 2) le<2>(m, tiles, ntiles, grid, mode, integ, s);
+    else if (ty == 4) le<4>(m, tiles, ntiles, grid, mode, integ, s);
+    else le<8>(m, tiles, ntiles, grid, mode, integ, s);
+}
+
+}  // namespace mfd

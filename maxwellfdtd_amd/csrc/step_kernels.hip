@@ -150,22 +150,26 @@ __device__ inline double aux_value(const DevSource &S, int Nz, int comp, int z) 
     }
 }
 
-__global__ __launch_bounds__(256) void k_tfsf_face(const DevModel *__restrict__ models,
-                                                   int slot, int f) {
+// Both faces of one axis run in one launch (they write disjoint planes); the three
+// axes run in three launches in the reference's face order.
+__global__ __launch_bounds__(256) void k_tfsf_axis(const DevModel *__restrict__ models,
+                                                   int slot, int axis) {
     const DevModel &m = models[blockIdx.y];
     if (slot >= m.nsrc) return;
     const DevSource &S = m.src[slot];
     if (S.kind != SRC_TFSF) return;
     const int it0 = *m.it;
     if (it0 >= m.Nt) return;
-    const int axis = f >> 1, side = f & 1;
     const int ua = axis == 0 ? 1 : 0, va = axis == 2 ? 1 : 2;
     const int U1 = S.box[2 * ua], U2 = S.box[2 * ua + 1];
     const int V1 = S.box[2 * va], V2 = S.box[2 * va + 1];
     const int N1 = S.box[2 * axis], N2 = S.box[2 * axis + 1];
     const int nu = U2 - U1 + 1, nv = V2 - V1 + 1;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nu * nv) return;
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 2 * nu * nv) return;
+    const int side = e >= nu * nv ? 1 : 0;
+    e -= side * nu * nv;
+    const int f = 2 * axis + side;
     const int u = U1 + e % nu, v = V1 + e / nu;
     const int nH = side ? N2 : N1 - 1, nE = side ? N2 : N1;
     const float sg = side ? -1.0f : 1.0f;
@@ -233,11 +237,11 @@ __global__ __launch_bounds__(256) void k_tfsf_face(const DevModel *__restrict__ 
 void launch_tfsf(const DevModel *m, int nmodels, int slot, int ndims, int maxface,
                  hipStream_t s) {
     hipLaunchKernelGGL(k_tfsf_aux, dim3(nmodels), dim3(256), 0, s, m, slot);
-    const int nb = (maxface + 255) / 256;
-    for (int f = 0; f < 6; f++) {
-        if (ndims == 1 && f < 4) continue;
-        if (ndims == 2 && (f == 2 || f == 3)) continue;
-        hipLaunchKernelGGL(k_tfsf_face, dim3(nb, nmodels), dim3(256), 0, s, m, slot, f);
+    const int nb = (2 * maxface + 255) / 256;
+    for (int axis = 0; axis < 3; axis++) {
+        if (ndims == 1 && axis < 2) continue;
+        if (ndims == 2 && axis == 1) continue;
+        hipLaunchKernelGGL(k_tfsf_axis, dim3(nb, nmodels), dim3(256), 0, s, m, slot, axis);
     }
 }
 
@@ -379,17 +383,17 @@ void launch_dft_flush(const DevModel *m, int nmodels, int maxn, int nk, hipStrea
     hipLaunchKernelGGL(k_dft_flush, dim3(nb, nmodels), dim3(256), 0, s, m, nk);
 }
 
-// ---- integral variables: update_integral_variables! (src/outputs.jl:182-191,318-327,
-// 475-485): Sa += |E x H| dt, E2 += |E|^2 dt over all cells, after the sources.
-__global__ __launch_bounds__(256) void k_integrals(const DevModel *__restrict__ models,
-                                                   int ndims) {
+// ---- integral variables of source-written cells: update_integral_variables!
+// (src/outputs.jl:182-191,318-327,475-485) for the cells the E kernels skipped, after
+// every source of the step has been applied.
+__global__ __launch_bounds__(256) void k_integrals_touched(const DevModel *__restrict__ models,
+                                                           int ndims) {
     const DevModel &m = models[blockIdx.y];
     if (*m.it >= m.Nt || m.int_flags == 0) return;
-    const long long N = m.N;
     const float dt = m.dtf;
-    const long long stride = (long long)gridDim.x * blockDim.x;
     float *Sa = m.Sa, *E2 = m.E2;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N; i += stride) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m.ntouched; k += gridDim.x * blockDim.x) {
+        const long long i = m.touched[k];
         if (ndims == 3) {
             const float hx = m.F[HX][i], hy = m.F[HY][i], hz = m.F[HZ][i];
             const float ex = m.F[EX][i], ey = m.F[EY][i], ez = m.F[EZ][i];
@@ -416,8 +420,11 @@ __global__ __launch_bounds__(256) void k_integrals(const DevModel *__restrict__ 
     }
 }
 
-void launch_integrals(const DevModel *m, int nmodels, int grid, int ndims, hipStream_t s) {
-    hipLaunchKernelGGL(k_integrals, dim3(grid, nmodels), dim3(256), 0, s, m, ndims);
+void launch_integrals_touched(const DevModel *m, int nmodels, int maxn, int ndims, hipStream_t s) {
+    int nb = (maxn + 255) / 256;
+    if (nb < 1) nb = 1;
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(k_integrals_touched, dim3(nb, nmodels), dim3(256), 0, s, m, ndims);
 }
 
 // ---- step counter ----------------------------------------------------------------------
