@@ -185,6 +185,30 @@ int32_t mfd_model_get_monitor(mfd_model *m, int32_t imon, int32_t comp, double *
 int32_t mfd_model_get_output(mfd_model *m, int32_t which, double *out);
 int32_t mfd_model_destroy(mfd_model *m);
 
+/* ---- engine options and instrumentation (no counterpart in the reference: its backends
+ * have no tunables; these select launch shapes of this engine) ----------------------- */
+enum {
+    MFD_OPT_GRAPH_STEPS = 0,   /* steps captured per hipGraph replay (0 = eager launches) */
+    MFD_OPT_ZCHUNK = 1,        /* z planes marched by one 3D/2D tile (1..65535)            */
+    MFD_OPT_PERSISTENT = 2,    /* workgroups per CU of the persistent form (0 = off)       */
+    MFD_OPT_KERNEL_TIMING = 3, /* hipEvent pairs around every H and E launch (eager)       */
+    MFD_OPT_DFT_BLOCK = 4,     /* time steps per DFT flush (1..64, before upload)          */
+    MFD_OPT_ROWS = 5,          /* y rows per 3D workgroup (1, 2, 4, 8)                     */
+    MFD_OPT_KERNEL_FORM = 6,   /* 3D kernel form: 0 scalar, 1 4-cell vector, 2 pipelined   */
+    MFD_OPT_STAGGER = 7        /* byte offset between field arrays (before upload)         */
+};
+/* Changing an option rebuilds the launch plan; field state is kept.                   */
+int32_t mfd_model_set_options(mfd_model *m, int32_t opt, int32_t value);
+/* Accumulated hipEvent time of the H (which = 0) or E (which = 1) half-step launches
+ * since the last reset (which = 2 resets), measured on the model's stream.             */
+int32_t mfd_model_timing(mfd_model *m, int32_t which, double *total_ms, int32_t *count);
+/* HBM bandwidth probes: a float4 device copy of `bytes` bytes, and a grid-stride kernel
+ * with nin reads and nout writes (nin == nout, 1..6) of `bytes_per_array` each, launched
+ * with blocks_per_cu (>= 1) workgroups per CU; average milliseconds per iteration.    */
+int32_t mfd_stream_copy(int64_t bytes, int32_t iters, double *ms_per_iter);
+int32_t mfd_stream_multi(int64_t bytes_per_array, int32_t nin, int32_t nout,
+                         int32_t blocks_per_cu, int32_t iters, double *ms_per_iter);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,7 +12,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ENGINE_LIB = os.path.join(HERE, "libmaxwellfdtd_amd.so")
 
-MFD_OK, MFD_ERR_NAN, MFD_ERR_INF = 0, -5, -6
+MFD_OK, MFD_ERR_ARG, MFD_ERR_STATE, MFD_ERR_HIP = 0, -1, -2, -3
+MFD_ERR_NOMEM, MFD_ERR_NAN, MFD_ERR_INF, MFD_ERR_NODEV = -4, -5, -6, -7
 NAN_MSG = "Something went wrong. I found NaN field values."
 INF_MSG = "Something went wrong. I found Inf field values."
 

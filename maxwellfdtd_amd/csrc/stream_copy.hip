@@ -77,8 +77,9 @@ __global__ __launch_bounds__(256) void k_multi(float4 *const *__restrict__ in,
 extern "C" int32_t mfd_stream_multi(int64_t bytes_per_array, int32_t nin, int32_t nout,
                                     int32_t blocks_per_cu, int32_t iters, double *ms_per_iter) {
     using namespace mfd;
-    if (nin < 1 || nin > 6 || nout < 1 || nout > 6 || nin != nout || iters < 1) {
-        set_error("nin == nout in 1..6");
+    if (nin < 1 || nin > 6 || nout < 1 || nout > 6 || nin != nout || iters < 1 ||
+        blocks_per_cu < 1 || bytes_per_array < 1024) {
+        set_error("need nin == nout in 1..6, blocks_per_cu >= 1, bytes >= 1024");
         return MFD_ERR_ARG;
     }
     const long long n4 = bytes_per_array / 16;

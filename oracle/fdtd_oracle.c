@@ -151,10 +151,17 @@ static int comp_present(const ofd_model *m, int c) {
 }
 
 int32_t ofd_model_create(const mfd_model_desc *d, ofd_model **out) {
-    if (!d || !out || d->ndims < 1 || d->ndims > 3 || d->Nx < 1 || d->Ny < 1 || d->Nz < 2) {
+    if (!d || !out || d->ndims < 1 || d->ndims > 3 || d->Nx < 1 || d->Ny < 1 || d->Nz < 2 ||
+        (d->ndims == 1 && (d->Nx != 1 || d->Ny != 1)) || (d->ndims == 2 && d->Ny != 1) ||
+        d->Nt < 1 || !(d->dt > 0)) {
         set_err("bad model descriptor");
         return MFD_ERR_ARG;
     }
+    for (int i = 0; i < 6; i++)
+        if (d->bcs[i] < 1 || d->bcs[i] > 3) {
+            set_err("Wrong 'bc' value. It can be either ':periodic', ':dirichlet', or ':neumann'.");
+            return MFD_ERR_ARG;
+        }
     ofd_model *m = xcalloc(1, sizeof *m);
     m->ndims = d->ndims; m->Nx = d->Nx; m->Ny = d->Ny; m->Nz = d->Nz;
     m->N = (int64_t)d->Nx * d->Ny * d->Nz;
