@@ -1,0 +1,29 @@
+"""HBM bandwidth probe: times the engine library's device copy kernel.
+
+    python -m maxwellfdtd_amd.hbm_probe [--mib 1024] [--iters 5]
+
+Prints the copy size and the achieved read+write bandwidth.  A copy of a known size is
+also the calibration run for hardware byte counters (FETCH_SIZE / WRITE_SIZE): the
+counters' reading for it, against its 2 x size bytes, gives the counters' scale.
+"""
+import argparse
+import ctypes as C
+
+from . import _lib
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--mib", type=int, default=1024)
+    ap.add_argument("--iters", type=int, default=5)
+    a = ap.parse_args()
+    lib = _lib.engine()
+    nbytes = a.mib << 20
+    ms = C.c_double()
+    lib.check(lib.stream_copy(nbytes, a.iters, C.byref(ms)), "stream_copy")
+    print(f"copy {nbytes} bytes: read {nbytes} + write {nbytes} per launch, "
+          f"{ms.value * 1e3:.1f} us, {2 * nbytes / ms.value / 1e6:.0f} GB/s", flush=True)
+
+
+if __name__ == "__main__":
+    main()
