@@ -195,7 +195,8 @@ enum {
     MFD_OPT_DFT_BLOCK = 4,     /* time steps per DFT flush (1..64, before upload)          */
     MFD_OPT_ROWS = 5,          /* y rows per 3D workgroup (1, 2, 4, 8)                     */
     MFD_OPT_KERNEL_FORM = 6,   /* 3D kernel form: 0 scalar, 1 4-cell vector, 2 pipelined   */
-    MFD_OPT_STAGGER = 7        /* byte offset between field arrays (before upload)         */
+    MFD_OPT_STAGGER = 7,       /* byte offset between field arrays (before upload)         */
+    MFD_OPT_QUANTUM = 8        /* server: steps per scheduling round (default 64)          */
 };
 /* Changing an option rebuilds the launch plan; field state is kept.                   */
 int32_t mfd_model_set_options(mfd_model *m, int32_t opt, int32_t value);
@@ -208,6 +209,62 @@ int32_t mfd_model_timing(mfd_model *m, int32_t which, double *total_ms, int32_t 
 int32_t mfd_stream_copy(int64_t bytes, int32_t iters, double *ms_per_iter);
 int32_t mfd_stream_multi(int64_t bytes_per_array, int32_t nin, int32_t nout,
                          int32_t blocks_per_cu, int32_t iters, double *ms_per_iter);
+
+/* ---- batches (no counterpart in the reference, which runs one model per solve!,
+ * src/models.jl:166-242) ------------------------------------------------------------
+ * A batch advances many built models of one dimensionality together: one H and one E
+ * launch per half-step over the packed tile list of all unfinished models, replayed
+ * from a hipGraph.  Models keep their own Nt; a model that reaches it drops out of the
+ * tile list.  Every model's results are those of stepping it alone.  The caller keeps
+ * ownership of the models: read them with mfd_model_get_* once the batch has run, and
+ * destroy the batch before the models.                                                */
+typedef struct mfd_batch mfd_batch;
+int32_t mfd_batch_create(mfd_batch **out);
+int32_t mfd_batch_add(mfd_batch *b, mfd_model *m);
+/* MFD_OPT_GRAPH_STEPS, ZCHUNK, PERSISTENT, KERNEL_TIMING, ROWS, KERNEL_FORM.          */
+int32_t mfd_batch_set_options(mfd_batch *b, int32_t opt, int32_t value);
+/* Advance every unfinished model by nsteps (nsteps <= 0: until every model is done).  */
+int32_t mfd_batch_run(mfd_batch *b, int32_t nsteps);
+/* Time trial_steps real steps under each candidate launch shape (z-chunk, rows per
+ * workgroup, persistent workgroups per CU) and keep the fastest.  The trial steps are
+ * part of the solves: results do not depend on the launch shape.                      */
+int32_t mfd_batch_autotune(mfd_batch *b, int32_t trial_steps, int32_t *zchunk, int32_t *rows,
+                           int32_t *persistent, double *ms_per_step);
+int32_t mfd_batch_active(mfd_batch *b, int32_t *nactive);
+int32_t mfd_batch_destroy(mfd_batch *b);
+
+/* ---- serving loop (no counterpart in the reference) ----------------------------------
+ * A server owns a worker thread and a stream.  mfd_server_submit queues a built,
+ * not-yet-stepped model; the worker admits queued models in arrival order while their
+ * device footprint fits the HBM budget (and free HBM), uploads them, and advances the
+ * active set together in rounds of MFD_OPT_QUANTUM steps, so requests that arrive while
+ * others run join between rounds.  A model that finishes leaves the tile list before it
+ * is reported done; its device state stays until the caller destroys the model.
+ * Latency is submit -> done on the host clock.                                         */
+enum { MFD_REQ_QUEUED = 0, MFD_REQ_RUNNING = 1, MFD_REQ_DONE = 2, MFD_REQ_FAILED = 3 };
+typedef struct mfd_server_stats {
+    int32_t submitted, queued, active, done, failed;
+    double p50_ms, p99_ms, mean_ms;   /* nearest-rank percentiles over finished requests */
+    double cell_updates;              /* sum of cells * Nt over finished requests        */
+    double busy_s;                    /* first admitted submit -> last completion          */
+    double gcells_per_s;              /* cell_updates / busy_s / 1e9                     */
+    int64_t hbm_in_use;               /* footprint of admitted, unfinished requests      */
+} mfd_server_stats;
+typedef struct mfd_server mfd_server;
+/* hbm_budget <= 0: free HBM minus 2 GiB.  max_active bounds the active set.           */
+int32_t mfd_server_create(int64_t hbm_budget, int32_t max_active, mfd_server **out);
+/* MFD_OPT_QUANTUM, GRAPH_STEPS, ZCHUNK, PERSISTENT, ROWS, KERNEL_FORM; before submits. */
+int32_t mfd_server_set_options(mfd_server *s, int32_t opt, int32_t value);
+int32_t mfd_server_submit(mfd_server *s, mfd_model *m, int64_t *ticket);
+/* state = MFD_REQ_*; status = the request's MFD_OK / MFD_ERR_* once finished.          */
+int32_t mfd_server_poll(mfd_server *s, int64_t ticket, int32_t *state, int32_t *status,
+                        double *latency_ms);
+/* Block until the ticket (ticket <= 0: every request) finished, or timeout_ms passed
+ * (timeout_ms < 0: no limit).                                                         */
+int32_t mfd_server_wait(mfd_server *s, int64_t ticket, int32_t timeout_ms, int32_t *state);
+int32_t mfd_server_get_stats(mfd_server *s, mfd_server_stats *out);
+/* Finishes every submitted request, then stops the worker.                            */
+int32_t mfd_server_destroy(mfd_server *s);
 
 #ifdef __cplusplus
 }

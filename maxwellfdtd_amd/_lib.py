@@ -117,12 +117,37 @@ class Binding:
         raise EngineError(f"{self.prefix}{what} failed ({status}): {self.error()}")
 
 
+class mfd_server_stats(C.Structure):
+    _fields_ = [("submitted", C.c_int32), ("queued", C.c_int32), ("active", C.c_int32),
+                ("done", C.c_int32), ("failed", C.c_int32), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("cell_updates", C.c_double),
+                ("busy_s", C.c_double), ("gcells_per_s", C.c_double), ("hbm_in_use", C.c_int64)]
+
+
+MFD_OPT_GRAPH_STEPS, MFD_OPT_ZCHUNK, MFD_OPT_PERSISTENT, MFD_OPT_KERNEL_TIMING = 0, 1, 2, 3
+MFD_OPT_DFT_BLOCK, MFD_OPT_ROWS, MFD_OPT_KERNEL_FORM, MFD_OPT_STAGGER, MFD_OPT_QUANTUM = 4, 5, 6, 7, 8
+MFD_REQ_QUEUED, MFD_REQ_RUNNING, MFD_REQ_DONE, MFD_REQ_FAILED = 0, 1, 2, 3
+
 ENGINE_EXTRA = {
     "device_info": (I32, [C.POINTER(C.c_int64), C.POINTER(C.c_int64), IP, C.c_char_p, I32]),
     "model_set_options": (I32, [P, I32, I32]),
     "model_timing": (I32, [P, I32, DP, IP]),
     "stream_copy": (I32, [C.c_int64, I32, DP]),
     "stream_multi": (I32, [C.c_int64, I32, I32, I32, I32, DP]),
+    "batch_create": (I32, [C.POINTER(P)]),
+    "batch_add": (I32, [P, P]),
+    "batch_set_options": (I32, [P, I32, I32]),
+    "batch_run": (I32, [P, I32]),
+    "batch_autotune": (I32, [P, I32, IP, IP, IP, DP]),
+    "batch_active": (I32, [P, IP]),
+    "batch_destroy": (I32, [P]),
+    "server_create": (I32, [C.c_int64, I32, C.POINTER(P)]),
+    "server_set_options": (I32, [P, I32, I32]),
+    "server_submit": (I32, [P, P, C.POINTER(C.c_int64)]),
+    "server_poll": (I32, [P, C.c_int64, IP, IP, DP]),
+    "server_wait": (I32, [P, C.c_int64, I32, IP]),
+    "server_get_stats": (I32, [P, C.POINTER(mfd_server_stats)]),
+    "server_destroy": (I32, [P]),
 }
 
 _engine = None

@@ -437,11 +437,28 @@ int Model::upload() {
     UP(hipMemset(ctr, 0, 2 * sizeof(int)));
     m.it = ctr;
     m.flag = ctr + 1;
+    if (!ev) UP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    // the fills and copies above run on the null stream; runners use non-blocking
+    // streams, so finish them before any kernel can read this model
+    UP(hipStreamSynchronize(nullptr));
     dm = m;
     it = 0;
     dft_done = 0;
+    status = MFD_OK;
     uploaded = true;
     return MFD_OK;
+}
+
+// Waits for the last run that stepped this model, whichever runner's stream it used.
+int Model::sync() {
+    if (ev) MFD_HIP(hipEventSynchronize(ev));
+    return MFD_OK;
+}
+
+void Model::release() {
+    if (ev) { hipEventSynchronize(ev); hipEventDestroy(ev); ev = nullptr; }
+    arena.release();
+    uploaded = false;
 }
 
 int Model::read_field(int comp, double *out) {
@@ -454,7 +471,8 @@ int Model::read_field(int comp, double *out) {
         return MFD_OK;
     }
     std::vector<float> tmp(N);
-    MFD_HIP(hipDeviceSynchronize());
+    int st = sync();
+    if (st != MFD_OK) return st;
     MFD_HIP(hipMemcpy(tmp.data(), dm.F[comp], N * sizeof(float), hipMemcpyDeviceToHost));
     for (long long i = 0; i < N; i++) out[i] = tmp[i];
     return MFD_OK;
@@ -469,7 +487,8 @@ int Model::write_field(int comp, const double *in) {
     if (st != MFD_OK) return st;
     std::vector<float> tmp(N);
     for (long long i = 0; i < N; i++) tmp[i] = (float)in[i];
-    MFD_HIP(hipDeviceSynchronize());
+    st = sync();
+    if (st != MFD_OK) return st;
     MFD_HIP(hipMemcpy(dm.F[comp], tmp.data(), N * sizeof(float), hipMemcpyHostToDevice));
     return MFD_OK;
 }
@@ -677,6 +696,7 @@ int Runner::run(int nsteps) {
             done += 1;
         }
     }
+    for (Model *m : models) MFD_HIP(hipEventRecord(m->ev, stream));
     return check_flags();
 }
 
@@ -693,17 +713,57 @@ int Runner::flush_dft_partial() {
     return MFD_OK;
 }
 
+// Reads every model's non-finite flag; each model keeps its own status, and the first
+// failing model's status is returned.
 int Runner::check_flags() {
     MFD_HIP(hipStreamSynchronize(stream));
+    int first = MFD_OK;
     for (Model *m : models) {
         int f = 0;
         MFD_HIP(hipMemcpy(&f, m->dm.flag, sizeof(int), hipMemcpyDeviceToHost));
-        if (f & 1) { set_error("Something went wrong. I found NaN field values."); return MFD_ERR_NAN; }
-        if (f & 2) { set_error("Something went wrong. I found Inf field values."); return MFD_ERR_INF; }
+        if (f & 1) m->status = MFD_ERR_NAN;
+        else if (f & 2) m->status = MFD_ERR_INF;
+        if (m->status != MFD_OK && first == MFD_OK) first = m->status;
+    }
+    if (first == MFD_ERR_NAN) set_error("Something went wrong. I found NaN field values.");
+    if (first == MFD_ERR_INF) set_error("Something went wrong. I found Inf field values.");
+    return first;
+}
+
+}  // namespace mfd
+
+namespace mfd {
+// Launch options shared by single models, batches and the server.  `m` (optional) is
+// the model whose pre-upload settings (DFT block, stagger) are changed.
+int set_option(RunOptions &o, Model *m, int opt, int value) {
+    switch (opt) {
+        case MFD_OPT_GRAPH_STEPS: o.graph_steps = std::max(0, value); break;
+        case MFD_OPT_ZCHUNK:
+            if (value < 1 || value > 65535) { set_error("bad z chunk"); return MFD_ERR_ARG; }
+            o.zchunk = value;
+            break;
+        case MFD_OPT_PERSISTENT: o.persistent = std::max(0, value); break;
+        case MFD_OPT_KERNEL_TIMING: o.kernel_timing = value != 0; break;
+        case MFD_OPT_DFT_BLOCK:
+            if (!m || m->uploaded || value < 1 || value > 64) { set_error("DFT block must be 1..64, set on a model before upload"); return MFD_ERR_ARG; }
+            m->dft_block = value;
+            break;
+        case MFD_OPT_ROWS:
+            if (value != 1 && value != 2 && value != 4 && value != 8) { set_error("rows per workgroup must be 1, 2, 4 or 8"); return MFD_ERR_ARG; }
+            o.ty = value;
+            break;
+        case MFD_OPT_KERNEL_FORM:
+            if (value < 0 || value > 2) { set_error("3D kernel form must be 0, 1 or 2"); return MFD_ERR_ARG; }
+            o.vec = value;
+            break;
+        case MFD_OPT_STAGGER:
+            if (!m || m->uploaded || value < 0) { set_error("stagger must be >= 0, set on a model before upload"); return MFD_ERR_ARG; }
+            m->stagger = value;
+            break;
+        default: set_error("unknown option"); return MFD_ERR_ARG;
     }
     return MFD_OK;
 }
-
 }  // namespace mfd
 
 // =======================================================================================
@@ -711,13 +771,6 @@ int Runner::check_flags() {
 // =======================================================================================
 using namespace mfd;
 
-struct mfd_model {
-    Model m;
-    Runner runner;
-    bool runner_ready = false;
-    RunOptions opt;
-    hipStream_t stream = nullptr;
-};
 
 static int check_device() {
     static int ok = -1;
@@ -988,7 +1041,7 @@ int32_t mfd_model_upload(mfd_model *h) {
 
 static int ensure_runner(mfd_model *h) {
     if (h->runner_ready) return MFD_OK;
-    if (!h->stream) MFD_HIP(hipStreamCreate(&h->stream));
+    if (!h->stream) MFD_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     int st = h->runner.init({&h->m}, h->opt, h->stream);
     if (st != MFD_OK) return st;
     h->runner_ready = true;
@@ -997,6 +1050,7 @@ static int ensure_runner(mfd_model *h) {
 
 int32_t mfd_model_step(mfd_model *h, int32_t nsteps) {
     if (!h || nsteps < 0) { set_error("bad arguments"); return MFD_ERR_ARG; }
+    if (h->owner) { set_error("model belongs to a batch or server"); return MFD_ERR_STATE; }
     if (h->m.it + nsteps > h->m.d.Nt) { set_error("model already ran all Nt steps"); return MFD_ERR_STATE; }
     int st = ensure_runner(h);
     if (st != MFD_OK) return st;
@@ -1034,11 +1088,10 @@ int32_t mfd_model_get_monitor(mfd_model *h, int32_t imon, int32_t comp, double *
         std::fill(out, out + len, 0.0);
         return MFD_OK;
     }
-    if (h->runner_ready) {
-        int st = h->runner.flush_dft_partial();
-        if (st != MFD_OK) return st;
-    }
-    MFD_HIP(hipDeviceSynchronize());
+    int st = ensure_runner(h);
+    if (st == MFD_OK) st = m.sync();
+    if (st == MFD_OK) st = h->runner.flush_dft_partial();
+    if (st != MFD_OK) return st;
     const DevMon &dmn = m.hmon[imon];
     if (!hm.spectral) {
         if (hm.issum) {
@@ -1070,7 +1123,8 @@ int32_t mfd_model_get_output(mfd_model *h, int32_t which, double *out) {
     Model &m = h->m;
     const long long N = m.N;
     if (!m.uploaded) { std::fill(out, out + N, 0.0); return MFD_OK; }
-    MFD_HIP(hipDeviceSynchronize());
+    int st = m.sync();
+    if (st != MFD_OK) return st;
     if (which == MFD_OUT_SA || which == MFD_OUT_E2) {
         const float *src = which == MFD_OUT_SA ? m.dm.Sa : m.dm.E2;
         if (!src) { set_error("integral not enabled"); return MFD_ERR_STATE; }
@@ -1105,9 +1159,10 @@ int32_t mfd_model_get_output(mfd_model *h, int32_t which, double *out) {
 
 int32_t mfd_model_destroy(mfd_model *h) {
     if (!h) return MFD_OK;
+    if (h->owner) { set_error("model belongs to a batch or server"); return MFD_ERR_STATE; }
     if (h->stream) hipStreamSynchronize(h->stream);
     h->runner.release();
-    h->m.arena.release();
+    h->m.release();
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
     return MFD_OK;
@@ -1120,32 +1175,8 @@ int32_t mfd_model_destroy(mfd_model *h) {
 // 2 pipelined), 7 per-array stagger of the field pool (bytes).
 int32_t mfd_model_set_options(mfd_model *h, int32_t opt, int32_t value) {
     if (!h) { set_error("null model"); return MFD_ERR_ARG; }
-    switch (opt) {
-        case 0: h->opt.graph_steps = std::max(0, value); break;
-        case 1:
-            if (value < 1 || value > 65535) { set_error("bad z chunk"); return MFD_ERR_ARG; }
-            h->opt.zchunk = value;
-            break;
-        case 2: h->opt.persistent = std::max(0, value); break;
-        case 3: h->opt.kernel_timing = value != 0; break;
-        case 4:
-            if (h->m.uploaded || value < 1 || value > 64) { set_error("DFT block must be 1..64, set before upload"); return MFD_ERR_ARG; }
-            h->m.dft_block = value;
-            break;
-        case 5:
-            if (value != 1 && value != 2 && value != 4 && value != 8) { set_error("rows per workgroup must be 1, 2, 4 or 8"); return MFD_ERR_ARG; }
-            h->opt.ty = value;
-            break;
-        case 6:
-            if (value < 0 || value > 2) { set_error("3D kernel form must be 0, 1 or 2"); return MFD_ERR_ARG; }
-            h->opt.vec = value;
-            break;
-        case 7:
-            if (h->m.uploaded || value < 0) { set_error("stagger must be >= 0, set before upload"); return MFD_ERR_ARG; }
-            h->m.stagger = value;
-            break;
-        default: set_error("unknown option"); return MFD_ERR_ARG;
-    }
+    int st = set_option(h->opt, &h->m, opt, value);
+    if (st != MFD_OK) return st;
     if (h->runner_ready) {
         h->runner.release();
         h->runner_ready = false;

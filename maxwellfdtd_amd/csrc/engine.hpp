@@ -102,10 +102,15 @@ struct Model {
     std::vector<int> touched;   // cells written by sources (sorted)
     bool has_mats = false, has_tfsf = false, has_spectral = false;
 
+    hipEvent_t ev = nullptr;    // recorded after every run that stepped this model
+    int status = 0;             // MFD_OK or the model's non-finite status
+
     int ncomp() const { return d.ndims == 3 ? 3 : d.ndims == 2 ? 2 : 1; }
     bool comp_present(int c) const;
     int64_t footprint() const;
     int upload();
+    int sync();
+    void release();
     int read_field(int comp, double *out);
     int write_field(int comp, const double *in);
 };
@@ -153,4 +158,16 @@ struct Runner {
     int collect_timing();
 };
 
+int set_option(RunOptions &o, Model *m, int opt, int value);
+
 }  // namespace mfd
+
+// A model handle of the C ABI.  `owner` is set while a batch or a server holds it.
+struct mfd_model {
+    mfd::Model m;
+    mfd::Runner runner;
+    bool runner_ready = false;
+    mfd::RunOptions opt;
+    hipStream_t stream = nullptr;
+    void *owner = nullptr;
+};

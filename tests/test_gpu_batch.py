@@ -1,0 +1,153 @@
+"""Batched runs and the serving loop on the GPU: every solve's result in a batch or in
+the served mix equals the same solve stepped alone (bitwise: the per-cell arithmetic
+does not depend on the tile list or launch shape), and matches the FP64 oracle within
+1e-4 relative L2."""
+import itertools
+import time
+
+import numpy as np
+import pytest
+
+from maxwellfdtd_amd import _lib, workloads
+from maxwellfdtd_amd.batch import Batch, Server, replay
+from maxwellfdtd_amd.models import Handle
+from oracle_util import field_errors, oracle, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+COMPS = {2: ("Hy", "Dx", "Dz", "Ex", "Ez"), 3: ("Hx", "Hy", "Hz", "Dx", "Dy", "Dz", "Ex", "Ey", "Ez")}
+
+
+def small_2d(count=6, seed=2028):
+    return list(itertools.islice(workloads.batched_2d(count=count, seed=seed, nmin=48, nmax=120,
+                                                      smin=60, smax=160), count))
+
+
+def small_3d(count=4, seed=2029):
+    return [m for _, m in itertools.islice(workloads.served_3d(count=count, seed=seed, nmin=32,
+                                                               nmax=44, smin=40, smax=90), count)]
+
+
+def alone(desc):
+    h = Handle(desc, _lib.engine())
+    h.step(desc.Nt)
+    return h
+
+
+def assert_same(h1, h2, ndims):
+    for c in COMPS[ndims]:
+        a, b = h1.field(c), h2.field(c)
+        assert np.array_equal(a, b), (c, np.max(np.abs(a - b)))
+    if h1.desc.integrals & 1:
+        assert np.array_equal(h1.output(0), h2.output(0))
+
+
+def test_batch_2d_equals_alone_and_oracle():
+    models = small_2d()
+    descs = [m.describe() for m in models]
+    assert len({d.Nt for d in descs}) > 1 and len({d.dims for d in descs}) > 1
+    b = Batch(descs)
+    b.run(0)
+    assert b.active == 0
+    for d, hb in zip(descs, b.handles):
+        assert hb.it == d.Nt
+        ha = alone(d)
+        assert_same(hb, ha, 2)
+        ha.close()
+    for d, hb in list(zip(descs, b.handles))[:2]:
+        ho = Handle(d, oracle("f64"))
+        ho.step(d.Nt)
+        errs = field_errors(hb, ho, 2)
+        assert max(errs.values()) <= 1e-4, errs
+        ho.close()
+    b.close()
+
+
+def test_batch_autotune_keeps_results():
+    models = small_2d(count=4, seed=7)
+    descs = [m.describe() for m in models]
+    b = Batch(descs)
+    best = b.autotune(trial_steps=2)
+    assert best["zchunk"] in (8, 16, 32, 64, 128) and best["ms_per_step"] > 0
+    b.run(0)
+    for d, hb in zip(descs, b.handles):
+        ha = alone(d)
+        assert_same(hb, ha, 2)
+        ha.close()
+    b.close()
+
+
+def test_batch_3d_uneven():
+    models = small_3d(count=3)
+    descs = [m.describe() for m in models]
+    b = Batch(descs, options={_lib.MFD_OPT_GRAPH_STEPS: 8})
+    b.run(0)
+    for d, hb in zip(descs, b.handles):
+        ha = alone(d)
+        assert_same(hb, ha, 3)
+        ha.close()
+    d, hb = descs[0], b.handles[0]
+    ho = Handle(d, oracle("f64"))
+    ho.step(d.Nt)
+    errs = field_errors(hb, ho, 3)
+    assert max(errs.values()) <= 1e-4, errs
+    assert rel_l2(hb.output(3), ho.output(3)) <= 1e-4     # plasma density
+    ho.close()
+    b.close()
+
+
+def test_server_results_equal_alone():
+    models = small_3d(count=5, seed=11)
+    descs = [m.describe() for m in models]
+    ref = []
+    for d in descs:
+        h = alone(d)
+        ref.append({c: h.field(c) for c in COMPS[3]} | {"Sa": h.output(0)})
+        h.close()
+    srv = Server(options={_lib.MFD_OPT_QUANTUM: 16})
+    got = {}
+
+    def done(t, h, lat):
+        assert lat is not None and lat > 0
+        got[t] = {c: h.field(c) for c in COMPS[3]} | {"Sa": h.output(0)}
+
+    arrivals = [(0.01 * k, d) for k, d in enumerate(descs)]
+    tickets = replay(srv, arrivals, describe=lambda d: d, on_done=done)
+    st = srv.stats()
+    assert st["done"] == 5 and st["failed"] == 0
+    assert 0 < st["p50_ms"] <= st["p99_ms"]
+    assert st["gcells_per_s"] > 0
+    for t, r in zip(tickets, ref):
+        for k, v in r.items():
+            assert np.array_equal(got[t][k], v), k
+    srv.close()
+
+
+def test_server_budget_and_oversize():
+    models = small_3d(count=3, seed=5)
+    descs = [m.describe() for m in models]
+    fps = []
+    for d in descs:
+        h = Handle(d, _lib.engine())
+        fps.append(C_footprint(h))
+        h.close()
+    # room for the largest request plus a bit: requests run about one at a time
+    srv = Server(hbm_budget=int(max(fps) * 1.2), max_active=8)
+    # a model whose footprint exceeds the whole budget fails with MFD_ERR_NOMEM
+    big = workloads.plane_wave_spheres_3d(n=96, steps=4, nspheres=2, rmin=2, rmax=4, box_margin=14)
+    tb, _ = srv.submit(big.describe())
+    ts = [srv.submit(d)[0] for d in descs]
+    assert srv.wait(0, 120000) == _lib.MFD_REQ_DONE
+    st, code, _ = srv.poll(tb)
+    assert st == _lib.MFD_REQ_FAILED and code == _lib.MFD_ERR_NOMEM
+    for t in ts:
+        st, code, _ = srv.poll(t)
+        assert st == _lib.MFD_REQ_DONE and code == 0
+    srv.close()
+
+
+def C_footprint(h):
+    import ctypes as C
+    v = C.c_int64()
+    h.lib.check(h.lib.model_footprint(h.h, C.byref(v)), "footprint")
+    return v.value
