@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X per-time-step engine.
 
-Workload (default): the 3D plane-wave / 40-sphere model of
+Default workload: the 3D plane-wave / 40-sphere model of
 maxwellfdtd_amd.workloads.plane_wave_spheres_3d -- 256^3 cells, 10-cell CPML on every
 face, TFSF plane wave, FP32 fields -- stepped W (warm-up) + K (timed) time steps.  A
 "step" is one full time step of the model: H half, E half, TFSF, integral variables.
@@ -11,8 +11,14 @@ the dominant kernel (E half-step, hipEvent-timed on the engine's stream), the CP
 baseline (the oracle, a C restatement of the reference algorithm, on the host cores for
 a bounded number of steps of the same model) and the relative L2 error of the GPU
 fields against that FP64 run at the same step.
+
+Other workloads (--workload): dispersive_3d, batched_2d (batch vs one solve at a time),
+served_3d (continuous-batching server vs one request at a time), slab_1d.  Each prints
+its own JSON line.
 """
 import argparse
+import ctypes as C
+import itertools
 import json
 import os
 import sys
@@ -31,36 +37,68 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--n", type=int, default=256, help="cells per axis")
-    ap.add_argument("--timing-steps", type=int, default=200)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-integrals", action="store_true")
-    ap.add_argument("--graph-steps", type=int, default=16)
-    ap.add_argument("--zchunk", type=int, default=16)
-    ap.add_argument("--ty", type=int, default=4)
-    ap.add_argument("--kform", type=int, default=2)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU z-slab sharding is "
-                                   "a design only (see DESIGN.md), so --gpus must be 1"}))
-        sys.exit(2)
+def set_opts(lib, h, opts):
+    for k, v in opts.items():
+        lib.check(lib.model_set_options(h.h, k, v), "set_options")
 
-    from maxwellfdtd_amd import _lib
+
+def kernel_timing(lib, h, steps):
+    """Average hipEvent time (ms) of the H and E launches over `steps` eager steps."""
+    lib.check(lib.model_set_options(h.h, 3, 1), "opt")
+    lib.check(lib.model_timing(h.h, 2, C.byref(C.c_double()), C.byref(C.c_int32())), "timing")
+    h.step(steps)
+    tot, cnt = C.c_double(), C.c_int32()
+    lib.check(lib.model_timing(h.h, 0, C.byref(tot), C.byref(cnt)), "timing")
+    h_ms = tot.value / max(cnt.value, 1)
+    lib.check(lib.model_timing(h.h, 1, C.byref(tot), C.byref(cnt)), "timing")
+    e_ms = tot.value / max(cnt.value, 1)
+    lib.check(lib.model_set_options(h.h, 3, 0), "opt")
+    return h_ms, e_ms, cnt.value
+
+
+def cpu_leg(desc, seconds, N, label, gpu_lib):
+    """The oracle (FP64) for a bounded number of steps, then the GPU engine for the same
+    steps: CPU rate and GPU-vs-FP64 error."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    from oracle_util import field_errors, oracle
+    from maxwellfdtd_amd.models import Handle
+    orc = oracle("f64")
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = orc.set_threads(threads)
+    ho = Handle(desc, orc)
+    t0 = time.perf_counter()
+    ns = 0
+    while time.perf_counter() - t0 < seconds and ns < desc.Nt:
+        ho.step(1)
+        ns += 1
+    cw = time.perf_counter() - t0
+    cpu = {"value": round(N * ns / cw / 1e9, 4), "unit": "Gcell-updates/s", "cores": threads,
+           "kind": "port",
+           "sample": f"first {ns} steps of the same {label} model, FP64, {threads} OpenMP "
+                     f"threads (oracle/fdtd_oracle.c)"}
+    log(f"cpu baseline: {ns} steps in {cw:.1f}s -> {cpu['value']} Gcell/s")
+    hg = Handle(desc, gpu_lib)
+    hg.step(ns)
+    errs = field_errors(hg, ho, desc.ndims)
+    parity = {"steps": ns, "rel_l2_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()}}
+    log(f"parity after {ns} steps: {errs}")
+    hg.close()
+    ho.close()
+    return cpu, parity
+
+
+def base_record(args, value, wall, steps, dtype="f32"):
+    return {"metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s",
+            "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+            "ms_per_step": round(wall / max(steps, 1) * 1e3, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic (seeded geometry and sources)"}
+
+
+# ---------------------------------------------------------------------------------------
+def bench_3d(args, lib):
     from maxwellfdtd_amd.models import Handle
     from maxwellfdtd_amd.workloads import bytes_per_cell_update, plane_wave_spheres_3d
-
-    lib = _lib.engine()
-    import ctypes as C
-    free, total, cus = C.c_int64(), C.c_int64(), C.c_int32()
-    name = C.create_string_buffer(128)
-    lib.check(lib.device_info(C.byref(free), C.byref(total), C.byref(cus), name, 128), "device_info")
-    log(f"device {name.value.decode()} CUs={cus.value} HBM free={free.value/2**30:.1f} GiB")
 
     t0 = time.perf_counter()
     nsteps_model = args.warmup + args.steps + args.timing_steps
@@ -72,10 +110,7 @@ def main():
         f"materials={len(desc.materials)}, integrals={desc.integrals}")
 
     h = Handle(desc, lib)
-    lib.check(lib.model_set_options(h.h, 0, args.graph_steps), "opt")
-    lib.check(lib.model_set_options(h.h, 1, args.zchunk), "opt")
-    lib.check(lib.model_set_options(h.h, 5, args.ty), "opt")
-    lib.check(lib.model_set_options(h.h, 6, args.kform), "opt")
+    set_opts(lib, h, {0: args.graph_steps, 1: args.zchunk, 5: args.ty, 6: args.kform})
     h.step(args.warmup)
     log(f"warmup {args.warmup} steps done")
     t0 = time.perf_counter()
@@ -85,84 +120,295 @@ def main():
     log(f"timed {args.steps} steps: {wall*1e3:.1f} ms, {gcells:.2f} Gcell-updates/s")
 
     # dominant-kernel timing: hipEvent pairs around the H and E launches on the engine's
-    # stream, eager launches (no graph) over timing_steps steps
-    lib.check(lib.model_set_options(h.h, 3, 1), "opt")
-    h.step(args.timing_steps)
-    tot, cnt = C.c_double(), C.c_int32()
-    lib.check(lib.model_timing(h.h, 0, C.byref(tot), C.byref(cnt)), "timing")
-    h_ms = tot.value / max(cnt.value, 1)
-    lib.check(lib.model_timing(h.h, 1, C.byref(tot), C.byref(cnt)), "timing")
-    e_ms = tot.value / max(cnt.value, 1)
+    # stream, eager launches (no graph)
+    h_ms, e_ms, cnt = kernel_timing(lib, h, args.timing_steps)
     bh, be = bytes_per_cell_update(model)
     log(f"H kernel {h_ms*1e3:.1f} us ({N*bh/h_ms/1e6:.0f} GB/s), E kernel {e_ms*1e3:.1f} us "
-        f"({N*be/e_ms/1e6:.0f} GB/s) over {cnt.value} launches")
+        f"({N*be/e_ms/1e6:.0f} GB/s) over {cnt} launches")
     h.close()
 
-    e_bytes = N * be
-    achieved = e_bytes / (e_ms * 1e-3) / 1e9
+    achieved = N * be / (e_ms * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(HERE, "profiles", "traffic_plane_wave_spheres_3d_256.json")
     if os.path.exists(tpath) and args.n == 256:
         with open(tpath) as f:
             traffic = json.load(f).get("e_kernel_bytes_per_launch")
 
-    cpu = None
-    parity = None
+    cpu = parity = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(HERE, "tests"))
-        from oracle_util import field_errors, oracle
-        orc = oracle("f64")
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-        threads = orc.set_threads(threads)
-        ho = Handle(desc, orc)
-        t0 = time.perf_counter()
-        ns = 0
-        while time.perf_counter() - t0 < args.cpu_seconds and ns < desc.Nt:
-            ho.step(1)
-            ns += 1
-        cw = time.perf_counter() - t0
-        cpu = {"value": round(N * ns / cw / 1e9, 4), "unit": "Gcell-updates/s", "cores": threads,
-               "kind": "port",
-               "sample": f"first {ns} steps of the same {args.n}^3 model, FP64, "
-                         f"{threads} OpenMP threads (oracle/fdtd_oracle.c)"}
-        log(f"cpu baseline: {ns} steps in {cw:.1f}s -> {cpu['value']} Gcell/s")
-        hg = Handle(desc, lib)
-        hg.step(ns)
-        errs = field_errors(hg, ho, 3)
-        parity = {"steps": ns, "rel_l2_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()}}
-        log(f"parity after {ns} steps: {errs}")
-        hg.close()
-        ho.close()
+        cpu, parity = cpu_leg(desc, args.cpu_seconds, N, f"{args.n}^3", lib)
 
-    out = {
-        "metric": METRIC,
-        "value": round(gcells, 3),
-        "unit": "Gcell-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded geometry and sources)",
-        "config": {"workload": f"plane_wave_spheres_3d: {args.n}^3 cells, 10-cell CPML, TFSF "
-                               f"plane wave, 40 dielectric spheres, Sa/E2 integrals "
-                               f"{'off' if args.no_integrals else 'on'}",
-                   "cells": N, "model_steps": desc.Nt, "graph_steps": args.graph_steps,
-                   "zchunk": args.zchunk, "rows_per_workgroup": args.ty, "kernel_form": args.kform},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic,
-                     "kernel": "k_e3d_p (E half-step)", "bytes_per_cell": round(be, 3),
-                     "avg_launch_us": round(e_ms * 1e3, 2),
-                     "h_kernel": {"avg_launch_us": round(h_ms * 1e3, 2), "bytes_per_cell": round(bh, 3),
-                                  "achieved": round(N * bh / h_ms / 1e6, 1)}},
-        "step_bandwidth_GBs": round(N * (bh + be) * args.steps / wall / 1e9, 1),
-        "cpu_baseline": cpu,
-        "parity": parity,
-    }
+    out = base_record(args, gcells, wall, args.steps)
+    out["config"] = {"workload": f"plane_wave_spheres_3d: {args.n}^3 cells, 10-cell CPML, TFSF "
+                                 f"plane wave, 40 dielectric spheres, Sa/E2 integrals "
+                                 f"{'off' if args.no_integrals else 'on'}",
+                     "cells": N, "model_steps": desc.Nt, "graph_steps": args.graph_steps,
+                     "zchunk": args.zchunk, "rows_per_workgroup": args.ty, "kernel_form": args.kform}
+    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                       "traffic": traffic,
+                       "kernel": "k_e3d_p (E half-step)", "bytes_per_cell": round(be, 3),
+                       "avg_launch_us": round(e_ms * 1e3, 2),
+                       "h_kernel": {"avg_launch_us": round(h_ms * 1e3, 2), "bytes_per_cell": round(bh, 3),
+                                    "achieved": round(N * bh / h_ms / 1e6, 1)}}
+    out["step_bandwidth_GBs"] = round(N * (bh + be) * args.steps / wall / 1e9, 1)
+    out["cpu_baseline"] = cpu
+    out["parity"] = parity
+    return out
+
+
+# ---------------------------------------------------------------------------------------
+def bench_dispersive_3d(args, lib):
+    from maxwellfdtd_amd.models import Handle
+    from maxwellfdtd_amd.workloads import cpml_slab_cells, dispersive_3d
+
+    steps = args.steps if args.steps_given else 5000
+    t0 = time.perf_counter()
+    model, mons = dispersive_3d(steps=steps + args.timing_steps)
+    desc = model.describe(monitors=mons)
+    N = int(np.prod(desc.dims))
+    geo = model.geometry
+    f_sub, f_metal = float(np.mean(geo == 1)), float(np.mean(geo == 2))
+    s = cpml_slab_cells(model) / N
+    # algorithmic bytes per cell-update: base 86 + CPML, + per dispersive cell 12 (E^n)
+    # + 16 per Drude/Lorentz pole per component (3 poles x 3) or 8 per Debye component
+    b_cell = 86 + 32 * s + f_metal * (12 + 3 * 3 * 16) + f_sub * (12 + 3 * 8)
+    log(f"model built in {time.perf_counter()-t0:.1f}s: {desc.dims}, f_metal={f_metal:.4f}, "
+        f"f_sub={f_sub:.4f}, B_cell={b_cell:.2f}")
+    h = Handle(desc, lib)
+    set_opts(lib, h, {0: args.graph_steps, 1: args.zchunk, 5: args.ty, 6: args.kform})
+    t0 = time.perf_counter()
+    h.step(steps)
+    wall = time.perf_counter() - t0
+    gcells = N * steps / wall / 1e9
+    log(f"{steps} steps: {wall:.2f} s, {gcells:.2f} Gcell/s")
+    h_ms, e_ms, cnt = kernel_timing(lib, h, args.timing_steps)
+    bh = 37 + 16 * s
+    be = b_cell - bh
+    h.close()
+    achieved = N * be / (e_ms * 1e-3) / 1e9
+    cpu = parity = None
+    if not args.no_cpu:
+        cpu, parity = cpu_leg(desc, args.cpu_seconds, N, "384x384x192 dispersive", lib)
+    out = base_record(args, gcells, wall, steps)
+    out["config"] = {"workload": "dispersive_3d: 384x384x192 cells, dx=5 nm, 10-cell CPML, "
+                                 "broadband TFSF, 6x6 Drude+2 Lorentz spheres on a Debye "
+                                 "substrate, 384x384 DFT plane at 16 frequencies",
+                     "cells": N, "model_steps": desc.Nt, "f_metal": round(f_metal, 5),
+                     "f_sub": round(f_sub, 5), "bytes_per_cell_update": round(b_cell, 3)}
+    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                       "kernel": "k_e3d_p (E half-step, MODE_FULL)", "bytes_per_cell": round(be, 3),
+                       "avg_launch_us": round(e_ms * 1e3, 2),
+                       "h_kernel": {"avg_launch_us": round(h_ms * 1e3, 2),
+                                    "bytes_per_cell": round(bh, 3)}}
+    out["step_bandwidth_GBs"] = round(N * b_cell * steps / wall / 1e9, 1)
+    out["cpu_baseline"] = cpu
+    out["parity"] = parity
+    return out
+
+
+# ---------------------------------------------------------------------------------------
+def percentiles(lat):
+    lat = np.sort(np.asarray(lat, dtype=float))
+    if len(lat) == 0:
+        return 0.0, 0.0
+    k = lambda p: lat[min(len(lat), max(1, int(np.ceil(p / 100 * len(lat))))) - 1]
+    return float(k(50)), float(k(99))
+
+
+def bench_batched_2d(args, lib):
+    from maxwellfdtd_amd.batch import Batch
+    from maxwellfdtd_amd.models import Handle
+    from maxwellfdtd_amd.workloads import batched_2d
+
+    t0 = time.perf_counter()
+    descs = [m.describe() for m in batched_2d(count=args.count,
+                                               smin=max(2, int(2000 * args.step_scale)),
+                                               smax=max(2, int(10000 * args.step_scale)))]
+    cells = np.array([int(np.prod(d.dims)) for d in descs], dtype=float)
+    work = float(np.sum(cells * np.array([d.Nt for d in descs])))
+    log(f"{len(descs)} instances built in {time.perf_counter()-t0:.1f}s: "
+        f"{cells.sum()/1e6:.1f} M cells, {work/1e12:.3f} T cell-updates")
+
+    # batched: one batch, autotuned launch shape, solves retire as they finish
+    t0 = time.perf_counter()
+    b = Batch(descs)
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tune = b.autotune(trial_steps=8)
+    log(f"autotune: {tune}")
+    done_t = np.zeros(len(descs))
+    while b.active > 0:
+        b.run(args.chunk)
+        now = time.perf_counter() - t0
+        for i, h in enumerate(b.handles):
+            if done_t[i] == 0 and h.it >= descs[i].Nt:
+                done_t[i] = now
+    wall_b = time.perf_counter() - t0
+    g_batch = work / wall_b / 1e9
+    p50_b, p99_b = percentiles(done_t * 1e3)
+    log(f"batched: {wall_b:.2f} s ({t_build:.1f} s host build+copy outside), {g_batch:.2f} Gcell/s, "
+        f"p50 {p50_b:.0f} ms p99 {p99_b:.0f} ms")
+    b.close()
+
+    # plain: the same solves one at a time (each its own graph-replayed run)
+    t_start = time.perf_counter()
+    busy = 0.0
+    lat_p = []
+    nplain = len(descs) if args.plain_count <= 0 else min(args.plain_count, len(descs))
+    for d in descs[:nplain]:
+        h = Handle(d, lib)
+        h.lib.check(h.lib.model_upload(h.h), "upload")
+        t1 = time.perf_counter()
+        h.step(d.Nt)
+        busy += time.perf_counter() - t1
+        lat_p.append((time.perf_counter() - t_start) * 1e3)
+        h.close()
+    work_p = float(np.sum(cells[:nplain] * np.array([d.Nt for d in descs[:nplain]])))
+    g_plain = work_p / busy / 1e9
+    p50_p, p99_p = percentiles(lat_p)
+    log(f"plain ({nplain} solves): {busy:.2f} s GPU-busy, {g_plain:.2f} Gcell/s, p50 {p50_p:.0f} ms "
+        f"p99 {p99_p:.0f} ms")
+    out = base_record(args, g_batch, wall_b, int(max(d.Nt for d in descs)))
+    out["config"] = {"workload": f"batched_2d: {len(descs)} uneven 2D instances, Nx,Nz in "
+                                 f"256..2048, Lorentz rod + dielectrics, steps 2000..10000"
+                                 + (f" x {args.step_scale}" if args.step_scale != 1.0 else ""),
+                     "instances": len(descs), "cells": int(cells.sum()), "cell_updates": work,
+                     "autotune": tune}
+    out["batched"] = {"gcells_per_s": round(g_batch, 3), "wall_s": round(wall_b, 3),
+                      "p50_ms": round(p50_b, 1), "p99_ms": round(p99_b, 1),
+                      "latency": "batch start -> solve finished, all submitted at t=0, "
+                                 f"observed every {args.chunk} steps"}
+    out["plain"] = {"gcells_per_s": round(g_plain, 3), "solves": nplain, "busy_s": round(busy, 3),
+                    "p50_ms": round(p50_p, 1), "p99_ms": round(p99_p, 1),
+                    "latency": "queue of all solves at t=0 run one after another"}
+    out["speedup_vs_plain"] = round(g_batch / g_plain, 3)
+    out["roofline"] = None
+    return out
+
+
+def bench_served_3d(args, lib):
+    from maxwellfdtd_amd.batch import Server, replay
+    from maxwellfdtd_amd.models import Handle
+    from maxwellfdtd_amd.workloads import served_3d
+
+    t0 = time.perf_counter()
+    reqs = [(t_arr, m.describe()) for t_arr, m in
+            served_3d(count=args.count, smin=max(2, int(1000 * args.step_scale)),
+                      smax=max(2, int(4000 * args.step_scale)))]
+    cells = np.array([int(np.prod(d.dims)) for _, d in reqs], dtype=float)
+    work = float(np.sum(cells * np.array([d.Nt for _, d in reqs])))
+    log(f"{len(reqs)} requests built in {time.perf_counter()-t0:.1f}s: {work/1e12:.3f} T cell-updates, "
+        f"arrivals over {reqs[-1][0]:.2f} s")
+
+    srv = Server(options={8: args.quantum})
+    t0 = time.perf_counter()
+    replay(srv, reqs, describe=lambda d: d)
+    wall_s = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    log(f"served: {wall_s:.2f} s, {st}")
+
+    # plain: one request at a time in arrival order (a FIFO queue in front of a single
+    # solver); service times measured, waiting derived from the arrival times
+    clock, busy, lat_p = 0.0, 0.0, []
+    for t_arr, d in reqs:
+        h = Handle(d, lib)
+        h.lib.check(h.lib.model_upload(h.h), "upload")
+        t1 = time.perf_counter()
+        h.step(d.Nt)
+        svc = time.perf_counter() - t1
+        h.close()
+        start = max(clock, t_arr)
+        clock = start + svc
+        busy += svc
+        lat_p.append((clock - t_arr) * 1e3)
+    g_plain = work / busy / 1e9
+    p50_p, p99_p = percentiles(lat_p)
+    log(f"plain: {busy:.2f} s busy, {g_plain:.2f} Gcell/s, p50 {p50_p:.0f} ms p99 {p99_p:.0f} ms")
+    out = base_record(args, st["gcells_per_s"], wall_s, 0)
+    out["config"] = {"workload": f"served_3d: first {len(reqs)} requests of the Poisson trace "
+                                 "(20 req/s), N^3 with N in 64..320, Kerr + 6-photon plasma "
+                                 "sphere, TFSF 800 nm, steps 1000..4000"
+                                 + (f" x {args.step_scale}" if args.step_scale != 1.0 else ""),
+                     "requests": len(reqs), "cell_updates": work, "quantum_steps": args.quantum}
+    out["served"] = {"gcells_per_s": round(st["gcells_per_s"], 3), "p50_ms": round(st["p50_ms"], 1),
+                     "p99_ms": round(st["p99_ms"], 1), "mean_ms": round(st["mean_ms"], 1),
+                     "done": st["done"], "failed": st["failed"], "busy_s": round(st["busy_s"], 3)}
+    out["plain"] = {"gcells_per_s": round(g_plain, 3), "p50_ms": round(p50_p, 1),
+                    "p99_ms": round(p99_p, 1), "busy_s": round(busy, 3)}
+    out["speedup_vs_plain"] = round(st["gcells_per_s"] / g_plain, 3) if g_plain else None
+    out["roofline"] = None
+    return out
+
+
+def bench_slab_1d(args, lib):
+    from maxwellfdtd_amd.models import Handle
+    from maxwellfdtd_amd.workloads import slab_1d
+    model, mons = slab_1d()
+    desc = model.describe(monitors=mons)
+    N = int(np.prod(desc.dims))
+    h = Handle(desc, lib)
+    t0 = time.perf_counter()
+    h.step(desc.Nt)
+    wall = time.perf_counter() - t0
+    h.close()
+    cpu = parity = None
+    if not args.no_cpu:
+        cpu, parity = cpu_leg(desc, args.cpu_seconds, N, "1D slab", lib)
+    out = base_record(args, N * desc.Nt / wall / 1e9, wall, desc.Nt)
+    out["config"] = {"workload": "slab_1d: Nz=4000, 8000 steps, eps=2.25 slab, point + DFT monitor",
+                     "cells": N}
+    out["roofline"] = None
+    out["cpu_baseline"] = cpu
+    out["parity"] = parity
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="plane_wave_spheres_3d",
+                    choices=["plane_wave_spheres_3d", "dispersive_3d", "batched_2d", "served_3d",
+                             "slab_1d"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--n", type=int, default=256, help="cells per axis")
+    ap.add_argument("--timing-steps", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-integrals", action="store_true")
+    ap.add_argument("--graph-steps", type=int, default=16)
+    ap.add_argument("--zchunk", type=int, default=16)
+    ap.add_argument("--ty", type=int, default=4)
+    ap.add_argument("--kform", type=int, default=2)
+    ap.add_argument("--count", type=int, default=None, help="instances / requests")
+    ap.add_argument("--step-scale", type=float, default=1.0)
+    ap.add_argument("--chunk", type=int, default=200)
+    ap.add_argument("--plain-count", type=int, default=0)
+    ap.add_argument("--quantum", type=int, default=64)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU z-slab sharding is "
+                                   "a design only (see DESIGN.md), so --gpus must be 1"}))
+        sys.exit(2)
+    args.steps_given = args.steps is not None
+    if args.steps is None:
+        args.steps = 2000
+    if args.count is None:
+        args.count = 512 if args.workload == "batched_2d" else 32
+
+    from maxwellfdtd_amd import _lib
+    lib = _lib.engine()
+    free, total, cus = C.c_int64(), C.c_int64(), C.c_int32()
+    name = C.create_string_buffer(128)
+    lib.check(lib.device_info(C.byref(free), C.byref(total), C.byref(cus), name, 128), "device_info")
+    log(f"device {name.value.decode()} CUs={cus.value} HBM free={free.value/2**30:.1f} GiB")
+    fn = {"plane_wave_spheres_3d": bench_3d, "dispersive_3d": bench_dispersive_3d,
+          "batched_2d": bench_batched_2d, "served_3d": bench_served_3d,
+          "slab_1d": bench_slab_1d}[args.workload]
+    out = fn(args, lib)
     print(json.dumps(out), flush=True)
 
 

@@ -22,6 +22,11 @@ __device__ __forceinline__ gp<T> G(T *p) { return (gp<T>)p; }
 template <class T>
 __device__ __forceinline__ gp<const T> G(const T *p) { return (gp<const T>)p; }
 
+// Two-element vectors usable through address-space-1 pointers (HIP's float2/double2
+// classes only bind to generic references).
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+
 enum { HX = 0, HY, HZ, DX, DY, DZ, EX, EY, EZ };
 enum { BC_PERIODIC = 1, BC_DIRICHLET = 2, BC_NEUMANN = 3 };
 enum { MAT_DISP = 1, MAT_KERR = 2, MAT_PLASMA = 4 };
@@ -58,6 +63,7 @@ struct DevMat {
     float2 *Pp;                 // [nc][cnt]: (Pp^n, Pp^{n-1})
     float *Pa;                  // [nc][cnt]
     float *JE;                  // [cnt]
+    const int *cells;           // [cnt]: linear cell index of each state slot (stateful materials)
 };
 
 // Soft/hard point source or TFSF source of one model.
@@ -157,5 +163,9 @@ void launch_tfsf(const DevModel *m, int nmodels, int slot, int ndims, int maxfac
 void launch_monitors(const DevModel *m, int nmodels, int maxn, hipStream_t s);
 void launch_dft_flush(const DevModel *m, int nmodels, int maxn, int nk, hipStream_t s);
 void launch_finish(const DevModel *m, int nmodels, hipStream_t s);
+// E of the cells of stateful (ADE / plasma) materials, after the E half-step kernel
+// updated their D: one work item per state slot, chunks {model, material, s0, s1}.
+void launch_e_state(const DevModel *m, const int4 *chunks, int nchunks, int ndims, bool integ,
+                    hipStream_t s);
 
 }  // namespace mfd

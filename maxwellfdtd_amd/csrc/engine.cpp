@@ -56,7 +56,7 @@ int64_t Model::footprint() const {
     for (size_t k = 0; k < mats.size(); k++) {
         const auto &md = mats[k].d;
         const long long c = cnt[k + 1];
-        if (md.nq > 0 || md.isplasma) stateful = true;
+        if (md.nq > 0 || md.isplasma) { stateful = true; b += c * 4; }
         b += (int64_t)md.nq * ncomp() * c * 8;
         if (md.isplasma) b += c * (16 + ncomp() * 12 + 4);
     }
@@ -205,11 +205,13 @@ int Model::upload() {
         std::vector<unsigned char> mark(N, 0);
         mark_touched(*this, mark);
         touched.clear();
+        state_cells.assign(nmat + 1, std::vector<int>());
         for (long long i = 0; i < N; i++) {
             const int g = geom[i];
             ids[i] = (unsigned char)(g | (mark[i] ? ID_TOUCHED : 0));
             if (mark[i]) touched.push_back((int)i);
             if (stateful) slot[i] = (int)cnt[g];
+            if (g > 0 && (mats[g - 1].d.nq > 0 || mats[g - 1].d.isplasma)) state_cells[g].push_back((int)i);
             cnt[g]++;
         }
         unsigned char *did;
@@ -243,6 +245,11 @@ int Model::upload() {
         v.cnt = (int)c;
         v.K2 = (float)(hm.d.Mk2 * hm.d.Me * hm.d.Me);
         v.K3 = (float)(hm.d.Mk3 * hm.d.Me * hm.d.Me * hm.d.Me);
+        if (!state_cells[k + 1].empty()) {
+            int *cp;
+            UP(upload_vec(arena, &cp, state_cells[k + 1]));
+            v.cells = cp;
+        }
         if (hm.d.nq > 0) {
             float *p;
             UP(upload_vec(arena, &p, to_f32(hm.Aq))); v.Aq = p;
@@ -437,7 +444,6 @@ int Model::upload() {
     UP(hipMemset(ctr, 0, 2 * sizeof(int)));
     m.it = ctr;
     m.flag = ctr + 1;
-    if (!ev) UP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     // the fills and copies above run on the null stream; runners use non-blocking
     // streams, so finish them before any kernel can read this model
     UP(hipStreamSynchronize(nullptr));
@@ -449,14 +455,11 @@ int Model::upload() {
     return MFD_OK;
 }
 
-// Waits for the last run that stepped this model, whichever runner's stream it used.
-int Model::sync() {
-    if (ev) MFD_HIP(hipEventSynchronize(ev));
-    return MFD_OK;
-}
+// Device state is complete whenever no run is in flight: every Runner::run ends with a
+// synchronisation of its stream (check_flags), whichever runner stepped the model.
+int Model::sync() { return MFD_OK; }
 
 void Model::release() {
-    if (ev) { hipEventSynchronize(ev); hipEventDestroy(ev); ev = nullptr; }
     arena.release();
     uploaded = false;
 }
@@ -500,6 +503,8 @@ void Runner::release() {
     if (gexec) { hipGraphExecDestroy(gexec); gexec = nullptr; }
     if (d_models) { hipFree(d_models); d_models = nullptr; }
     if (d_tiles) { hipFree(d_tiles); d_tiles = nullptr; }
+    if (d_chunks) { hipFree(d_chunks); d_chunks = nullptr; }
+    nchunks = 0;
     for (auto e : ev) hipEventDestroy(e);
     ev.clear();
 }
@@ -571,6 +576,22 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
         maxN = std::max(maxN, m->N);
         if (m->dft_block != dft_block) { set_error("DFT block mismatch"); return MFD_ERR_ARG; }
     }
+    // cells of stateful materials get their E from the state pass on the pipelined 3D
+    // kernels and the 2D kernels; the other forms update them inline
+    split_state = (ndims == 3 && kform == 2) || ndims == 2;
+    std::vector<int4> chunks;
+    if (split_state)
+        for (size_t k = 0; k < ms.size(); k++)
+            for (size_t j = 1; j < ms[k]->state_cells.size(); j++) {
+                const int cnt = (int)ms[k]->state_cells[j].size();
+                for (int s0 = 0; s0 < cnt; s0 += 256)
+                    chunks.push_back(make_int4((int)k, (int)j, s0, std::min(cnt, s0 + 256)));
+            }
+    nchunks = (int)chunks.size();
+    if (nchunks) {
+        MFD_HIP(hipMalloc(&d_chunks, chunks.size() * sizeof(int4)));
+        MFD_HIP(hipMemcpy(d_chunks, chunks.data(), chunks.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
     src_kind_mask.assign(maxsrc, 0);
     for (Model *m : ms)
         for (size_t s2 = 0; s2 < m->srcs.size(); s2++)
@@ -615,6 +636,7 @@ int Runner::launch_step(bool timing) {
     else if (ndims == 3) launch_e3d(d_models, d_tiles, ntiles, grid, mode, any_int, stream);
     else if (ndims == 2) launch_e2d(d_models, d_tiles, ntiles, grid, mode, any_int, stream);
     else launch_e1d(d_models, d_tiles, ntiles, grid, mode, any_int, stream);
+    if (split_state && nchunks) launch_e_state(d_models, d_chunks, nchunks, ndims, any_int, stream);
     if (timing) MFD_HIP(hipEventRecord(ev[3], stream));
     for (int s = 0; s < maxsrc; s++) {
         if (src_kind_mask[s] & 1) launch_point_sources(d_models, nm, s, maxsrc_n, stream);
@@ -696,7 +718,6 @@ int Runner::run(int nsteps) {
             done += 1;
         }
     }
-    for (Model *m : models) MFD_HIP(hipEventRecord(m->ev, stream));
     return check_flags();
 }
 

@@ -100,9 +100,9 @@ struct Model {
     int ncomp_mon = 0;
     int maxsrc_n = 0, maxmon_n = 0, maxface = 0;
     std::vector<int> touched;   // cells written by sources (sorted)
+    std::vector<std::vector<int>> state_cells;   // per material: cells holding ADE/plasma state
     bool has_mats = false, has_tfsf = false, has_spectral = false;
 
-    hipEvent_t ev = nullptr;    // recorded after every run that stepped this model
     int status = 0;             // MFD_OK or the model's non-finite status
 
     int ncomp() const { return d.ndims == 3 ? 3 : d.ndims == 2 ? 2 : 1; }
@@ -134,6 +134,9 @@ struct Runner {
     bool any_mats = false, any_full = false, any_int = false, any_mon = false, any_spectral = false;
     int maxtouched = 1;
     int kform = 0;              // 3D kernel form in use
+    bool split_state = false;   // stateful cells' E computed by the state pass
+    int4 *d_chunks = nullptr;   // state-pass work list {model, material, s0, s1}
+    int nchunks = 0;
     int maxsrc = 0;
     std::vector<int> src_kind_mask;   // per slot: 1 point, 2 tfsf
     int maxsrc_n = 1, maxmon_n = 1, maxface = 1, int_grid = 1;

@@ -188,17 +188,16 @@ __global__ __launch_bounds__(256) void k_e2d(const DevModel *__restrict__ models
             d[1] = Md1 * d[1] + Md2 * (dHyx - 0.0f);
             Dx[i] = d[0]; Dz[i] = d[1];
             float e[2] = {0.0f, 0.0f};
-            if (MODE == MODE_FULL && (mt.flags & (MAT_DISP | MAT_PLASMA))) {
-                e[0] = Ex[i]; e[1] = Ez[i];
-                material_e<2>(mt, m.slot[i], d, e, m.dt, m.dtf);
-            } else if (MODE == MODE_FULL && (mt.flags & MAT_KERR)) {
+            // cells of ADE / plasma materials: E is finished by the state pass (k_e_state)
+            const bool defer = MODE == MODE_FULL && (mt.flags & (MAT_DISP | MAT_PLASMA));
+            if (MODE == MODE_FULL && !defer && (mt.flags & MAT_KERR)) {
                 material_e<2>(mt, 0, d, e, m.dt, m.dtf);
             } else {
                 e[0] = d[0] * mt.Me; e[1] = d[1] * mt.Me;
             }
-            Ex[i] = e[0]; Ez[i] = e[1];
-            check_finite(e[0], m.flag, true);
-            if (INTEG && !(idb & ID_TOUCHED)) {   // src/outputs.jl:322-325
+            if (!defer) { Ex[i] = e[0]; Ez[i] = e[1]; }
+            check_finite(e[0], m.flag, !defer);
+            if (INTEG && !defer && !(idb & ID_TOUCHED)) {   // src/outputs.jl:322-325
                 const float dt = m.dtf;
                 if (m.Sa) {
                     const float a = -e[1] * hy, b = e[0] * hy;

@@ -176,7 +176,10 @@ struct EP {
 template <int MODE, bool INTEG, int TY>
 __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ models,
                                                    const Tile *__restrict__ tiles, int ntiles) {
-    __shared__ float4 ctab[128];    // (Md1, Md2, Me, has ADE/Kerr/plasma)
+    // (Md1, Md2, Me, kind): kind 0 plain, 1 Kerr only, 2 ADE/plasma state (E of such
+    // cells is finished by the state pass, k_e_state)
+    __shared__ float4 ctab[128];
+    __shared__ float2 ktab[128];    // Kerr Pade (K2, K3)
     const int lane = threadIdx.x, ty = threadIdx.y;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile tl = tiles[t];
@@ -186,7 +189,9 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
         const int nm = MODE == MODE_VAC ? 0 : m.nmat;
         for (int k = lane + 64 * ty; k <= nm; k += 64 * TY) {
             const DevMat &mt = m.mats[k];
-            ctab[k] = make_float4(mt.Md1, mt.Md2, mt.Me, mt.flags ? 1.0f : 0.0f);
+            const float kind = (mt.flags & (MAT_DISP | MAT_PLASMA)) ? 2.0f : (mt.flags & MAT_KERR) ? 1.0f : 0.0f;
+            ctab[k] = make_float4(mt.Md1, mt.Md2, mt.Me, kind);
+            if (MODE == MODE_FULL) ktab[k] = make_float2(mt.K2, mt.K3);
         }
         __syncthreads();
         const int Nx = m.Nx, Ny = m.Ny, Nz = m.Nz;
@@ -285,20 +290,25 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
                 d2 = Md1 * d2 + Md2 * (dHyx - dHxy);
                 float e[3];
                 e[0] = d0 * cf.z; e[1] = d1 * cf.z; e[2] = d2 * cf.z;
+                bool defer = false;
                 if (MODE == MODE_FULL && cf.w != 0.0f) {
-                    const DevMat &mt = m.mats[g];
-                    const float dd[3] = {d0, d1, d2};
-                    int s = 0;
-                    if (mt.flags & (MAT_DISP | MAT_PLASMA)) {
-                        e[0] = Ex[i]; e[1] = Ey[i]; e[2] = Ez[i];
-                        s = G(m.slot)[i];
+                    if (cf.w > 1.5f) {
+                        defer = true;
+                    } else {   // Kerr Pade, src/models.jl:1140-1149
+                        const float2 kk = ktab[g];
+                        const float dd[3] = {d0, d1, d2};
+#pragma unroll
+                        for (int c = 0; c < 3; c++) {
+                            const float u = dd[c];
+                            e[c] = (1.0f + kk.x * u + 2.0f * kk.y * (u * u)) /
+                                   (1.0f + 2.0f * kk.x * u + 3.0f * kk.y * (u * u)) * u * cf.z;
+                        }
                     }
-                    material_e<3>(mt, s, dd, e, m.dt, dtf);
                 }
                 Dx[i] = d0; Dy[i] = d1; Dz[i] = d2;
-                Ex[i] = e[0]; Ey[i] = e[1]; Ez[i] = e[2];
-                check_finite(e[0], m.flag, true);
-                if (INTEG && !(P.g & ID_TOUCHED)) {   // src/outputs.jl:480-483
+                if (!defer) { Ex[i] = e[0]; Ey[i] = e[1]; Ez[i] = e[2]; }
+                check_finite(e[0], m.flag, !defer);
+                if (INTEG && !defer && !(P.g & ID_TOUCHED)) {   // src/outputs.jl:480-483
                     if (doSa) {
                         const float a = e[1] * P.hz - e[2] * P.hy, b = e[2] * P.hx - e[0] * P.hz;
                         const float cc = e[0] * P.hy - e[1] * P.hx;

@@ -46,13 +46,16 @@ __device__ __forceinline__ void material_e(const DevMat &mt, int s, const float 
     if (flags & MAT_DISP) {
         const int nq = mt.nq, cnt = mt.cnt;
         for (int q = 0; q < nq; q++) {
-            const float A = mt.Aq[q], B = mt.Bq[q], Cc = mt.Cq[q];
+            const float A = G(mt.Aq)[q], B = G(mt.Bq)[q], Cc = G(mt.Cq)[q];
 #pragma unroll
             for (int c = 0; c < NC; c++) {
-                float2 *pp = mt.P + (size_t)(q * NC + c) * cnt + s;
-                const float2 p = *pp;
+                gp<v2f> pp = (gp<v2f>)(mt.P + (size_t)(q * NC + c) * cnt + s);
+                const v2f p = *pp;
                 const float pn = A * p.x + B * p.y + Cc * e[c];
-                *pp = make_float2(pn, p.x);
+                v2f np;
+                np.x = pn;
+                np.y = p.x;
+                *pp = np;
                 sumP[c] += pn;
             }
         }
@@ -63,15 +66,19 @@ __device__ __forceinline__ void material_e(const DevMat &mt, int s, const float 
 #pragma unroll
         for (int c = 0; c < NC; c++) E2 += (double)e[c] * (double)e[c];
         const double II = mt.ksi * E2;
-        double2 rd = mt.rhod[s];
+        gp<v2d> prd = (gp<v2d>)(mt.rhod + s);
+        v2d rd = *prd;
         const float cp = (float)(mt.Cprho0 * rd.x);
         float J[NC];
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            float2 *pp = mt.Pp + (size_t)c * cnt + s;
-            const float2 p = *pp;
+            gp<v2f> pp = (gp<v2f>)(mt.Pp + (size_t)c * cnt + s);
+            const v2f p = *pp;
             const float pn = mt.Ap * p.x + mt.Bp * p.y + cp * e[c];
-            *pp = make_float2(pn, p.x);
+            v2f np;
+            np.x = pn;
+            np.y = p.x;
+            *pp = np;
             sumP[c] += pn;
             J[c] = (pn - p.y) / (2.0f * dtf);
         }
@@ -79,7 +86,7 @@ __device__ __forceinline__ void material_e(const DevMat &mt, int s, const float 
         const float ca = (float)(mt.Marho0 * rd.y * invE2);
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            float *pa = mt.Pa + (size_t)c * cnt + s;
+            gp<float> pa = G(mt.Pa) + (size_t)c * cnt + s;
             const float v = *pa + ca * e[c];
             *pa = v;
             sumP[c] += v;
@@ -95,11 +102,11 @@ __device__ __forceinline__ void material_e(const DevMat &mt, int s, const float 
         }
         rd.x = rho;
         rd.y = R1 * (1.0 - rho);
-        mt.rhod[s] = rd;
+        *prd = rd;
         float je = 0.0f;
 #pragma unroll
         for (int c = 0; c < NC; c++) je += J[c] * e[c];
-        mt.JE[s] += je * dtf;
+        G(mt.JE)[s] += je * dtf;
     }
     const float Me = mt.Me;
     if (flags & MAT_KERR) {

@@ -11,7 +11,7 @@ slab_1d                1D, Nz = 4000, dz = 40 nm, CN = 0.99, 10-cell CPML, eps =
 dispersive_3d          3D, 384 x 384 x 192 cells, dx = 5 nm, CN = 0.99, 10-cell CPML,
                        broadband TFSF pulse (700 nm carrier, 3 fs FWHM: 500-900 nm), a 6 x 6
                        array of metal spheres (seed 2027, radius U[15, 30] cells, centres
-                       jittered on a regular lattice) with a Drude pole (wp = 1.37e16,
+                       jittered on a regular lattice; one material) with a Drude pole (wp = 1.37e16,
                        gamma = 1e14) and two Lorentz poles, on a Debye substrate (deps = 2,
                        tau = 1e-14 s) filling the lowest fifth of the TFSF box; DFT plane
                        monitor of 384 x 384 points at 16 frequencies above the spheres.
@@ -104,18 +104,23 @@ def dispersive_3d(nx=384, ny=384, nz=192, steps=5000, seed=2027, dx=5e-9, pml_ce
     sub = Material(geometry=lambda X, Y, Z: (X > xb[0]) & (X < xb[1]) & (Y > yb[0]) &
                    (Y < yb[1]) & (Z > zb[0]) & (Z <= zsub),
                    eps=2.0, chi=DebyeSusceptibility(deps=2.0, tau=1e-14))
-    mats = [sub]
     pitch_x, pitch_y = (xb[1] - xb[0]) / 6, (yb[1] - yb[0]) / 6
+    spheres = []
     for i in range(6):
         for j in range(6):
-            r = rng.uniform(rmin, rmax) * dx
-            r = min(r, 0.45 * min(pitch_x, pitch_y))
-            c = (xb[0] + (i + 0.5) * pitch_x + rng.uniform(-0.05, 0.05) * pitch_x,
-                 yb[0] + (j + 0.5) * pitch_y + rng.uniform(-0.05, 0.05) * pitch_y,
-                 zsub + r + dx)
-            mats.append(Material(
-                geometry=lambda X, Y, Z, c=c, r=r: (X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2 <= r ** 2,
-                eps=1.0, chi=[drude, lor1, lor2]))
+            r = min(rng.uniform(rmin, rmax) * dx, 0.45 * min(pitch_x, pitch_y))
+            spheres.append((xb[0] + (i + 0.5) * pitch_x + rng.uniform(-0.05, 0.05) * pitch_x,
+                            yb[0] + (j + 0.5) * pitch_y + rng.uniform(-0.05, 0.05) * pitch_y,
+                            zsub + r + dx, r))
+
+    def in_spheres(X, Y, Z):
+        inside = np.zeros(np.broadcast(X, Y, Z).shape, dtype=bool)
+        for cx, cy, cz, r in spheres:
+            inside |= (X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2 <= r ** 2
+        return inside
+
+    metal = Material(geometry=in_spheres, eps=1.0, chi=[drude, lor1, lor2])
+    mats = [sub, metal]
     w0 = 2 * np.pi * C0 / 700e-9
     src = TFSFSource(box=(xb[0], xb[1], yb[0], yb[1], zb[0], zb[1]),
                      waveform=gaussian_pulse(fwhm=3e-15, t0=8e-15, w0=w0), component="Ex",
