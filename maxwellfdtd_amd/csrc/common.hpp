@@ -89,6 +89,7 @@ struct DevSource {
     const int *aux_comp, *aux_iz;
     const double *aux_coeff, *aux_wave;   // [aux_n][Nt]
     double Mh0, Me0, Md10, Md20, idz;
+    const float *auxtab;        // tabulated aux run [Nt][4][Nz]: Hx, Hy, Ex, Ey (or null)
 };
 
 // Field / spectral monitor of one model.
@@ -159,7 +160,13 @@ void launch_h1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int 
 void launch_e1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, hipStream_t s);
 void launch_integrals_touched(const DevModel *m, int nmodels, int maxn, int ndims, hipStream_t s);
 void launch_point_sources(const DevModel *m, int nmodels, int slot, int maxn, hipStream_t s);
-void launch_tfsf(const DevModel *m, int nmodels, int slot, int ndims, int maxface, hipStream_t s);
+void launch_tfsf(const DevModel *m, int nmodels, int slot, int ndims, int maxface, hipStream_t s,
+                 bool aux);
+void launch_tfsf_aux(const DevModel *m, int nmodels, int slot, hipStream_t s);
+void launch_tfsf_gather(const DevModel *m, int nmodels, int slot, int maxn, bool integ,
+                        hipStream_t s);
+void launch_tfsf_aux_table(const DevSource *S, int Nz, int Nt, int bcl, int bcr, float *tab,
+                           hipStream_t s);
 void launch_monitors(const DevModel *m, int nmodels, int maxn, hipStream_t s);
 void launch_dft_flush(const DevModel *m, int nmodels, int maxn, int nk, hipStream_t s);
 void launch_finish(const DevModel *m, int nmodels, hipStream_t s);

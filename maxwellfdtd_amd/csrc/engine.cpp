@@ -390,12 +390,23 @@ int Model::upload() {
             UP(upload_vec(arena, &dp, hs.aux_wave)); v.aux_wave = dp;
             v.Mh0 = d.Mh; v.Me0 = d.Me; v.Md10 = d.Md1; v.Md20 = d.Md2;
             v.idz = 1.0 / d.dz;
+            const size_t tab_n = (size_t)d.Nt * 4 * Nz;
+            if (tab_n * sizeof(float) <= ((size_t)1 << 30)) {
+                float *tab;
+                UP(arena.alloc(&tab, tab_n));
+                v.auxtab = tab;
+            }
         }
     }
     DevSource *dsrc;
     UP(upload_vec(arena, &dsrc, hsrc));
     m.src = dsrc;
     m.nsrc = (int)srcs.size();
+    for (size_t k = 0; k < hsrc.size(); k++)
+        if (hsrc[k].auxtab) {
+            launch_tfsf_aux_table(dsrc + k, d.Nz, d.Nt, d.bcs[4], d.bcs[5], (float *)hsrc[k].auxtab, nullptr);
+            UP(hipGetLastError());
+        }
     // monitors
     int comps[6];
     ncomp_mon = monitor_comps(d.ndims, comps);
@@ -592,6 +603,14 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
         MFD_HIP(hipMalloc(&d_chunks, chunks.size() * sizeof(int4)));
         MFD_HIP(hipMemcpy(d_chunks, chunks.data(), chunks.size() * sizeof(int4), hipMemcpyHostToDevice));
     }
+    // sources: a model whose only source is a TFSF source gets its face corrections and
+    // its touched cells' integrals from one gather launch (3D)
+    aux_needed = false;
+    tfsf_gather = ndims == 3;
+    for (Model *m : ms) {
+        tfsf_gather &= m->srcs.size() == 1 && m->srcs[0].type == 1;
+        for (const DevSource &v : m->hsrc) aux_needed |= v.kind == SRC_TFSF && v.mode == 1 && !v.auxtab;
+    }
     src_kind_mask.assign(maxsrc, 0);
     for (Model *m : ms)
         for (size_t s2 = 0; s2 < m->srcs.size(); s2++)
@@ -638,12 +657,17 @@ int Runner::launch_step(bool timing) {
     else launch_e1d(d_models, d_tiles, ntiles, grid, mode, any_int, stream);
     if (split_state && nchunks) launch_e_state(d_models, d_chunks, nchunks, ndims, any_int, stream);
     if (timing) MFD_HIP(hipEventRecord(ev[3], stream));
-    for (int s = 0; s < maxsrc; s++) {
-        if (src_kind_mask[s] & 1) launch_point_sources(d_models, nm, s, maxsrc_n, stream);
-        if (src_kind_mask[s] & 2) launch_tfsf(d_models, nm, s, ndims, maxface, stream);
+    if (tfsf_gather) {
+        if (aux_needed) launch_tfsf_aux(d_models, nm, 0, stream);
+        launch_tfsf_gather(d_models, nm, 0, maxtouched, any_int, stream);
+    } else {
+        for (int s = 0; s < maxsrc; s++) {
+            if (src_kind_mask[s] & 1) launch_point_sources(d_models, nm, s, maxsrc_n, stream);
+            if (src_kind_mask[s] & 2) launch_tfsf(d_models, nm, s, ndims, maxface, stream, aux_needed);
+        }
     }
     if (any_mon) launch_monitors(d_models, nm, maxmon_n, stream);
-    if (any_int) launch_integrals_touched(d_models, nm, maxtouched, ndims, stream);
+    if (any_int && !tfsf_gather) launch_integrals_touched(d_models, nm, maxtouched, ndims, stream);
     launch_finish(d_models, nm, stream);
     MFD_HIP(hipGetLastError());
     if (timing) {

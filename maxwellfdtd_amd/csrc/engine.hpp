@@ -135,6 +135,8 @@ struct Runner {
     int maxtouched = 1;
     int kform = 0;              // 3D kernel form in use
     bool split_state = false;   // stateful cells' E computed by the state pass
+    bool tfsf_gather = false;   // TFSF corrections + touched integrals in one launch
+    bool aux_needed = false;    // some TFSF aux run is stepped on the device each step
     int4 *d_chunks = nullptr;   // state-pass work list {model, material, s0, s1}
     int nchunks = 0;
     int maxsrc = 0;

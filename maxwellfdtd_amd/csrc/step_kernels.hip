@@ -6,6 +6,8 @@
 #include "common.hpp"
 #include "material.hpp"
 
+#include <algorithm>
+
 namespace mfd {
 
 constexpr double MU0_ = 1.25663706212e-6;
@@ -42,20 +44,11 @@ void launch_point_sources(const DevModel *m, int nmodels, int slot, int maxn, hi
 // One step of the x/y-uniform record run in FP64 (the transverse differences of the 3D
 // update vanish, so update_H_kernel!/update_E_kernel! 3D reduce to these z recursions,
 // src/models.jl:735-1162): H half, E half, then the aux soft sources.
-__global__ __launch_bounds__(256) void k_tfsf_aux(const DevModel *__restrict__ models,
-                                                  int slot) {
-    const DevModel &m = models[blockIdx.x];
-    if (slot >= m.nsrc) return;
-    const DevSource &S = m.src[slot];
-    if (S.kind != SRC_TFSF || S.mode != 1) return;
-    const int it0 = *m.it;
-    if (it0 >= m.Nt) return;
-    const int Nz = m.Nz;
+__device__ void aux_advance(const DevSource &S, int Nz, int Nt, int bcl, int bcr, int it0) {
     double *Hx = S.aux, *Hy = S.aux + Nz, *Dx = S.aux + 2 * Nz, *Dy = S.aux + 3 * Nz;
     double *Ex = S.aux + 4 * Nz, *Ey = S.aux + 5 * Nz;
     const int nbm = S.aux_nbmax;
     const double Mh = S.Mh0, Me = S.Me0, Md1 = S.Md10, Md2 = S.Md20, dz = 1.0 / S.idz;
-    const int bcl = m.bcs[4], bcr = m.bcs[5];
     for (int iz = threadIdx.x; iz < Nz; iz += blockDim.x) {
         double ex1, ey1;
         if (iz + 1 < Nz) { ex1 = Ex[iz + 1]; ey1 = Ey[iz + 1]; }
@@ -121,7 +114,7 @@ __global__ __launch_bounds__(256) void k_tfsf_aux(const DevModel *__restrict__ m
     if (threadIdx.x == 0) {
         for (int k = 0; k < S.aux_n; k++) {
             const int iz = S.aux_iz[k];
-            const double v = S.aux_coeff[k] * S.aux_wave[(size_t)k * m.Nt + it0];
+            const double v = S.aux_coeff[k] * S.aux_wave[(size_t)k * Nt + it0];
             double *dst;
             switch (S.aux_comp[k]) {
                 case HX: dst = Hx; break;
@@ -134,19 +127,60 @@ __global__ __launch_bounds__(256) void k_tfsf_aux(const DevModel *__restrict__ m
             dst[iz] = dst[iz] + v;
         }
     }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_tfsf_aux(const DevModel *__restrict__ models,
+                                                  int slot) {
+    const DevModel &m = models[blockIdx.x];
+    if (slot >= m.nsrc) return;
+    const DevSource &S = m.src[slot];
+    if (S.kind != SRC_TFSF || S.mode != 1 || S.auxtab) return;
+    const int it0 = *m.it;
+    if (it0 >= m.Nt) return;
+    aux_advance(S, m.Nz, m.Nt, m.bcs[4], m.bcs[5], it0);
+}
+
+// The whole auxiliary run at once: the incident field does not depend on the model's
+// fields, so its Hx, Hy, Ex, Ey after every step are tabulated (FP32, the precision the
+// face corrections use) into tab[Nt][4][Nz] before the first step.
+__global__ __launch_bounds__(256) void k_tfsf_aux_table(const DevSource *__restrict__ src,
+                                                        int Nz, int Nt, int bcl, int bcr,
+                                                        float *__restrict__ tab) {
+    const DevSource &S = *src;
+    for (int it0 = 0; it0 < Nt; it0++) {
+        aux_advance(S, Nz, Nt, bcl, bcr, it0);
+        float *row = tab + (size_t)it0 * 4 * Nz;
+        for (int iz = threadIdx.x; iz < Nz; iz += blockDim.x) {
+            row[iz] = (float)S.aux[iz];
+            row[Nz + iz] = (float)S.aux[Nz + iz];
+            row[2 * Nz + iz] = (float)S.aux[4 * Nz + iz];
+            row[3 * Nz + iz] = (float)S.aux[5 * Nz + iz];
+        }
+        __syncthreads();
+    }
+}
+
+void launch_tfsf_aux_table(const DevSource *S, int Nz, int Nt, int bcl, int bcr, float *tab,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_tfsf_aux_table, dim3(1), dim3(256), 0, s, S, Nz, Nt, bcl, bcr, tab);
 }
 
 // ---- TFSF face corrections: add_source!(::TFSFSource3D) (src/sources.jl:713-776) -------
 // Face f = 2*axis + side.  (u, v) are the tangential axes (x face: y,z; y face: x,z;
 // z face: x,y); the face tables are H_u, H_v, E_u, E_v.  1D and 2D models use faces
 // 4,5 (resp. 0,1,4,5) with degenerate tangential ranges.
-__device__ inline double aux_value(const DevSource &S, int Nz, int comp, int z) {
+__device__ inline float aux_value(const DevSource &S, int Nz, int comp, int z, int it0) {
+    if (S.auxtab) {
+        const int k = comp == HX ? 0 : comp == HY ? 1 : comp == EX ? 2 : 3;
+        return G(S.auxtab)[((size_t)it0 * 4 + k) * Nz + z];
+    }
     switch (comp) {
-        case HX: return S.aux[z];
-        case HY: return S.aux[Nz + z];
-        case EX: return S.aux[4 * Nz + z];
-        case EY: return S.aux[5 * Nz + z];
-        default: return 0.0;
+        case HX: return (float)S.aux[z];
+        case HY: return (float)S.aux[Nz + z];
+        case EX: return (float)S.aux[4 * Nz + z];
+        case EY: return (float)S.aux[5 * Nz + z];
+        default: return 0.0f;
     }
 }
 
@@ -185,7 +219,7 @@ __global__ __launch_bounds__(256) void k_tfsf_axis(const DevModel *__restrict__ 
             int z;
             if (axis == 2) z = side ? N2 : (k < 2 ? N1 - 1 : N1);
             else z = v;
-            return (float)aux_value(S, m.Nz, tabc[k], z);
+            return aux_value(S, m.Nz, tabc[k], z, it0);
         }
         const float *T = S.tab[4 * f + k];
         if (!T) return 0.0f;
@@ -234,9 +268,117 @@ __global__ __launch_bounds__(256) void k_tfsf_axis(const DevModel *__restrict__ 
     }
 }
 
-void launch_tfsf(const DevModel *m, int nmodels, int slot, int ndims, int maxface,
-                 hipStream_t s) {
+// ---- TFSF corrections and integral variables of the touched cells, 3D ---------------
+// One lane per cell of the model's touched list (every cell a TFSF face correction can
+// write).  The lane applies every correction of source `slot` that lands on its cell, in
+// the reference's face order (x, y, z; lower side, upper side) and with the statements
+// of k_tfsf_axis, then accumulates the cell's Sa/E2 (which the E kernel skipped) and
+// checks Ex.  Used when the TFSF source is a model's only source.
+__global__ __launch_bounds__(256) void k_tfsf_gather(const DevModel *__restrict__ models,
+                                                     int slot, int integ) {
+    const DevModel &m = models[blockIdx.y];
+    const int it0 = *m.it;
+    if (it0 >= m.Nt) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = k < m.ntouched;
+    float exf = 0.0f;
+    if (active) {
+        const DevSource &S = m.src[slot];
+        const unsigned i = (unsigned)G(m.touched)[k];
+        const int Nx = m.Nx, Ny = m.Ny;
+        const int c3[3] = {(int)(i % (unsigned)Nx), (int)((i / (unsigned)Nx) % (unsigned)Ny),
+                           (int)(i / ((unsigned)Nx * (unsigned)Ny))};
+        float f[9];
+#pragma unroll
+        for (int c = 0; c < 9; c++) f[c] = G((const float *)m.F[c])[i];
+        unsigned dirty = 0;
+#pragma unroll
+        for (int axis = 0; axis < 3; axis++) {
+            const int ua = axis == 0 ? 1 : 0, va = axis == 2 ? 1 : 2;
+            const int U1 = S.box[2 * ua], U2 = S.box[2 * ua + 1];
+            const int V1 = S.box[2 * va], V2 = S.box[2 * va + 1];
+            const int N1 = S.box[2 * axis], N2 = S.box[2 * axis + 1];
+            const int u = c3[ua], v = c3[va], n = c3[axis];
+            if (u < U1 || u > U2 || v < V1 || v > V2) continue;
+            const int Nu = U2 - U1, Nv = V2 - V1;
+            constexpr int cH[3][2] = {{HY, HZ}, {HX, HZ}, {HX, HY}};
+            constexpr int cE[3][2] = {{EY, EZ}, {EX, EZ}, {EX, EY}};
+            constexpr int cD[3][2] = {{DY, DZ}, {DX, DZ}, {DX, DY}};
+            const int cHu = cH[axis][0], cHv = cH[axis][1], cEu = cE[axis][0], cEv = cE[axis][1];
+            const int cDu = cD[axis][0], cDv = cD[axis][1];
+            const float cm = S.cm[axis], ce = S.ce[axis], cd = S.cd[axis];
+#pragma unroll
+            for (int side = 0; side < 2; side++) {
+                const int nH = side ? N2 : N1 - 1, nE = side ? N2 : N1;
+                if (n != nH && n != nE) continue;
+                const int f_ = 2 * axis + side;
+                const float sg = side ? -1.0f : 1.0f;
+                const float s0 = (axis == 1 ? 1.0f : -1.0f) * sg;
+                const int tabc[4] = {cHu, cHv, cEu, cEv};
+                auto live = [&](int kk) { return S.mode == 0 || (tabc[kk] != HZ && tabc[kk] != EZ); };
+                auto inc = [&](int kk) -> float {
+                    if (S.mode == 1) {
+                        int z;
+                        if (axis == 2) z = side ? N2 : (kk < 2 ? N1 - 1 : N1);
+                        else z = v;
+                        return aux_value(S, m.Nz, tabc[kk], z, it0);
+                    }
+                    const float *T = S.tab[4 * f_ + kk];
+                    if (!T) return 0.0f;
+                    const int na = (kk == 0 || kk == 3) ? Nu + 1 : Nu;
+                    const int nb = (kk == 0 || kk == 3) ? Nv : Nv + 1;
+                    return G(T)[(u - U1) + (size_t)na * ((v - V1) + (size_t)nb * it0)];
+                };
+                if (n == nH) {
+                    if (u <= U2 && v <= V2 - 1 && live(3)) { f[cHu] = f[cHu] + (s0 * cm) * inc(3); dirty |= 1u << cHu; }
+                    if (u <= U2 - 1 && v <= V2 && live(2)) { f[cHv] = f[cHv] + (-s0 * cm) * inc(2); dirty |= 1u << cHv; }
+                }
+                if (n == nE) {
+                    if (u <= U2 - 1 && v <= V2 && live(1)) {
+                        const float iv = inc(1);
+                        f[cEu] = f[cEu] + (-s0 * ce) * iv;
+                        f[cDu] = f[cDu] + (-s0 * cd) * iv;
+                        dirty |= (1u << cEu) | (1u << cDu);
+                    }
+                    if (u <= U2 && v <= V2 - 1 && live(0)) {
+                        const float iv = inc(0);
+                        f[cEv] = f[cEv] + (s0 * ce) * iv;
+                        f[cDv] = f[cDv] + (s0 * cd) * iv;
+                        dirty |= (1u << cEv) | (1u << cDv);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 9; c++)
+            if (dirty & (1u << c)) G(m.F[c])[i] = f[c];
+        if (integ) {   // src/outputs.jl:475-485
+            const float dt = m.dtf;
+            if (m.Sa) {
+                const float a = f[EY] * f[HZ] - f[EZ] * f[HY], b = f[EZ] * f[HX] - f[EX] * f[HZ];
+                const float cc = f[EX] * f[HY] - f[EY] * f[HX];
+                G(m.Sa)[i] += sqrtf(a * a + b * b + cc * cc) * dt;
+            }
+            if (m.E2) G(m.E2)[i] += (f[EX] * f[EX] + f[EY] * f[EY] + f[EZ] * f[EZ]) * dt;
+        }
+        exf = f[EX];
+    }
+    check_finite(exf, m.flag, active);
+}
+
+void launch_tfsf_gather(const DevModel *m, int nmodels, int slot, int maxn, bool integ,
+                        hipStream_t s) {
+    const int nb = std::max(1, (maxn + 255) / 256);
+    hipLaunchKernelGGL(k_tfsf_gather, dim3(nb, nmodels), dim3(256), 0, s, m, slot, integ ? 1 : 0);
+}
+
+void launch_tfsf_aux(const DevModel *m, int nmodels, int slot, hipStream_t s) {
     hipLaunchKernelGGL(k_tfsf_aux, dim3(nmodels), dim3(256), 0, s, m, slot);
+}
+
+void launch_tfsf(const DevModel *m, int nmodels, int slot, int ndims, int maxface,
+                 hipStream_t s, bool aux) {
+    if (aux) launch_tfsf_aux(m, nmodels, slot, s);
     const int nb = (2 * maxface + 255) / 256;
     for (int axis = 0; axis < 3; axis++) {
         if (ndims == 1 && axis < 2) continue;
