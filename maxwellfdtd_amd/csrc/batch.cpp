@@ -43,11 +43,12 @@ std::vector<RunOptions> candidates(int ndims, const RunOptions &base) {
                 out.push_back(o);
             }
     } else if (ndims == 2) {
-        for (int zc : {8, 16, 32, 64, 128})
-            for (int p : {0, 4}) {
+        for (int zc : {8, 16, 32, 64})
+            for (int tw : {1, 2, 4}) {
                 RunOptions o = base;
                 o.zchunk = zc;
-                o.persistent = p;
+                o.ty = tw;          // waves (64 x-columns each) per 2D tile
+                o.persistent = 0;
                 out.push_back(o);
             }
     } else {

@@ -144,7 +144,7 @@ __host__ __device__ inline int tile_z1(const Tile &t) { return (t.w >> 16) & 0xf
 // tile geometry per dimensionality
 constexpr int TX3 = 64, TY3 = 4;    // 3D scalar kernels: 64 x 4 columns, marching z
 constexpr int TX3V = 256;           // 3D vector kernels: 256 x (4 per lane) x TY rows
-constexpr int TX2 = 256;            // 2D: 256 x-columns, marching z
+constexpr int TX2 = 64;             // 2D: 64 x-columns per wave, TW waves per tile, marching z
 constexpr int TX1 = 256;            // 1D: 256 cells
 
 // launchers (defined in the .hip files)
@@ -154,8 +154,8 @@ void launch_h3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, in
 void launch_e3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int ty, hipStream_t s);
 void launch_h3d_v4(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int ty, hipStream_t s);
 void launch_e3d_v4(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int ty, hipStream_t s);
-void launch_h2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, hipStream_t s);
-void launch_e2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, hipStream_t s);
+void launch_h2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int tw, hipStream_t s);
+void launch_e2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int tw, hipStream_t s);
 void launch_h1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, hipStream_t s);
 void launch_e1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, hipStream_t s);
 void launch_integrals_touched(const DevModel *m, int nmodels, int maxn, int ndims, hipStream_t s);

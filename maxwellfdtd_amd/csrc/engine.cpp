@@ -535,7 +535,7 @@ static void make_tiles(const Model &m, int idx, int zchunk, int ty, int kform, s
                     out.push_back(make_int4(idx, x0, y0, z0 | (std::min(Nz, z0 + zchunk) << 16)));
     } else if (m.d.ndims == 2) {
         for (int z0 = 0; z0 < Nz; z0 += zchunk)
-            for (int x0 = 0; x0 < Nx; x0 += TX2)
+            for (int x0 = 0; x0 < Nx; x0 += TX2 * ty)
                 out.push_back(make_int4(idx, x0, 0, z0 | (std::min(Nz, z0 + zchunk) << 16)));
     } else {
         for (int z0 = 0; z0 < Nz; z0 += TX1)
@@ -647,13 +647,13 @@ int Runner::launch_step(bool timing) {
     if (ndims == 3 && kform == 1) launch_h3d_v4(d_models, d_tiles, ntiles, grid, mode, opt.ty, stream);
     else if (ndims == 3 && kform == 2) launch_h3d_p(d_models, d_tiles, ntiles, grid, mode, opt.ty, stream);
     else if (ndims == 3) launch_h3d(d_models, d_tiles, ntiles, grid, mode, stream);
-    else if (ndims == 2) launch_h2d(d_models, d_tiles, ntiles, grid, mode, stream);
+    else if (ndims == 2) launch_h2d(d_models, d_tiles, ntiles, grid, mode, opt.ty, stream);
     else launch_h1d(d_models, d_tiles, ntiles, grid, mode, stream);
     if (timing) { MFD_HIP(hipEventRecord(ev[1], stream)); MFD_HIP(hipEventRecord(ev[2], stream)); }
     if (ndims == 3 && kform == 1) launch_e3d_v4(d_models, d_tiles, ntiles, grid, mode, any_int, opt.ty, stream);
     else if (ndims == 3 && kform == 2) launch_e3d_p(d_models, d_tiles, ntiles, grid, mode, any_int, opt.ty, stream);
     else if (ndims == 3) launch_e3d(d_models, d_tiles, ntiles, grid, mode, any_int, stream);
-    else if (ndims == 2) launch_e2d(d_models, d_tiles, ntiles, grid, mode, any_int, stream);
+    else if (ndims == 2) launch_e2d(d_models, d_tiles, ntiles, grid, mode, any_int, opt.ty, stream);
     else launch_e1d(d_models, d_tiles, ntiles, grid, mode, any_int, stream);
     if (split_state && nchunks) launch_e_state(d_models, d_chunks, nchunks, ndims, any_int, stream);
     if (timing) MFD_HIP(hipEventRecord(ev[3], stream));
