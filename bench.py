@@ -218,6 +218,7 @@ def percentiles(lat):
 
 
 def bench_batched_2d(args, lib):
+    from maxwellfdtd_amd import _lib
     from maxwellfdtd_amd.batch import Batch
     from maxwellfdtd_amd.models import Handle
     from maxwellfdtd_amd.workloads import batched_2d
@@ -240,12 +241,21 @@ def bench_batched_2d(args, lib):
     log(f"autotune: {tune}")
     done_t = np.zeros(len(descs))
     while b.active > 0:
-        b.run(args.chunk)
+        try:
+            b.run(args.chunk)
+        except _lib.EngineError as e:      # a solve went non-finite; the others go on
+            log(f"batch: {e}")
         now = time.perf_counter() - t0
         for i, h in enumerate(b.handles):
-            if done_t[i] == 0 and h.it >= descs[i].Nt:
+            if done_t[i] == 0 and (h.it >= descs[i].Nt or h.status != 0):
                 done_t[i] = now
+    failed = [i for i, h in enumerate(b.handles) if h.status != 0]
+    if failed:
+        log(f"non-finite solves: {failed} dims {[descs[i].dims for i in failed]} "
+            f"Nt {[descs[i].Nt for i in failed]} it {[b.handles[i].it for i in failed]}")
     wall_b = time.perf_counter() - t0
+    done_steps = np.array([h.it for h in b.handles], dtype=float)
+    work = float(np.sum(cells * done_steps))
     g_batch = work / wall_b / 1e9
     p50_b, p99_b = percentiles(done_t * 1e3)
     log(f"batched: {wall_b:.2f} s ({t_build:.1f} s host build+copy outside), {g_batch:.2f} Gcell/s, "
@@ -257,15 +267,21 @@ def bench_batched_2d(args, lib):
     busy = 0.0
     lat_p = []
     nplain = len(descs) if args.plain_count <= 0 else min(args.plain_count, len(descs))
-    for d in descs[:nplain]:
+    work_p = 0.0
+    for j, d in enumerate(descs[:nplain]):
+        if j % 32 == 0:
+            log(f"plain: solve {j} of {nplain}, {busy:.1f} s so far")
         h = Handle(d, lib)
         h.lib.check(h.lib.model_upload(h.h), "upload")
         t1 = time.perf_counter()
-        h.step(d.Nt)
+        try:
+            h.step(d.Nt)
+        except _lib.EngineError as e:
+            log(f"plain: {e}")
         busy += time.perf_counter() - t1
         lat_p.append((time.perf_counter() - t_start) * 1e3)
+        work_p += float(np.prod(d.dims)) * h.it
         h.close()
-    work_p = float(np.sum(cells[:nplain] * np.array([d.Nt for d in descs[:nplain]])))
     g_plain = work_p / busy / 1e9
     p50_p, p99_p = percentiles(lat_p)
     log(f"plain ({nplain} solves): {busy:.2f} s GPU-busy, {g_plain:.2f} Gcell/s, p50 {p50_p:.0f} ms "
@@ -284,6 +300,7 @@ def bench_batched_2d(args, lib):
                     "p50_ms": round(p50_p, 1), "p99_ms": round(p99_p, 1),
                     "latency": "queue of all solves at t=0 run one after another"}
     out["speedup_vs_plain"] = round(g_batch / g_plain, 3)
+    out["nonfinite_solves"] = failed
     out["roofline"] = None
     return out
 
@@ -302,7 +319,7 @@ def bench_served_3d(args, lib):
     log(f"{len(reqs)} requests built in {time.perf_counter()-t0:.1f}s: {work/1e12:.3f} T cell-updates, "
         f"arrivals over {reqs[-1][0]:.2f} s")
 
-    srv = Server(options={8: args.quantum})
+    srv = Server(options={8: args.quantum, 9: args.fill})
     t0 = time.perf_counter()
     replay(srv, reqs, describe=lambda d: d)
     wall_s = time.perf_counter() - t0
@@ -324,20 +341,25 @@ def bench_served_3d(args, lib):
         clock = start + svc
         busy += svc
         lat_p.append((clock - t_arr) * 1e3)
-    g_plain = work / busy / 1e9
+    span = clock - reqs[0][0]
+    g_plain = work / span / 1e9
     p50_p, p99_p = percentiles(lat_p)
-    log(f"plain: {busy:.2f} s busy, {g_plain:.2f} Gcell/s, p50 {p50_p:.0f} ms p99 {p99_p:.0f} ms")
+    log(f"plain: {busy:.2f} s busy over a {span:.2f} s span, {g_plain:.2f} Gcell/s, "
+        f"p50 {p50_p:.0f} ms p99 {p99_p:.0f} ms")
     out = base_record(args, st["gcells_per_s"], wall_s, 0)
     out["config"] = {"workload": f"served_3d: first {len(reqs)} requests of the Poisson trace "
                                  "(20 req/s), N^3 with N in 64..320, Kerr + 6-photon plasma "
                                  "sphere, TFSF 800 nm, steps 1000..4000"
                                  + (f" x {args.step_scale}" if args.step_scale != 1.0 else ""),
-                     "requests": len(reqs), "cell_updates": work, "quantum_steps": args.quantum}
+                     "requests": len(reqs), "cell_updates": work, "quantum_steps": args.quantum,
+                     "fill_mcells": args.fill}
     out["served"] = {"gcells_per_s": round(st["gcells_per_s"], 3), "p50_ms": round(st["p50_ms"], 1),
                      "p99_ms": round(st["p99_ms"], 1), "mean_ms": round(st["mean_ms"], 1),
                      "done": st["done"], "failed": st["failed"], "busy_s": round(st["busy_s"], 3)}
     out["plain"] = {"gcells_per_s": round(g_plain, 3), "p50_ms": round(p50_p, 1),
-                    "p99_ms": round(p99_p, 1), "busy_s": round(busy, 3)}
+                    "p99_ms": round(p99_p, 1), "busy_s": round(busy, 3), "span_s": round(span, 3),
+                    "throughput": "cell-updates / (last completion - first arrival), "
+                                  "as for the server"}
     out["speedup_vs_plain"] = round(st["gcells_per_s"] / g_plain, 3) if g_plain else None
     out["roofline"] = None
     return out
@@ -388,6 +410,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=200)
     ap.add_argument("--plain-count", type=int, default=0)
     ap.add_argument("--quantum", type=int, default=64)
+    ap.add_argument("--fill", type=int, default=48, help="server: active cells bound, Mi-cells")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU z-slab sharding is "

@@ -171,6 +171,9 @@ int32_t mfd_model_upload(mfd_model *m);
 int32_t mfd_model_step(mfd_model *m, int32_t nsteps);
 /* Number of completed steps.                                                          */
 int32_t mfd_model_get_it(mfd_model *m, int32_t *it);
+/* MFD_OK, or MFD_ERR_NAN / MFD_ERR_INF once the model's Ex became non-finite (a model
+ * in that state is not stepped further by batches or the server).                    */
+int32_t mfd_model_get_status(mfd_model *m, int32_t *status);
 
 /* ---- results ----------------------------------------------------------------------- */
 /* Copy one field component (Nx*Ny*Nz values) to host FP64.                            */
@@ -196,7 +199,8 @@ enum {
     MFD_OPT_ROWS = 5,          /* y rows per 3D workgroup (1, 2, 4, 8)                     */
     MFD_OPT_KERNEL_FORM = 6,   /* 3D kernel form: 0 scalar, 1 4-cell vector, 2 pipelined   */
     MFD_OPT_STAGGER = 7,       /* byte offset between field arrays (before upload)         */
-    MFD_OPT_QUANTUM = 8        /* server: steps per scheduling round (default 64)          */
+    MFD_OPT_QUANTUM = 8,       /* server: steps per scheduling round (default 64)          */
+    MFD_OPT_FILL_MCELLS = 9    /* server: admit while active cells < value * 2^20 (48)     */
 };
 /* Changing an option rebuilds the launch plan; field state is kept.                   */
 int32_t mfd_model_set_options(mfd_model *m, int32_t opt, int32_t value);
@@ -236,9 +240,10 @@ int32_t mfd_batch_destroy(mfd_batch *b);
 /* ---- serving loop (no counterpart in the reference) ----------------------------------
  * A server owns a worker thread and a stream.  mfd_server_submit queues a built,
  * not-yet-stepped model; the worker admits queued models in arrival order while their
- * device footprint fits the HBM budget (and free HBM), uploads them, and advances the
- * active set together in rounds of MFD_OPT_QUANTUM steps, so requests that arrive while
- * others run join between rounds.  A model that finishes leaves the tile list before it
+ * device footprint fits the HBM budget (and free HBM) and the active set holds fewer
+ * than MFD_OPT_FILL_MCELLS * 2^20 cells (enough to fill the GPU), uploads them, and
+ * advances the active set together in rounds of MFD_OPT_QUANTUM steps, so requests that
+ * arrive while others run join between rounds.  A model that finishes leaves the tile list before it
  * is reported done; its device state stays until the caller destroys the model.
  * Latency is submit -> done on the host clock.                                         */
 enum { MFD_REQ_QUEUED = 0, MFD_REQ_RUNNING = 1, MFD_REQ_DONE = 2, MFD_REQ_FAILED = 3 };
@@ -253,7 +258,8 @@ typedef struct mfd_server_stats {
 typedef struct mfd_server mfd_server;
 /* hbm_budget <= 0: free HBM minus 2 GiB.  max_active bounds the active set.           */
 int32_t mfd_server_create(int64_t hbm_budget, int32_t max_active, mfd_server **out);
-/* MFD_OPT_QUANTUM, GRAPH_STEPS, ZCHUNK, PERSISTENT, ROWS, KERNEL_FORM; before submits. */
+/* MFD_OPT_QUANTUM, FILL_MCELLS, GRAPH_STEPS, ZCHUNK, PERSISTENT, ROWS, KERNEL_FORM;
+ * before the first submit.                                                            */
 int32_t mfd_server_set_options(mfd_server *s, int32_t opt, int32_t value);
 int32_t mfd_server_submit(mfd_server *s, mfd_model *m, int64_t *ticket);
 /* state = MFD_REQ_*; status = the request's MFD_OK / MFD_ERR_* once finished.          */

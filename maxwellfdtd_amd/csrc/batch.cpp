@@ -211,6 +211,7 @@ struct Request {
     int64_t ticket = 0;
     mfd_model *h = nullptr;
     int state = MFD_REQ_QUEUED;
+    int state_before_finish = MFD_REQ_QUEUED;
     int status = MFD_OK;
     int64_t fp = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
@@ -223,6 +224,8 @@ struct mfd_server {
     int64_t budget = 0;
     int max_active = 64;
     int quantum = 64;
+    long long fill_cells = 48LL << 20;   // admit while the active set is below this size
+    long long active_cells = 0;
     int ndims = 0;
     RunOptions opt;
     int device = 0;
@@ -252,12 +255,14 @@ struct mfd_server {
 
 void mfd_server::finish(Request *r, int state, int status, const std::string &err) {
     std::lock_guard<std::mutex> lk(mu);
+    r->state_before_finish = r->state;
     r->state = state;
     r->status = status;
     r->err = err;
     r->t_done = now_s();
     r->h->owner = nullptr;
     in_use -= r->fp;
+    if (r->state_before_finish == MFD_REQ_RUNNING) active_cells -= r->h->m.N;
     if (state == MFD_REQ_DONE) {
         latencies_ms.push_back((r->t_done - r->t_submit) * 1e3);
         cell_updates += (double)r->h->m.N * r->h->m.d.Nt;
@@ -292,8 +297,12 @@ void mfd_server::loop() {
                     continue;
                 }
                 if (in_use + r->fp > budget || r->fp + reserve > (int64_t)free_b) break;
+                // enough cells in flight to fill the GPU: later requests wait, so that
+                // requests finish close to arrival order instead of sharing the GPU
+                if (!(active.empty() && admitted.empty()) && active_cells + r->h->m.N > fill_cells) break;
                 queue.pop_front();
                 in_use += r->fp;
+                active_cells += r->h->m.N;
                 free_b -= (size_t)std::min<int64_t>((int64_t)free_b, r->fp);
                 r->state = MFD_REQ_RUNNING;
                 if (t_first < 0) t_first = r->t_submit;
@@ -393,6 +402,11 @@ int32_t mfd_server_set_options(mfd_server *s, int32_t opt, int32_t value) {
     if (opt == MFD_OPT_QUANTUM) {
         if (value < 1) { set_error("quantum must be >= 1"); return MFD_ERR_ARG; }
         s->quantum = value;
+        return MFD_OK;
+    }
+    if (opt == MFD_OPT_FILL_MCELLS) {
+        if (value < 1) { set_error("fill must be >= 1 Mi-cells"); return MFD_ERR_ARG; }
+        s->fill_cells = (long long)value << 20;
         return MFD_OK;
     }
     if (opt == MFD_OPT_DFT_BLOCK || opt == MFD_OPT_STAGGER || opt == MFD_OPT_KERNEL_TIMING) {

@@ -170,6 +170,7 @@ void launch_tfsf_aux_table(const DevSource *S, int Nz, int Nt, int bcl, int bcr,
 void launch_monitors(const DevModel *m, int nmodels, int maxn, hipStream_t s);
 void launch_dft_flush(const DevModel *m, int nmodels, int maxn, int nk, hipStream_t s);
 void launch_finish(const DevModel *m, int nmodels, hipStream_t s);
+void launch_collect_flags(const DevModel *m, int nmodels, int *out, hipStream_t s);
 // E of the cells of stateful (ADE / plasma) materials, after the E half-step kernel
 // updated their D: one work item per state slot, chunks {model, material, s0, s1}.
 void launch_e_state(const DevModel *m, const int4 *chunks, int nchunks, int ndims, bool integ,

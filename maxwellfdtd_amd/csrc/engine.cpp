@@ -515,6 +515,8 @@ void Runner::release() {
     if (d_models) { hipFree(d_models); d_models = nullptr; }
     if (d_tiles) { hipFree(d_tiles); d_tiles = nullptr; }
     if (d_chunks) { hipFree(d_chunks); d_chunks = nullptr; }
+    if (d_flags) { hipFree(d_flags); d_flags = nullptr; }
+    if (h_flags) { hipHostFree(h_flags); h_flags = nullptr; }
     nchunks = 0;
     for (auto e : ev) hipEventDestroy(e);
     ev.clear();
@@ -617,6 +619,8 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
             src_kind_mask[s2] |= m->srcs[s2].type == 0 ? 1 : 2;
     int_grid = (int)std::min<long long>((maxN + 255) / 256, (long long)cu_count * 16);
     MFD_HIP(hipMalloc(&d_models, hd.size() * sizeof(DevModel)));
+    MFD_HIP(hipMalloc(&d_flags, hd.size() * sizeof(int)));
+    MFD_HIP(hipHostMalloc(&h_flags, hd.size() * sizeof(int)));
     MFD_HIP(hipMemcpy(d_models, hd.data(), hd.size() * sizeof(DevModel), hipMemcpyHostToDevice));
     // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs, so block b
     // runs on XCD b % 8.  Give every XCD one contiguous chunk of the tile list, so tiles
@@ -761,11 +765,14 @@ int Runner::flush_dft_partial() {
 // Reads every model's non-finite flag; each model keeps its own status, and the first
 // failing model's status is returned.
 int Runner::check_flags() {
+    const int n = (int)models.size();
+    launch_collect_flags(d_models, n, d_flags, stream);
+    MFD_HIP(hipMemcpyAsync(h_flags, d_flags, n * sizeof(int), hipMemcpyDeviceToHost, stream));
     MFD_HIP(hipStreamSynchronize(stream));
     int first = MFD_OK;
-    for (Model *m : models) {
-        int f = 0;
-        MFD_HIP(hipMemcpy(&f, m->dm.flag, sizeof(int), hipMemcpyDeviceToHost));
+    for (int k = 0; k < n; k++) {
+        Model *m = models[k];
+        const int f = h_flags[k];
         if (f & 1) m->status = MFD_ERR_NAN;
         else if (f & 2) m->status = MFD_ERR_INF;
         if (m->status != MFD_OK && first == MFD_OK) first = m->status;
@@ -1100,6 +1107,12 @@ int32_t mfd_model_step(mfd_model *h, int32_t nsteps) {
     int st = ensure_runner(h);
     if (st != MFD_OK) return st;
     return h->runner.run(nsteps);
+}
+
+int32_t mfd_model_get_status(mfd_model *h, int32_t *status) {
+    if (!h || !status) { set_error("null argument"); return MFD_ERR_ARG; }
+    *status = h->m.status;
+    return MFD_OK;
 }
 
 int32_t mfd_model_get_it(mfd_model *h, int32_t *it) {

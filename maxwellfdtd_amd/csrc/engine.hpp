@@ -138,6 +138,8 @@ struct Runner {
     bool tfsf_gather = false;   // TFSF corrections + touched integrals in one launch
     bool aux_needed = false;    // some TFSF aux run is stepped on the device each step
     int4 *d_chunks = nullptr;   // state-pass work list {model, material, s0, s1}
+    int *d_flags = nullptr;     // per-model non-finite flags, gathered
+    int *h_flags = nullptr;     // pinned host copy
     int nchunks = 0;
     int maxsrc = 0;
     std::vector<int> src_kind_mask;   // per slot: 1 point, 2 tfsf

@@ -579,6 +579,16 @@ __global__ void k_finish(const DevModel *__restrict__ models, int n) {
     }
 }
 
+// Non-finite flags of every model of a run gathered into one array for one copy.
+__global__ void k_collect_flags(const DevModel *__restrict__ models, int n, int *__restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = *models[k].flag;
+}
+
+void launch_collect_flags(const DevModel *m, int nmodels, int *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_collect_flags, dim3((nmodels + 255) / 256), dim3(256), 0, s, m, nmodels, out);
+}
+
 void launch_finish(const DevModel *m, int nmodels, hipStream_t s) {
     hipLaunchKernelGGL(k_finish, dim3((nmodels + 255) / 256), dim3(256), 0, s, m, nmodels);
 }

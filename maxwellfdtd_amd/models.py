@@ -261,6 +261,12 @@ class Handle:
         self.lib.check(self.lib.model_get_it(self.h, C.byref(v)), "model_get_it")
         return v.value
 
+    @property
+    def status(self):
+        v = C.c_int32()
+        self.lib.check(self.lib.model_get_status(self.h, C.byref(v)), "model_get_status")
+        return v.value
+
     def field(self, comp):
         out = np.empty(self.N, dtype=float)
         self.lib.check(self.lib.model_get_field(self.h, comp_index(comp), _lib.dptr(out)), "get_field")

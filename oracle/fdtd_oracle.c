@@ -115,6 +115,7 @@ typedef struct {
 } mon_t;
 
 typedef struct ofd_model {
+    int status;                 /* MFD_OK, or the non-finite status of the last step */
     int ndims, Nx, Ny, Nz;
     int64_t N;
     real dx, dy, dz, dt;
@@ -1170,8 +1171,8 @@ int32_t ofd_model_step(ofd_model *m, int32_t nsteps) {
             if (isnan(Ex[i])) nan = 1;
             else if (isinf(Ex[i])) inf = 1;
         }
-        if (nan) { set_err("Something went wrong. I found NaN field values."); return MFD_ERR_NAN; }
-        if (inf) { set_err("Something went wrong. I found Inf field values."); return MFD_ERR_INF; }
+        if (nan) { set_err("Something went wrong. I found NaN field values."); m->status = MFD_ERR_NAN; return MFD_ERR_NAN; }
+        if (inf) { set_err("Something went wrong. I found Inf field values."); m->status = MFD_ERR_INF; return MFD_ERR_INF; }
         monitors(m, it0);
         if (m->int_flags) integrals(m);
     }
@@ -1179,6 +1180,7 @@ int32_t ofd_model_step(ofd_model *m, int32_t nsteps) {
 }
 
 int32_t ofd_model_get_it(ofd_model *m, int32_t *it) { *it = m->it; return MFD_OK; }
+int32_t ofd_model_get_status(ofd_model *m, int32_t *status) { *status = m->status; return MFD_OK; }
 
 int32_t ofd_model_get_field(ofd_model *m, int32_t comp, double *out) {
     if (comp < 0 || comp > 8 || !m->F[comp]) { set_err("bad component"); return MFD_ERR_ARG; }
