@@ -110,7 +110,8 @@ def bench_3d(args, lib):
         f"materials={len(desc.materials)}, integrals={desc.integrals}")
 
     h = Handle(desc, lib)
-    set_opts(lib, h, {0: args.graph_steps, 1: args.zchunk, 5: args.ty, 6: args.kform})
+    set_opts(lib, h, {0: args.graph_steps, 1: args.zchunk, 5: args.ty, 6: args.kform,
+                      10: args.snake})
     h.step(args.warmup)
     log(f"warmup {args.warmup} steps done")
     t0 = time.perf_counter()
@@ -143,7 +144,8 @@ def bench_3d(args, lib):
                                  f"plane wave, 40 dielectric spheres, Sa/E2 integrals "
                                  f"{'off' if args.no_integrals else 'on'}",
                      "cells": N, "model_steps": desc.Nt, "graph_steps": args.graph_steps,
-                     "zchunk": args.zchunk, "rows_per_workgroup": args.ty, "kernel_form": args.kform}
+                     "zchunk": args.zchunk, "rows_per_workgroup": args.ty, "kernel_form": args.kform,
+                     "snake_order": args.snake}
     out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                        "traffic": traffic,
@@ -405,6 +407,7 @@ def main():
     ap.add_argument("--zchunk", type=int, default=16)
     ap.add_argument("--ty", type=int, default=4)
     ap.add_argument("--kform", type=int, default=2)
+    ap.add_argument("--snake", type=int, default=1)
     ap.add_argument("--count", type=int, default=None, help="instances / requests")
     ap.add_argument("--step-scale", type=float, default=1.0)
     ap.add_argument("--chunk", type=int, default=200)

@@ -82,11 +82,20 @@ int64_t Model::footprint() const {
     return b;
 }
 
+// Every host<->device copy of the engine runs on a non-blocking stream and waits for
+// it: nothing touches the legacy stream, so copies from one thread never depend on a
+// stream another thread (a server worker) is capturing into a graph.
+hipError_t copy_sync(void *dst, const void *src, size_t n, hipMemcpyKind kind, hipStream_t s) {
+    hipError_t e = hipMemcpyAsync(dst, src, n, kind, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
 template <class T>
-static hipError_t upload_vec(Arena &a, T **dst, const std::vector<T> &v) {
+static hipError_t upload_vec(Arena &a, T **dst, const std::vector<T> &v, hipStream_t s) {
     hipError_t e = a.alloc(dst, v.size());
     if (e != hipSuccess) return e;
-    if (!v.empty()) e = hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (!v.empty()) e = copy_sync(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
     return e;
 }
 
@@ -149,8 +158,17 @@ static void mark_touched(const Model &m, std::vector<unsigned char> &mark) {
         }                                                                              \
     } while (0)
 
+int Model::ensure_io() {
+    if (!io) MFD_HIP(hipStreamCreateWithFlags(&io, hipStreamNonBlocking));
+    return MFD_OK;
+}
+
 int Model::upload() {
     if (uploaded) return MFD_OK;
+    {
+        int st = ensure_io();
+        if (st != MFD_OK) return st;
+    }
     DevModel m{};
     m.ndims = d.ndims;
     m.Nx = d.Nx; m.Ny = d.Ny; m.Nz = d.Nz;
@@ -182,7 +200,7 @@ int Model::upload() {
         const size_t total = nslot * arr + (size_t)nslot * st;
         char *pool;
         UP(arena.alloc(&pool, total));
-        UP(hipMemset(pool, 0, total));
+        UP(hipMemsetAsync(pool, 0, total, io));
         for (int k = 0; k < nslot; k++) *slots[k] = (float *)(pool + k * (arr + st));
     }
     // materials: id per cell (uint8) and compact state slots
@@ -215,15 +233,15 @@ int Model::upload() {
             cnt[g]++;
         }
         unsigned char *did;
-        UP(upload_vec(arena, &did, ids));
+        UP(upload_vec(arena, &did, ids, io));
         m.id = did;
         int *dt_;
-        UP(upload_vec(arena, &dt_, touched));
+        UP(upload_vec(arena, &dt_, touched, io));
         m.touched = dt_;
         m.ntouched = (int)touched.size();
         if (stateful) {
             int *ds;
-            UP(upload_vec(arena, &ds, slot));
+            UP(upload_vec(arena, &ds, slot, io));
             m.slot = ds;
         }
     }
@@ -247,16 +265,16 @@ int Model::upload() {
         v.K3 = (float)(hm.d.Mk3 * hm.d.Me * hm.d.Me * hm.d.Me);
         if (!state_cells[k + 1].empty()) {
             int *cp;
-            UP(upload_vec(arena, &cp, state_cells[k + 1]));
+            UP(upload_vec(arena, &cp, state_cells[k + 1], io));
             v.cells = cp;
         }
         if (hm.d.nq > 0) {
             float *p;
-            UP(upload_vec(arena, &p, to_f32(hm.Aq))); v.Aq = p;
-            UP(upload_vec(arena, &p, to_f32(hm.Bq))); v.Bq = p;
-            UP(upload_vec(arena, &p, to_f32(hm.Cq))); v.Cq = p;
+            UP(upload_vec(arena, &p, to_f32(hm.Aq), io)); v.Aq = p;
+            UP(upload_vec(arena, &p, to_f32(hm.Bq), io)); v.Bq = p;
+            UP(upload_vec(arena, &p, to_f32(hm.Cq), io)); v.Cq = p;
             UP(arena.alloc(&v.P, (size_t)hm.d.nq * nc * c));
-            UP(hipMemset(v.P, 0, (size_t)hm.d.nq * nc * c * sizeof(float2)));
+            UP(hipMemsetAsync(v.P, 0, (size_t)hm.d.nq * nc * c * sizeof(float2), io));
         }
         if (hm.d.isplasma) {
             v.Rava = hm.d.Rava; v.ksi = hm.d.ksi; v.rho0 = hm.d.rho0;
@@ -266,20 +284,20 @@ int Model::upload() {
             v.rate_kind = hm.d.rate_kind; v.rate_c = hm.d.rate_c; v.rate_k = hm.d.rate_k;
             v.rate_n = (int)hm.rx.size();
             double *dp;
-            UP(upload_vec(arena, &dp, hm.rx)); v.rate_x = dp;
-            UP(upload_vec(arena, &dp, hm.ry)); v.rate_y = dp;
+            UP(upload_vec(arena, &dp, hm.rx, io)); v.rate_x = dp;
+            UP(upload_vec(arena, &dp, hm.ry, io)); v.rate_y = dp;
             UP(arena.alloc(&v.rhod, c));
-            UP(hipMemset(v.rhod, 0, c * sizeof(double2)));
+            UP(hipMemsetAsync(v.rhod, 0, c * sizeof(double2), io));
             UP(arena.alloc(&v.Pp, (size_t)nc * c));
-            UP(hipMemset(v.Pp, 0, (size_t)nc * c * sizeof(float2)));
+            UP(hipMemsetAsync(v.Pp, 0, (size_t)nc * c * sizeof(float2), io));
             UP(arena.alloc(&v.Pa, (size_t)nc * c));
-            UP(hipMemset(v.Pa, 0, (size_t)nc * c * sizeof(float)));
+            UP(hipMemsetAsync(v.Pa, 0, (size_t)nc * c * sizeof(float), io));
             UP(arena.alloc(&v.JE, c));
-            UP(hipMemset(v.JE, 0, c * sizeof(float)));
+            UP(hipMemsetAsync(v.JE, 0, c * sizeof(float), io));
         }
     }
     DevMat *dmats;
-    UP(upload_vec(arena, &dmats, hmats));
+    UP(upload_vec(arena, &dmats, hmats, io));
     m.mats = dmats;
     m.nmat = nmat;
     // CPML layers
@@ -291,14 +309,14 @@ int Model::upload() {
         l.ib = L[f].ib - 1;
         l.nb = L[f].nb;
         float *p;
-        UP(upload_vec(arena, &p, to_f32(L[f].K))); l.K = p;
-        UP(upload_vec(arena, &p, to_f32(L[f].A))); l.A = p;
-        UP(upload_vec(arena, &p, to_f32(L[f].B))); l.B = p;
+        UP(upload_vec(arena, &p, to_f32(L[f].K), io)); l.K = p;
+        UP(upload_vec(arena, &p, to_f32(L[f].A), io)); l.A = p;
+        UP(upload_vec(arena, &p, to_f32(L[f].B), io)); l.B = p;
         const size_t n = psi_len(*this, f, l.nb);
         float **dst[4] = {&l.psiE0, &l.psiE1, &l.psiH0, &l.psiH1};
         for (int k = 0; k < 4; k++) {
             UP(arena.alloc(dst[k], n));
-            UP(hipMemset(*dst[k], 0, n * sizeof(float)));
+            UP(hipMemsetAsync(*dst[k], 0, n * sizeof(float), io));
         }
     }
     // sources
@@ -315,9 +333,9 @@ int Model::upload() {
             v.n = (int)hs.cells.size();
             v.coeff = (float)hs.coeff;
             int *ip;
-            UP(upload_vec(arena, &ip, hs.cells)); v.cells = ip;
+            UP(upload_vec(arena, &ip, hs.cells, io)); v.cells = ip;
             float *fp;
-            UP(upload_vec(arena, &fp, to_f32(hs.wave))); v.wave = fp;
+            UP(upload_vec(arena, &fp, to_f32(hs.wave), io)); v.wave = fp;
             maxsrc_n = std::max(maxsrc_n, v.n);
             continue;
         }
@@ -357,13 +375,13 @@ int Model::upload() {
                 v.tab[k] = nullptr;
                 if (slotmap[k] < 0) continue;
                 float *fp;
-                UP(upload_vec(arena, &fp, to_f32(hs.tables[slotmap[k]])));
+                UP(upload_vec(arena, &fp, to_f32(hs.tables[slotmap[k]]), io));
                 v.tab[k] = fp;
             }
         } else {
             const int Nz = d.Nz;
             UP(arena.alloc(&v.aux, 6 * (size_t)Nz));
-            UP(hipMemset(v.aux, 0, 6 * (size_t)Nz * sizeof(double)));
+            UP(hipMemsetAsync(v.aux, 0, 6 * (size_t)Nz * sizeof(double), io));
             int nbmax = 1;
             for (int f = 0; f < 2; f++) {
                 const HostLayer &hl = L[4 + f];
@@ -373,21 +391,21 @@ int Model::upload() {
                 if (hl.present) {
                     nbmax = std::max(nbmax, hl.nb);
                     double *dp;
-                    UP(upload_vec(arena, &dp, hl.K)); v.auxK[f] = dp;
-                    UP(upload_vec(arena, &dp, hl.A)); v.auxA[f] = dp;
-                    UP(upload_vec(arena, &dp, hl.B)); v.auxB[f] = dp;
+                    UP(upload_vec(arena, &dp, hl.K, io)); v.auxK[f] = dp;
+                    UP(upload_vec(arena, &dp, hl.A, io)); v.auxA[f] = dp;
+                    UP(upload_vec(arena, &dp, hl.B, io)); v.auxB[f] = dp;
                 }
             }
             v.aux_nbmax = nbmax;
             UP(arena.alloc(&v.auxpsi, 8 * (size_t)nbmax));
-            UP(hipMemset(v.auxpsi, 0, 8 * (size_t)nbmax * sizeof(double)));
+            UP(hipMemsetAsync(v.auxpsi, 0, 8 * (size_t)nbmax * sizeof(double), io));
             v.aux_n = (int)hs.aux_comp.size();
             int *ip;
-            UP(upload_vec(arena, &ip, hs.aux_comp)); v.aux_comp = ip;
-            UP(upload_vec(arena, &ip, hs.aux_iz)); v.aux_iz = ip;
+            UP(upload_vec(arena, &ip, hs.aux_comp, io)); v.aux_comp = ip;
+            UP(upload_vec(arena, &ip, hs.aux_iz, io)); v.aux_iz = ip;
             double *dp;
-            UP(upload_vec(arena, &dp, hs.aux_coeff)); v.aux_coeff = dp;
-            UP(upload_vec(arena, &dp, hs.aux_wave)); v.aux_wave = dp;
+            UP(upload_vec(arena, &dp, hs.aux_coeff, io)); v.aux_coeff = dp;
+            UP(upload_vec(arena, &dp, hs.aux_wave, io)); v.aux_wave = dp;
             v.Mh0 = d.Mh; v.Me0 = d.Me; v.Md10 = d.Md1; v.Md20 = d.Md2;
             v.idz = 1.0 / d.dz;
             const size_t tab_n = (size_t)d.Nt * 4 * Nz;
@@ -399,12 +417,12 @@ int Model::upload() {
         }
     }
     DevSource *dsrc;
-    UP(upload_vec(arena, &dsrc, hsrc));
+    UP(upload_vec(arena, &dsrc, hsrc, io));
     m.src = dsrc;
     m.nsrc = (int)srcs.size();
     for (size_t k = 0; k < hsrc.size(); k++)
         if (hsrc[k].auxtab) {
-            launch_tfsf_aux_table(dsrc + k, d.Nz, d.Nt, d.bcs[4], d.bcs[5], (float *)hsrc[k].auxtab, nullptr);
+            launch_tfsf_aux_table(dsrc + k, d.Nz, d.Nt, d.bcs[4], d.bcs[5], (float *)hsrc[k].auxtab, io);
             UP(hipGetLastError());
         }
     // monitors
@@ -422,42 +440,42 @@ int Model::upload() {
         v.ncomp = ncomp_mon;
         for (int c = 0; c < 6; c++) v.comps[c] = c < ncomp_mon ? comps[c] : 0;
         int *ip;
-        UP(upload_vec(arena, &ip, hm.cells)); v.cells = ip;
+        UP(upload_vec(arena, &ip, hm.cells, io)); v.cells = ip;
         maxmon_n = std::max(maxmon_n, v.n);
         if (!hm.spectral) {
             if (hm.issum) {
                 UP(arena.alloc(&v.sum, (size_t)d.Nt * ncomp_mon));
-                UP(hipMemset(v.sum, 0, (size_t)d.Nt * ncomp_mon * sizeof(double)));
+                UP(hipMemsetAsync(v.sum, 0, (size_t)d.Nt * ncomp_mon * sizeof(double), io));
             } else {
                 UP(arena.alloc(&v.rec, (size_t)d.Nt * ncomp_mon * v.n));
-                UP(hipMemset(v.rec, 0, (size_t)d.Nt * ncomp_mon * v.n * sizeof(float)));
+                UP(hipMemsetAsync(v.rec, 0, (size_t)d.Nt * ncomp_mon * v.n * sizeof(float), io));
             }
         } else {
             has_spectral = true;
             v.nw = (int)hm.w.size();
             v.tshift = hm.tshift;
             double *dp;
-            UP(upload_vec(arena, &dp, hm.w)); v.w = dp;
+            UP(upload_vec(arena, &dp, hm.w, io)); v.w = dp;
             UP(arena.alloc(&v.samp, (size_t)dft_block * ncomp_mon * v.n));
             const size_t ns = (size_t)ncomp_mon * v.nw * (hm.issum ? 1 : v.n);
             UP(arena.alloc(&v.S, ns));
-            UP(hipMemset(v.S, 0, ns * sizeof(double2)));
+            UP(hipMemsetAsync(v.S, 0, ns * sizeof(double2), io));
         }
     }
     DevMon *dmon;
-    UP(upload_vec(arena, &dmon, hmon));
+    UP(upload_vec(arena, &dmon, hmon, io));
     m.mon = dmon;
     m.nmon = (int)mons.size();
     m.dft_block = dft_block;
 
     int *ctr;
     UP(arena.alloc(&ctr, 2));
-    UP(hipMemset(ctr, 0, 2 * sizeof(int)));
+    UP(hipMemsetAsync(ctr, 0, 2 * sizeof(int), io));
     m.it = ctr;
     m.flag = ctr + 1;
-    // the fills and copies above run on the null stream; runners use non-blocking
-    // streams, so finish them before any kernel can read this model
-    UP(hipStreamSynchronize(nullptr));
+    // the fills above run on the model's copy stream; runners use other streams, so
+    // finish them before any kernel can read this model
+    UP(hipStreamSynchronize(io));
     dm = m;
     it = 0;
     dft_done = 0;
@@ -471,7 +489,9 @@ int Model::upload() {
 int Model::sync() { return MFD_OK; }
 
 void Model::release() {
+    if (io) hipStreamSynchronize(io);
     arena.release();
+    if (io) { hipStreamDestroy(io); io = nullptr; }
     uploaded = false;
 }
 
@@ -487,7 +507,7 @@ int Model::read_field(int comp, double *out) {
     std::vector<float> tmp(N);
     int st = sync();
     if (st != MFD_OK) return st;
-    MFD_HIP(hipMemcpy(tmp.data(), dm.F[comp], N * sizeof(float), hipMemcpyDeviceToHost));
+    MFD_HIP(copy_sync(tmp.data(), dm.F[comp], N * sizeof(float), hipMemcpyDeviceToHost, io));
     for (long long i = 0; i < N; i++) out[i] = tmp[i];
     return MFD_OK;
 }
@@ -503,7 +523,7 @@ int Model::write_field(int comp, const double *in) {
     for (long long i = 0; i < N; i++) tmp[i] = (float)in[i];
     st = sync();
     if (st != MFD_OK) return st;
-    MFD_HIP(hipMemcpy(dm.F[comp], tmp.data(), N * sizeof(float), hipMemcpyHostToDevice));
+    MFD_HIP(copy_sync(dm.F[comp], tmp.data(), N * sizeof(float), hipMemcpyHostToDevice, io));
     return MFD_OK;
 }
 
@@ -513,7 +533,7 @@ Runner::~Runner() { release(); }
 void Runner::release() {
     if (gexec) { hipGraphExecDestroy(gexec); gexec = nullptr; }
     if (d_models) { hipFree(d_models); d_models = nullptr; }
-    if (d_tiles) { hipFree(d_tiles); d_tiles = nullptr; }
+    if (d_tiles) { hipFree(d_tiles); d_tiles = nullptr; d_tiles_e = nullptr; }
     if (d_chunks) { hipFree(d_chunks); d_chunks = nullptr; }
     if (d_flags) { hipFree(d_flags); d_flags = nullptr; }
     if (h_flags) { hipHostFree(h_flags); h_flags = nullptr; }
@@ -603,7 +623,7 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     nchunks = (int)chunks.size();
     if (nchunks) {
         MFD_HIP(hipMalloc(&d_chunks, chunks.size() * sizeof(int4)));
-        MFD_HIP(hipMemcpy(d_chunks, chunks.data(), chunks.size() * sizeof(int4), hipMemcpyHostToDevice));
+        MFD_HIP(copy_sync(d_chunks, chunks.data(), chunks.size() * sizeof(int4), hipMemcpyHostToDevice, stream));
     }
     // sources: a model whose only source is a TFSF source gets its face corrections and
     // its touched cells' integrals from one gather launch (3D)
@@ -621,7 +641,7 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     MFD_HIP(hipMalloc(&d_models, hd.size() * sizeof(DevModel)));
     MFD_HIP(hipMalloc(&d_flags, hd.size() * sizeof(int)));
     MFD_HIP(hipHostMalloc(&h_flags, hd.size() * sizeof(int)));
-    MFD_HIP(hipMemcpy(d_models, hd.data(), hd.size() * sizeof(DevModel), hipMemcpyHostToDevice));
+    MFD_HIP(copy_sync(d_models, hd.data(), hd.size() * sizeof(DevModel), hipMemcpyHostToDevice, stream));
     // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs, so block b
     // runs on XCD b % 8.  Give every XCD one contiguous chunk of the tile list, so tiles
     // that share halo rows and planes share that XCD's L2.
@@ -633,8 +653,19 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
         if (src < (int)tiles.size()) perm[b] = tiles[src];
     }
     ntiles = (int)perm.size();
-    MFD_HIP(hipMalloc(&d_tiles, perm.size() * sizeof(Tile)));
-    MFD_HIP(hipMemcpy(d_tiles, perm.data(), perm.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    // E half-step order: each XCD walks its chunk backwards, so the E kernel starts on
+    // the tiles the H kernel finished last (their H is still in the Infinity Cache), and
+    // the next H kernel, walking forwards, starts where E finished
+    std::vector<Tile> rev(perm.size(), make_int4(-1, 0, 0, 0));
+    for (int b = 0; b < per * nxcd; b++) {
+        const int src = (b % nxcd) * per + (per - 1 - b / nxcd);
+        if (src < (int)tiles.size()) rev[b] = tiles[src];
+    }
+    if (!opt.snake) rev = perm;
+    MFD_HIP(hipMalloc(&d_tiles, 2 * perm.size() * sizeof(Tile)));
+    MFD_HIP(copy_sync(d_tiles, perm.data(), perm.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
+    MFD_HIP(copy_sync(d_tiles + perm.size(), rev.data(), rev.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
+    d_tiles_e = d_tiles + perm.size();
     return MFD_OK;
 }
 
@@ -654,11 +685,11 @@ int Runner::launch_step(bool timing) {
     else if (ndims == 2) launch_h2d(d_models, d_tiles, ntiles, grid, mode, opt.ty, stream);
     else launch_h1d(d_models, d_tiles, ntiles, grid, mode, stream);
     if (timing) { MFD_HIP(hipEventRecord(ev[1], stream)); MFD_HIP(hipEventRecord(ev[2], stream)); }
-    if (ndims == 3 && kform == 1) launch_e3d_v4(d_models, d_tiles, ntiles, grid, mode, any_int, opt.ty, stream);
-    else if (ndims == 3 && kform == 2) launch_e3d_p(d_models, d_tiles, ntiles, grid, mode, any_int, opt.ty, stream);
-    else if (ndims == 3) launch_e3d(d_models, d_tiles, ntiles, grid, mode, any_int, stream);
-    else if (ndims == 2) launch_e2d(d_models, d_tiles, ntiles, grid, mode, any_int, opt.ty, stream);
-    else launch_e1d(d_models, d_tiles, ntiles, grid, mode, any_int, stream);
+    if (ndims == 3 && kform == 1) launch_e3d_v4(d_models, d_tiles_e, ntiles, grid, mode, any_int, opt.ty, stream);
+    else if (ndims == 3 && kform == 2) launch_e3d_p(d_models, d_tiles_e, ntiles, grid, mode, any_int, opt.ty, stream);
+    else if (ndims == 3) launch_e3d(d_models, d_tiles_e, ntiles, grid, mode, any_int, stream);
+    else if (ndims == 2) launch_e2d(d_models, d_tiles_e, ntiles, grid, mode, any_int, opt.ty, stream);
+    else launch_e1d(d_models, d_tiles_e, ntiles, grid, mode, any_int, stream);
     if (split_state && nchunks) launch_e_state(d_models, d_chunks, nchunks, ndims, any_int, stream);
     if (timing) MFD_HIP(hipEventRecord(ev[3], stream));
     if (tfsf_gather) {
@@ -808,6 +839,7 @@ int set_option(RunOptions &o, Model *m, int opt, int value) {
             if (value < 0 || value > 2) { set_error("3D kernel form must be 0, 1 or 2"); return MFD_ERR_ARG; }
             o.vec = value;
             break;
+        case MFD_OPT_SNAKE: o.snake = value != 0; break;
         case MFD_OPT_STAGGER:
             if (!m || m->uploaded || value < 0) { set_error("stagger must be >= 0, set on a model before upload"); return MFD_ERR_ARG; }
             m->stagger = value;
@@ -1154,18 +1186,18 @@ int32_t mfd_model_get_monitor(mfd_model *h, int32_t imon, int32_t comp, double *
     if (!hm.spectral) {
         if (hm.issum) {
             std::vector<double> tmp(Nt * nc);
-            MFD_HIP(hipMemcpy(tmp.data(), dmn.sum, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+            MFD_HIP(copy_sync(tmp.data(), dmn.sum, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, m.io));
             for (size_t t = 0; t < Nt; t++) out[t] = tmp[t * nc + ci];
         } else {
             std::vector<float> tmp(Nt * nc * n);
-            MFD_HIP(hipMemcpy(tmp.data(), dmn.rec, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+            MFD_HIP(copy_sync(tmp.data(), dmn.rec, tmp.size() * sizeof(float), hipMemcpyDeviceToHost, m.io));
             for (size_t t = 0; t < Nt; t++)
                 for (size_t k = 0; k < n; k++) out[k + n * t] = tmp[(t * nc + ci) * n + k];
         }
     } else {
         const size_t nw = hm.w.size(), np = hm.issum ? 1 : n;
         std::vector<double2> tmp(nc * nw * np);
-        MFD_HIP(hipMemcpy(tmp.data(), dmn.S, tmp.size() * sizeof(double2), hipMemcpyDeviceToHost));
+        MFD_HIP(copy_sync(tmp.data(), dmn.S, tmp.size() * sizeof(double2), hipMemcpyDeviceToHost, m.io));
         for (size_t w = 0; w < nw; w++)
             for (size_t k = 0; k < np; k++) {
                 const double2 v = tmp[((size_t)ci * nw + w) * np + k];
@@ -1187,7 +1219,7 @@ int32_t mfd_model_get_output(mfd_model *h, int32_t which, double *out) {
         const float *src = which == MFD_OUT_SA ? m.dm.Sa : m.dm.E2;
         if (!src) { set_error("integral not enabled"); return MFD_ERR_STATE; }
         std::vector<float> tmp(N);
-        MFD_HIP(hipMemcpy(tmp.data(), src, N * sizeof(float), hipMemcpyDeviceToHost));
+        MFD_HIP(copy_sync(tmp.data(), src, N * sizeof(float), hipMemcpyDeviceToHost, m.io));
         for (long long i = 0; i < N; i++) out[i] = tmp[i];
         return MFD_OK;
     }
@@ -1201,11 +1233,11 @@ int32_t mfd_model_get_output(mfd_model *h, int32_t which, double *out) {
         std::vector<double> vals(cnt);
         if (which == MFD_OUT_JE) {
             std::vector<float> tmp(cnt);
-            MFD_HIP(hipMemcpy(tmp.data(), v.JE, cnt * sizeof(float), hipMemcpyDeviceToHost));
+            MFD_HIP(copy_sync(tmp.data(), v.JE, cnt * sizeof(float), hipMemcpyDeviceToHost, m.io));
             for (int i = 0; i < cnt; i++) vals[i] = tmp[i];
         } else {
             std::vector<double2> tmp(cnt);
-            MFD_HIP(hipMemcpy(tmp.data(), v.rhod, cnt * sizeof(double2), hipMemcpyDeviceToHost));
+            MFD_HIP(copy_sync(tmp.data(), v.rhod, cnt * sizeof(double2), hipMemcpyDeviceToHost, m.io));
             for (int i = 0; i < cnt; i++) vals[i] = tmp[i].x * m.mats[k].d.rho0;
         }
         int s = 0;

@@ -14,6 +14,7 @@
 namespace mfd {
 
 void set_error(const std::string &msg);
+hipError_t copy_sync(void *dst, const void *src, size_t n, hipMemcpyKind kind, hipStream_t s);
 
 #define MFD_HIP(call)                                                                  \
     do {                                                                               \
@@ -104,10 +105,12 @@ struct Model {
     bool has_mats = false, has_tfsf = false, has_spectral = false;
 
     int status = 0;             // MFD_OK or the model's non-finite status
+    hipStream_t io = nullptr;   // non-blocking stream for this model's copies
 
     int ncomp() const { return d.ndims == 3 ? 3 : d.ndims == 2 ? 2 : 1; }
     bool comp_present(int c) const;
     int64_t footprint() const;
+    int ensure_io();
     int upload();
     int sync();
     void release();
@@ -122,13 +125,15 @@ struct RunOptions {
     int vec = 2;                // 3D kernel form: 0 scalar, 1 vector (Nx % 4 == 0), 2 pipelined
     int persistent = 0;         // 0: one workgroup per tile; >0: that many workgroups per CU
     bool kernel_timing = false; // hipEvent pairs around the H/E launches (eager mode)
+    bool snake = true;          // E half walks each XCD's tile chunk backwards
 };
 
 // Advances a fixed set of uploaded models together.
 struct Runner {
     std::vector<Model *> models;
     DevModel *d_models = nullptr;
-    Tile *d_tiles = nullptr;
+    Tile *d_tiles = nullptr;    // H-half tile order
+    Tile *d_tiles_e = nullptr;  // E-half tile order
     int ntiles = 0;
     int ndims = 3;
     bool any_mats = false, any_full = false, any_int = false, any_mon = false, any_spectral = false;
