@@ -129,6 +129,7 @@ struct DevModel {
     int dft_block;              // DFT time-block length
     const int *touched;         // cells written by a source (integral fixup list)
     int ntouched;
+    const DevSource *xtfsf;     // TFSF source whose x faces the E kernel finishes, or null
 };
 
 constexpr unsigned char ID_MASK = 0x7f, ID_TOUCHED = 0x80;
@@ -146,6 +147,7 @@ constexpr int TX3 = 64, TY3 = 4;    // 3D scalar kernels: 64 x 4 columns, marchi
 constexpr int TX3V = 256;           // 3D vector kernels: 256 x (4 per lane) x TY rows
 constexpr int TX2 = 64;             // 2D: 64 x-columns per wave, TW waves per tile, marching z
 constexpr int TX1 = 256;            // 1D: 256 cells
+constexpr int XFACE_ZC = 64;        // longest tile z range with the E kernel's x-face epilogue
 
 // launchers (defined in the .hip files)
 void launch_h3d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, hipStream_t s);

@@ -633,6 +633,9 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
         tfsf_gather &= m->srcs.size() == 1 && m->srcs[0].type == 1;
         for (const DevSource &v : m->hsrc) aux_needed |= v.kind == SRC_TFSF && v.mode == 1 && !v.auxtab;
     }
+    // the pipelined 3D E kernel finishes TFSF x-face cells itself
+    if (tfsf_gather && kform == 2 && opt.zchunk <= XFACE_ZC)
+        for (size_t k = 0; k < ms.size(); k++) hd[k].xtfsf = ms[k]->dm.src;
     src_kind_mask.assign(maxsrc, 0);
     for (Model *m : ms)
         for (size_t s2 = 0; s2 < m->srcs.size(); s2++)

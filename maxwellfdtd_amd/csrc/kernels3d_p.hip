@@ -10,6 +10,7 @@
 // from an LDS table.  All field pointers are accessed in the global address space.
 #include "common.hpp"
 #include "material.hpp"
+#include "tfsf.hpp"
 
 namespace mfd {
 
@@ -180,6 +181,9 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
     // cells is finished by the state pass, k_e_state)
     __shared__ float4 ctab[128];
     __shared__ float2 ktab[128];    // Kerr Pade (K2, K3)
+    // incident values of the TFSF x faces for this tile's rows and planes:
+    // [side][H_u, H_v, E_u, E_v][row][plane - z0]
+    __shared__ float xinc[2][4][TY][XFACE_ZC];
     const int lane = threadIdx.x, ty = threadIdx.y;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile tl = tiles[t];
@@ -192,6 +196,36 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
             const float kind = (mt.flags & (MAT_DISP | MAT_PLASMA)) ? 2.0f : (mt.flags & MAT_KERR) ? 1.0f : 0.0f;
             ctab[k] = make_float4(mt.Md1, mt.Md2, mt.Me, kind);
             if (MODE == MODE_FULL) ktab[k] = make_float2(mt.K2, mt.K3);
+        }
+        // TFSF x faces finished in this kernel: the source's scalars come into registers
+        // once per tile and the incident values of the tile's x-face cells into LDS, so
+        // the z march issues no dependent or flat load for them
+        int xb[6] = {0, -2, 0, -1, 0, -1};
+        float xcm = 0.0f, xce = 0.0f, xcd = 0.0f;
+        bool xl[4] = {false, false, false, false};
+        bool xtile = false;
+        if (m.xtfsf) {
+            const __attribute__((address_space(1))) DevSource &S =
+                *(const __attribute__((address_space(1))) DevSource *)m.xtfsf;
+#pragma unroll
+            for (int q = 0; q < 6; q++) xb[q] = S.box[q];
+            xcm = S.cm[0]; xce = S.ce[0]; xcd = S.cd[0];
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) xl[kk] = tfsf_live(S, 0, kk);
+            const int x0 = tl.y, zt0 = tile_z0(tl), nzt = tile_z1(tl) - zt0;
+            xtile = ((xb[0] - 1 >= x0 && xb[0] - 1 < x0 + 64) || (xb[0] >= x0 && xb[0] < x0 + 64) ||
+                     (xb[1] >= x0 && xb[1] < x0 + 64)) &&
+                    tl.z <= xb[3] && tl.z + TY - 1 >= xb[2] && zt0 <= xb[5] && zt0 + nzt - 1 >= xb[4];
+            if (xtile) {
+                const int it0 = *m.it, Nzm = m.Nz;
+                for (int k = lane + 64 * ty; k < 8 * TY * nzt; k += 64 * TY) {
+                    const int zz = k % nzt, r = (k / nzt) % TY, kk = (k / (nzt * TY)) % 4, side = k / (nzt * TY * 4);
+                    const int u = tl.z + r, v = zt0 + zz;
+                    const bool in = (kk == 0 || kk == 3) ? (u >= xb[2] && u <= xb[3] && v >= xb[4] && v <= xb[5] - 1)
+                                                         : (u >= xb[2] && u <= xb[3] - 1 && v >= xb[4] && v <= xb[5]);
+                    xinc[side][kk][r][zz] = in && xl[kk] ? tfsf_inc(S, Nzm, 0, side, kk, u, v, it0) : 0.0f;
+                }
+            }
         }
         __syncthreads();
         const int Nx = m.Nx, Ny = m.Ny, Nz = m.Nz;
@@ -219,6 +253,10 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
             for (int f = 0; f < 6; f++) L[f] = layp(m.L[f], false);
             const bool xs0 = L[0].present && ix <= L[0].ib, xs1 = L[1].present && ix >= L[1].ib;
             const bool ys0 = L[2].present && iy <= L[2].ib, ys1 = L[3].present && iy >= L[3].ib;
+            // lanes on a TFSF x-face column (planes N1-1, N1, N2 of the x faces)
+            const bool xcol = xtile && (ix == xb[0] - 1 || ix == xb[0] || ix == xb[1]) && iy >= xb[2] &&
+                              iy <= xb[3];
+            const int xside = ix == xb[1] ? 1 : 0;
             auto load = [&](int iz, EP &P) {
                 const unsigned i = col + sz * iz;
                 P.hx = Hx[i]; P.hy = Hy[i]; P.hz = Hz[i];
@@ -305,13 +343,46 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
                         }
                     }
                 }
+                // x-face TFSF epilogue: the corrections add_source!(::TFSFSource3D) makes on
+                // an x face (src/sources.jl:725-745) after this update, in face order; the
+                // H correction of the H-out cells enters only the integrals here (the face
+                // kernel writes it, after every neighbour has read the uncorrected H)
+                float hyI = P.hy, hzI = P.hz;
+                bool xh = false;
+                if (xcol && iz >= xb[4] && iz <= xb[5]) {
+                    xh = tfsf_xhandled_box(xb, ix, iy, iz, defer);
+                    if (xh) {
+                        const int u = iy, v = iz, side = xside, zz = iz - z0;
+                        const int U2 = xb[3], V2 = xb[5];
+                        const int nH = side ? xb[1] : xb[0] - 1, nE = side ? xb[1] : xb[0];
+                        const float s0 = side ? 1.0f : -1.0f;
+                        if (ix == nH) {
+                            if (u <= U2 && v <= V2 - 1 && xl[3])
+                                hyI = hyI + (s0 * xcm) * xinc[side][3][ty][zz];
+                            if (u <= U2 - 1 && v <= V2 && xl[2])
+                                hzI = hzI + (-s0 * xcm) * xinc[side][2][ty][zz];
+                        }
+                        if (ix == nE) {
+                            if (u <= U2 - 1 && v <= V2 && xl[1]) {
+                                const float iv = xinc[side][1][ty][zz];
+                                e[1] = e[1] + (-s0 * xce) * iv;
+                                d1 = d1 + (-s0 * xcd) * iv;
+                            }
+                            if (u <= U2 && v <= V2 - 1 && xl[0]) {
+                                const float iv = xinc[side][0][ty][zz];
+                                e[2] = e[2] + (s0 * xce) * iv;
+                                d2 = d2 + (s0 * xcd) * iv;
+                            }
+                        }
+                    }
+                }
                 Dx[i] = d0; Dy[i] = d1; Dz[i] = d2;
                 if (!defer) { Ex[i] = e[0]; Ey[i] = e[1]; Ez[i] = e[2]; }
                 check_finite(e[0], m.flag, !defer);
-                if (INTEG && !defer && !(P.g & ID_TOUCHED)) {   // src/outputs.jl:480-483
+                if (INTEG && !defer && (!(P.g & ID_TOUCHED) || xh)) {   // src/outputs.jl:480-483
                     if (doSa) {
-                        const float a = e[1] * P.hz - e[2] * P.hy, b = e[2] * P.hx - e[0] * P.hz;
-                        const float cc = e[0] * P.hy - e[1] * P.hx;
+                        const float a = e[1] * hzI - e[2] * hyI, b = e[2] * P.hx - e[0] * hzI;
+                        const float cc = e[0] * hyI - e[1] * P.hx;
                         Sa[i] = P.sa + sqrtf(a * a + b * b + cc * cc) * dtf;
                     }
                     if (doE2) E2[i] = P.e2 + (e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) * dtf;

@@ -5,6 +5,7 @@
 // and a model that finished its Nt steps is skipped.
 #include "common.hpp"
 #include "material.hpp"
+#include "tfsf.hpp"
 
 #include <algorithm>
 
@@ -170,19 +171,6 @@ void launch_tfsf_aux_table(const DevSource *S, int Nz, int Nt, int bcl, int bcr,
 // Face f = 2*axis + side.  (u, v) are the tangential axes (x face: y,z; y face: x,z;
 // z face: x,y); the face tables are H_u, H_v, E_u, E_v.  1D and 2D models use faces
 // 4,5 (resp. 0,1,4,5) with degenerate tangential ranges.
-__device__ inline float aux_value(const DevSource &S, int Nz, int comp, int z, int it0) {
-    if (S.auxtab) {
-        const int k = comp == HX ? 0 : comp == HY ? 1 : comp == EX ? 2 : 3;
-        return G(S.auxtab)[((size_t)it0 * 4 + k) * Nz + z];
-    }
-    switch (comp) {
-        case HX: return (float)S.aux[z];
-        case HY: return (float)S.aux[Nz + z];
-        case EX: return (float)S.aux[4 * Nz + z];
-        case EY: return (float)S.aux[5 * Nz + z];
-        default: return 0.0f;
-    }
-}
 
 // Both faces of one axis run in one launch (they write disjoint planes); the three
 // axes run in three launches in the reference's face order.
@@ -273,14 +261,16 @@ __global__ __launch_bounds__(256) void k_tfsf_axis(const DevModel *__restrict__ 
 // write).  The lane applies every correction of source `slot` that lands on its cell, in
 // the reference's face order (x, y, z; lower side, upper side) and with the statements
 // of k_tfsf_axis, then accumulates the cell's Sa/E2 (which the E kernel skipped) and
-// checks Ex.  Used when the TFSF source is a model's only source.
+// checks Ex.  Used when the TFSF source is a model's only source.  When the model's E
+// kernel finishes x-face cells itself (m.xtfsf set), such a cell only gets its pending
+// H correction here.
 __global__ __launch_bounds__(256) void k_tfsf_gather(const DevModel *__restrict__ models,
                                                      int slot, int integ) {
     const DevModel &m = models[blockIdx.y];
     const int it0 = *m.it;
     if (it0 >= m.Nt) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = k < m.ntouched;
+    bool active = k < m.ntouched;
     float exf = 0.0f;
     if (active) {
         const DevSource &S = m.src[slot];
@@ -288,9 +278,19 @@ __global__ __launch_bounds__(256) void k_tfsf_gather(const DevModel *__restrict_
         const int Nx = m.Nx, Ny = m.Ny;
         const int c3[3] = {(int)(i % (unsigned)Nx), (int)((i / (unsigned)Nx) % (unsigned)Ny),
                            (int)(i / ((unsigned)Nx * (unsigned)Ny))};
+        bool xh = false;
+        if (m.xtfsf) {
+            const int g = G(m.id)[i] & ID_MASK;
+            xh = tfsf_xhandled(S, c3[0], c3[1], c3[2], (m.mats[g].flags & (MAT_DISP | MAT_PLASMA)) != 0);
+        }
         float f[9];
+        if (xh) {
+            f[HY] = G((const float *)m.F[HY])[i];
+            f[HZ] = G((const float *)m.F[HZ])[i];
+        } else {
 #pragma unroll
-        for (int c = 0; c < 9; c++) f[c] = G((const float *)m.F[c])[i];
+            for (int c = 0; c < 9; c++) f[c] = G((const float *)m.F[c])[i];
+        }
         unsigned dirty = 0;
 #pragma unroll
         for (int axis = 0; axis < 3; axis++) {
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(256) void k_tfsf_gather(const DevModel *__restrict_
             const int N1 = S.box[2 * axis], N2 = S.box[2 * axis + 1];
             const int u = c3[ua], v = c3[va], n = c3[axis];
             if (u < U1 || u > U2 || v < V1 || v > V2) continue;
-            const int Nu = U2 - U1, Nv = V2 - V1;
+            if (xh && axis > 0) continue;
             constexpr int cH[3][2] = {{HY, HZ}, {HX, HZ}, {HX, HY}};
             constexpr int cE[3][2] = {{EY, EZ}, {EX, EZ}, {EX, EY}};
             constexpr int cD[3][2] = {{DY, DZ}, {DX, DZ}, {DX, DY}};
@@ -311,37 +311,27 @@ __global__ __launch_bounds__(256) void k_tfsf_gather(const DevModel *__restrict_
             for (int side = 0; side < 2; side++) {
                 const int nH = side ? N2 : N1 - 1, nE = side ? N2 : N1;
                 if (n != nH && n != nE) continue;
-                const int f_ = 2 * axis + side;
                 const float sg = side ? -1.0f : 1.0f;
                 const float s0 = (axis == 1 ? 1.0f : -1.0f) * sg;
-                const int tabc[4] = {cHu, cHv, cEu, cEv};
-                auto live = [&](int kk) { return S.mode == 0 || (tabc[kk] != HZ && tabc[kk] != EZ); };
-                auto inc = [&](int kk) -> float {
-                    if (S.mode == 1) {
-                        int z;
-                        if (axis == 2) z = side ? N2 : (kk < 2 ? N1 - 1 : N1);
-                        else z = v;
-                        return aux_value(S, m.Nz, tabc[kk], z, it0);
-                    }
-                    const float *T = S.tab[4 * f_ + kk];
-                    if (!T) return 0.0f;
-                    const int na = (kk == 0 || kk == 3) ? Nu + 1 : Nu;
-                    const int nb = (kk == 0 || kk == 3) ? Nv : Nv + 1;
-                    return G(T)[(u - U1) + (size_t)na * ((v - V1) + (size_t)nb * it0)];
-                };
                 if (n == nH) {
-                    if (u <= U2 && v <= V2 - 1 && live(3)) { f[cHu] = f[cHu] + (s0 * cm) * inc(3); dirty |= 1u << cHu; }
-                    if (u <= U2 - 1 && v <= V2 && live(2)) { f[cHv] = f[cHv] + (-s0 * cm) * inc(2); dirty |= 1u << cHv; }
+                    if (u <= U2 && v <= V2 - 1 && tfsf_live(S, axis, 3)) {
+                        f[cHu] = f[cHu] + (s0 * cm) * tfsf_inc(S, m.Nz, axis, side, 3, u, v, it0);
+                        dirty |= 1u << cHu;
+                    }
+                    if (u <= U2 - 1 && v <= V2 && tfsf_live(S, axis, 2)) {
+                        f[cHv] = f[cHv] + (-s0 * cm) * tfsf_inc(S, m.Nz, axis, side, 2, u, v, it0);
+                        dirty |= 1u << cHv;
+                    }
                 }
-                if (n == nE) {
-                    if (u <= U2 - 1 && v <= V2 && live(1)) {
-                        const float iv = inc(1);
+                if (n == nE && !xh) {
+                    if (u <= U2 - 1 && v <= V2 && tfsf_live(S, axis, 1)) {
+                        const float iv = tfsf_inc(S, m.Nz, axis, side, 1, u, v, it0);
                         f[cEu] = f[cEu] + (-s0 * ce) * iv;
                         f[cDu] = f[cDu] + (-s0 * cd) * iv;
                         dirty |= (1u << cEu) | (1u << cDu);
                     }
-                    if (u <= U2 && v <= V2 - 1 && live(0)) {
-                        const float iv = inc(0);
+                    if (u <= U2 && v <= V2 - 1 && tfsf_live(S, axis, 0)) {
+                        const float iv = tfsf_inc(S, m.Nz, axis, side, 0, u, v, it0);
                         f[cEv] = f[cEv] + (s0 * ce) * iv;
                         f[cDv] = f[cDv] + (s0 * cd) * iv;
                         dirty |= (1u << cEv) | (1u << cDv);
@@ -352,16 +342,20 @@ __global__ __launch_bounds__(256) void k_tfsf_gather(const DevModel *__restrict_
 #pragma unroll
         for (int c = 0; c < 9; c++)
             if (dirty & (1u << c)) G(m.F[c])[i] = f[c];
-        if (integ) {   // src/outputs.jl:475-485
-            const float dt = m.dtf;
-            if (m.Sa) {
-                const float a = f[EY] * f[HZ] - f[EZ] * f[HY], b = f[EZ] * f[HX] - f[EX] * f[HZ];
-                const float cc = f[EX] * f[HY] - f[EY] * f[HX];
-                G(m.Sa)[i] += sqrtf(a * a + b * b + cc * cc) * dt;
+        if (xh) {
+            active = false;     // integrals and the Ex check were done by the E kernel
+        } else {
+            if (integ) {   // src/outputs.jl:475-485
+                const float dt = m.dtf;
+                if (m.Sa) {
+                    const float a = f[EY] * f[HZ] - f[EZ] * f[HY], b = f[EZ] * f[HX] - f[EX] * f[HZ];
+                    const float cc = f[EX] * f[HY] - f[EY] * f[HX];
+                    G(m.Sa)[i] += sqrtf(a * a + b * b + cc * cc) * dt;
+                }
+                if (m.E2) G(m.E2)[i] += (f[EX] * f[EX] + f[EY] * f[EY] + f[EZ] * f[EZ]) * dt;
             }
-            if (m.E2) G(m.E2)[i] += (f[EX] * f[EX] + f[EY] * f[EY] + f[EZ] * f[EZ]) * dt;
+            exf = f[EX];
         }
-        exf = f[EX];
     }
     check_finite(exf, m.flag, active);
 }

@@ -256,3 +256,36 @@ def slab_1d(nz=600, steps=900, dispersive=True):
     mons = [FieldMonitor((0.875 * L,)),
             SpectralMonitor((0.875 * L,), w=2 * np.pi * C0 / np.linspace(600e-9, 1000e-9, 32))]
     return model, mons
+
+
+def tfsf_faces_3d(n=40, steps=150, pml=4, lo=9, hi=30):
+    """A TFSF-only 3D model whose box faces cut through materials: a Lorentz sphere
+    centred on the lower x face, a dielectric sphere on the upper x face, a Debye sphere
+    on the x/z edge and a Kerr sphere on a y face, so that face cells hold stateless,
+    stateful and nonlinear materials.  Box = cells lo..hi (0-based) on every axis."""
+    d = 40e-9
+    L = (n - 1) * d
+    grid = Grid(xmin=0, xmax=L, Nx=n, ymin=0, ymax=L, Ny=n, zmin=0, zmax=L, Nz=n)
+    x = grid.x
+    mid = x[(lo + hi) // 2]
+
+    def ball(c, r):
+        return lambda X, Y, Z: (X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2 <= r ** 2
+
+    w0 = 2 * np.pi * C0 / 800e-9
+    mats = [
+        Material(geometry=ball((x[lo], mid, mid), 4 * d), eps=1.8,
+                 chi=LorentzSusceptibility(deps=1.0, w0=1.2 * w0, delta=1e14)),
+        Material(geometry=ball((x[hi], mid + 3 * d, mid), 4 * d), eps=3.0),
+        Material(geometry=ball((x[lo], mid, x[lo]), 3.5 * d), eps=2.0,
+                 chi=DebyeSusceptibility(deps=1.5, tau=5e-15)),
+        Material(geometry=ball((mid, x[lo], mid + 4 * d), 3.5 * d), eps=2.1, chi3=1e-19),
+    ]
+    w = gauss_waveform(lam0=800e-9, tau0=5e-15, dt0=12e-15)
+    src = TFSFSource(box=(x[lo], x[hi], x[lo], x[hi], x[lo], x[hi]), waveform=lambda t: 1e8 * w(t),
+                     component="Ex", z0=x[pml + 1])
+    dt = 0.99 / C0 / np.sqrt(3 / d ** 2)
+    model = Model(grid, src, tmax=(steps - 1) * dt, CN=0.99, pml=pml * d, material=mats)
+    mons = [FieldMonitor((x[lo], mid, mid)), FieldMonitor((x[lo - 1], mid, mid)),
+            SpectralMonitor((x[hi], mid, mid), w=np.array([w0, 1.2 * w0]))]
+    return model, mons

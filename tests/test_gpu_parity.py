@@ -117,6 +117,17 @@ def test_slab_1d():
     assert_parity(hg, ho, desc, model)
 
 
+@pytest.mark.parametrize("kform", [2, 0])
+def test_tfsf_faces_through_materials(kform):
+    """TFSF as the only source (one gather launch per step; with the pipelined kernels
+    the E kernel finishes x-face cells itself) with face cells holding stateless,
+    ADE, Debye and Kerr materials."""
+    model, mons = cases.tfsf_faces_3d()
+    hg, ho, desc = run_pair(model, mons, options={6: kform})
+    assert_parity(hg, ho, desc, model)
+    assert np.max(np.abs(ho.field("Ex"))) > 1e6       # the pulse is in the box
+
+
 def test_graph_and_eager_agree_bitwise():
     model, mons = cases.spheres_3d(n=24, steps=70)
     desc = model.describe(monitors=mons)
