@@ -18,7 +18,10 @@ __global__ __launch_bounds__(256) void k_e_state(const DevModel *__restrict__ mo
     const int4 ch = chunks[blockIdx.x];
     const DevModel &m = models[ch.x];
     if (*m.it >= m.Nt) return;
-    const DevMat &mt = m.mats[ch.y];
+    // the material record through the global address space: its scalars are uniform
+    // over the block and load as such, with no flat access in the per-lane work
+    const __attribute__((address_space(1))) DevMat &mt =
+        *(const __attribute__((address_space(1))) DevMat *)(m.mats + ch.y);
     const int s = ch.z + threadIdx.x;
     bool active = s < ch.w;
     float e0 = 0.0f;

@@ -10,11 +10,12 @@ constexpr double DBL_EPS_ = 2.220446049250313e-16;   // eps(Float64), src/models
 
 // ionrate(I) in FP64: power law sigma*I^K, or the log-log table of IonizationRate
 // (examples/5_nonlinear_propagation/IonizationRate.jl:30-52).
-__device__ inline double ionrate(const DevMat &mt, double I) {
+template <class MR>
+__device__ inline double ionrate(MR &mt, double I) {
     if (!(I > 0.0)) return 0.0;
     if (mt.rate_kind == 1) return mt.rate_c * pow(I, mt.rate_k);
     if (mt.rate_kind == 2) {
-        const double *x = mt.rate_x, *y = mt.rate_y;
+        const gp<const double> x = G(mt.rate_x), y = G(mt.rate_y);
         const int n = mt.rate_n;
         const double xl = log10(I);
         int i;
@@ -35,9 +36,10 @@ __device__ inline double ionrate(const DevMat &mt, double I) {
 }
 
 // d[c]: updated D; e[c]: on entry E^n (read only when the material has ADE/plasma
-// state), on exit E^{n+1}.  s is the cell's state slot within its material.
-template <int NC>
-__device__ __forceinline__ void material_e(const DevMat &mt, int s, const float (&d)[NC],
+// state), on exit E^{n+1}.  s is the cell's state slot within its material.  mt may be a
+// generic or a global-address-space reference.
+template <int NC, class MR>
+__device__ __forceinline__ void material_e(MR &mt, int s, const float (&d)[NC],
                                            float (&e)[NC], double dt, float dtf) {
     const int flags = mt.flags;
     float sumP[NC];

@@ -436,7 +436,7 @@ void launch_monitors(const DevModel *m, int nmodels, int maxn, hipStream_t s) {
 // ---- time-blocked DFT (src/dft.jl:27-51): S += sum_k F_k * 2 exp(-i w (t_k - tshift)) dt
 // The nk buffered samples of the last time block are contracted against an FP64 phase
 // table staged in LDS; runs after the step counter advanced past the block.
-constexpr int WCH = 32;
+constexpr int WCH = 16;     // frequencies per pass: 32 FP64 accumulators per lane
 constexpr int KMAX = 64;
 
 __global__ __launch_bounds__(256) void k_dft_flush(const DevModel *__restrict__ models, int nk) {
