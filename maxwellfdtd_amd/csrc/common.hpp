@@ -49,6 +49,7 @@ struct DevLayer {
 // per-cell slot index (the reference keeps full-grid arrays, src/materials.jl:424-474).
 struct DevMat {
     float Mh, Me, Md1, Md2;     // src/materials.jl:393-400 (vacuum: src/models.jl:111-114)
+    float iMe;                  // 1/Me = EPS0*eps: D of a plain cell from its E
     int flags;                  // MAT_DISP | MAT_KERR | MAT_PLASMA
     int nq;                     // ADE poles
     float K2, K3;               // Kerr Pade: Mk2*Me^2, Mk3*Me^3 (src/models.jl:1140-1149)
@@ -130,6 +131,7 @@ struct DevModel {
     const int *touched;         // cells written by a source (integral fixup list)
     int ntouched;
     const DevSource *xtfsf;     // TFSF source whose x faces the E kernel finishes, or null
+    int dless;                  // plain cells keep no D: D = E/Me is rebuilt on demand
 };
 
 constexpr unsigned char ID_MASK = 0x7f, ID_TOUCHED = 0x80;
@@ -173,6 +175,8 @@ void launch_monitors(const DevModel *m, int nmodels, int maxn, hipStream_t s);
 void launch_dft_flush(const DevModel *m, int nmodels, int maxn, int nk, hipStream_t s);
 void launch_finish(const DevModel *m, int nmodels, hipStream_t s);
 void launch_collect_flags(const DevModel *m, int nmodels, int *out, hipStream_t s);
+// D = E/Me written back into the D arrays of a model that ran without stored D
+void launch_d_from_e(const DevModel &m, hipStream_t s);
 // E of the cells of stateful (ADE / plasma) materials, after the E half-step kernel
 // updated their D: one work item per state slot, chunks {model, material, s0, s1}.
 void launch_e_state(const DevModel *m, const int4 *chunks, int nchunks, int ndims, bool integ,

@@ -250,6 +250,7 @@ int Model::upload() {
     {
         DevMat &v = hmats[0];
         v.Mh = (float)d.Mh; v.Me = (float)d.Me; v.Md1 = (float)d.Md1; v.Md2 = (float)d.Md2;
+        v.iMe = (float)(1.0 / d.Me);
     }
     for (int k = 0; k < nmat; k++) {
         const HostMat &hm = mats[k];
@@ -257,6 +258,7 @@ int Model::upload() {
         const long long c = cnt[k + 1];
         v.Mh = (float)hm.d.Mh; v.Me = (float)hm.d.Me;
         v.Md1 = (float)hm.d.Md1; v.Md2 = (float)hm.d.Md2;
+        v.iMe = (float)(1.0 / hm.d.Me);
         v.flags = (hm.d.nq > 0 ? MAT_DISP : 0) | (hm.d.iskerr ? MAT_KERR : 0) |
                   (hm.d.isplasma ? MAT_PLASMA : 0);
         v.nq = hm.d.nq;
@@ -484,6 +486,18 @@ int Model::upload() {
     return MFD_OK;
 }
 
+// D of the plain cells after runs without stored D: rebuilt as E/Me in place.
+int Model::materialize_d() {
+    if (!d_stale || !uploaded) return MFD_OK;
+    DevModel t = dm;
+    t.xtfsf = d_tfsf ? dm.src : nullptr;
+    launch_d_from_e(t, io);
+    MFD_HIP(hipGetLastError());
+    MFD_HIP(hipStreamSynchronize(io));
+    d_stale = false;
+    return MFD_OK;
+}
+
 // Device state is complete whenever no run is in flight: every Runner::run ends with a
 // synchronisation of its stream (check_flags), whichever runner stepped the model.
 int Model::sync() { return MFD_OK; }
@@ -506,6 +520,7 @@ int Model::read_field(int comp, double *out) {
     }
     std::vector<float> tmp(N);
     int st = sync();
+    if (st == MFD_OK && (comp == DX || comp == DY || comp == DZ)) st = materialize_d();
     if (st != MFD_OK) return st;
     MFD_HIP(copy_sync(tmp.data(), dm.F[comp], N * sizeof(float), hipMemcpyDeviceToHost, io));
     for (long long i = 0; i < N; i++) out[i] = tmp[i];
@@ -522,7 +537,9 @@ int Model::write_field(int comp, const double *in) {
     std::vector<float> tmp(N);
     for (long long i = 0; i < N; i++) tmp[i] = (float)in[i];
     st = sync();
+    if (st == MFD_OK) st = materialize_d();
     if (st != MFD_OK) return st;
+    user_set = true;        // a state the caller wrote runs with stored D from now on
     MFD_HIP(copy_sync(dm.F[comp], tmp.data(), N * sizeof(float), hipMemcpyHostToDevice, io));
     return MFD_OK;
 }
@@ -577,7 +594,7 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     MFD_HIP(hipDeviceGetAttribute(&cu_count, hipDeviceAttributeMultiprocessorCount, dev));
     std::vector<DevModel> hd;
     std::vector<Tile> tiles;
-    any_mats = any_full = any_int = any_mon = any_spectral = false;
+    any_mats = any_full = any_int = any_mon = any_spectral = h_mats = false;
     maxtouched = 1;
     maxsrc = 0;
     maxsrc_n = maxmon_n = maxface = 1;
@@ -596,8 +613,11 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
         hd.push_back(m->dm);
         make_tiles(*m, (int)k, opt.zchunk, opt.ty, kform, tiles);
         any_mats |= m->has_mats;
-        for (const auto &hm : m->mats)
+        for (const auto &hm : m->mats) {
             any_full |= hm.d.nq > 0 || hm.d.iskerr || hm.d.isplasma;
+            // the H half needs the material id only where mu differs from vacuum
+            h_mats |= m->has_mats && (float)hm.d.Mh != (float)m->d.Mh;
+        }
         maxtouched = std::max(maxtouched, (int)m->touched.size());
         any_int |= m->int_flags != 0;
         any_mon |= !m->mons.empty();
@@ -634,8 +654,26 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
         for (const DevSource &v : m->hsrc) aux_needed |= v.kind == SRC_TFSF && v.mode == 1 && !v.auxtab;
     }
     // the pipelined 3D E kernel finishes TFSF x-face cells itself
-    if (tfsf_gather && kform == 2 && opt.zchunk <= XFACE_ZC)
+    const bool xface = tfsf_gather && kform == 2 && opt.zchunk <= XFACE_ZC;
+    if (xface)
         for (size_t k = 0; k < ms.size(); k++) hd[k].xtfsf = ms[k]->dm.src;
+    dless_models.clear();
+    // plain cells without stored D: pipelined 3D kernels, no ADE/Kerr/plasma in the run,
+    // TFSF (finished by the x-face path) or no source at all, and a state the caller has
+    // not written
+    for (size_t k = 0; k < ms.size(); k++) {
+        Model *m = ms[k];
+        const bool ok = opt.dless && ndims == 3 && kform == 2 && !any_full && !m->user_set &&
+                        ((xface && m->srcs.size() == 1) || m->srcs.empty());
+        if (!ok) {
+            int st = m->materialize_d();
+            if (st != MFD_OK) return st;
+            continue;
+        }
+        hd[k].dless = 1;
+        m->d_tfsf = !m->srcs.empty();
+        dless_models.push_back(m);
+    }
     src_kind_mask.assign(maxsrc, 0);
     for (Model *m : ms)
         for (size_t s2 = 0; s2 < m->srcs.size(); s2++)
@@ -682,11 +720,12 @@ int Runner::launch_step(bool timing) {
     }
     if (timing) MFD_HIP(hipEventRecord(ev[0], stream));
     const int mode = any_full ? MODE_FULL : any_mats ? MODE_SIMPLE : MODE_VAC;
-    if (ndims == 3 && kform == 1) launch_h3d_v4(d_models, d_tiles, ntiles, grid, mode, opt.ty, stream);
-    else if (ndims == 3 && kform == 2) launch_h3d_p(d_models, d_tiles, ntiles, grid, mode, opt.ty, stream);
-    else if (ndims == 3) launch_h3d(d_models, d_tiles, ntiles, grid, mode, stream);
-    else if (ndims == 2) launch_h2d(d_models, d_tiles, ntiles, grid, mode, opt.ty, stream);
-    else launch_h1d(d_models, d_tiles, ntiles, grid, mode, stream);
+    const int hmode = h_mats ? MODE_SIMPLE : MODE_VAC;
+    if (ndims == 3 && kform == 1) launch_h3d_v4(d_models, d_tiles, ntiles, grid, hmode, opt.ty, stream);
+    else if (ndims == 3 && kform == 2) launch_h3d_p(d_models, d_tiles, ntiles, grid, hmode, opt.ty, stream);
+    else if (ndims == 3) launch_h3d(d_models, d_tiles, ntiles, grid, hmode, stream);
+    else if (ndims == 2) launch_h2d(d_models, d_tiles, ntiles, grid, hmode, opt.ty, stream);
+    else launch_h1d(d_models, d_tiles, ntiles, grid, hmode, stream);
     if (timing) { MFD_HIP(hipEventRecord(ev[1], stream)); MFD_HIP(hipEventRecord(ev[2], stream)); }
     if (ndims == 3 && kform == 1) launch_e3d_v4(d_models, d_tiles_e, ntiles, grid, mode, any_int, opt.ty, stream);
     else if (ndims == 3 && kform == 2) launch_e3d_p(d_models, d_tiles_e, ntiles, grid, mode, any_int, opt.ty, stream);
@@ -740,6 +779,7 @@ int Runner::build_graph(int steps) {
 
 // One step of every model; models that already finished are skipped on the device.
 int Runner::run(int nsteps) {
+    for (Model *m : dless_models) m->d_stale = true;
     int done = 0;
     const bool timing = opt.kernel_timing;
     int G = opt.graph_steps;
@@ -843,6 +883,7 @@ int set_option(RunOptions &o, Model *m, int opt, int value) {
             o.vec = value;
             break;
         case MFD_OPT_SNAKE: o.snake = value != 0; break;
+        case MFD_OPT_DLESS: o.dless = value != 0; break;
         case MFD_OPT_STAGGER:
             if (!m || m->uploaded || value < 0) { set_error("stagger must be >= 0, set on a model before upload"); return MFD_ERR_ARG; }
             m->stagger = value;
@@ -1163,7 +1204,13 @@ int32_t mfd_model_get_field(mfd_model *h, int32_t comp, double *out) {
 
 int32_t mfd_model_set_field(mfd_model *h, int32_t comp, const double *in) {
     if (!h || !in) { set_error("null argument"); return MFD_ERR_ARG; }
-    return h->m.write_field(comp, in);
+    if (h->owner) { set_error("model belongs to a batch or server"); return MFD_ERR_STATE; }
+    const int st = h->m.write_field(comp, in);
+    if (st == MFD_OK && h->runner_ready) {   // the launch plan depends on user-written state
+        h->runner.release();
+        h->runner_ready = false;
+    }
+    return st;
 }
 
 int32_t mfd_model_get_monitor(mfd_model *h, int32_t imon, int32_t comp, double *out) {

@@ -105,6 +105,9 @@ struct Model {
     bool has_mats = false, has_tfsf = false, has_spectral = false;
 
     int status = 0;             // MFD_OK or the model's non-finite status
+    bool user_set = false;      // the caller wrote a field component
+    bool d_stale = false;       // D of plain cells not stored by the last runs
+    bool d_tfsf = false;        // ... whose stored-D cells are the TFSF face cells
     hipStream_t io = nullptr;   // non-blocking stream for this model's copies
 
     int ncomp() const { return d.ndims == 3 ? 3 : d.ndims == 2 ? 2 : 1; }
@@ -113,6 +116,7 @@ struct Model {
     int ensure_io();
     int upload();
     int sync();
+    int materialize_d();
     void release();
     int read_field(int comp, double *out);
     int write_field(int comp, const double *in);
@@ -126,6 +130,7 @@ struct RunOptions {
     int persistent = 0;         // 0: one workgroup per tile; >0: that many workgroups per CU
     bool kernel_timing = false; // hipEvent pairs around the H/E launches (eager mode)
     bool snake = true;          // E half walks each XCD's tile chunk backwards
+    bool dless = true;          // plain cells keep no D where the run allows it
 };
 
 // Advances a fixed set of uploaded models together.
@@ -137,6 +142,8 @@ struct Runner {
     int ntiles = 0;
     int ndims = 3;
     bool any_mats = false, any_full = false, any_int = false, any_mon = false, any_spectral = false;
+    bool h_mats = false;        // some material has mu != 1: the H half reads the id
+    std::vector<Model *> dless_models;   // models this runner steps without stored D
     int maxtouched = 1;
     int kform = 0;              // 3D kernel form in use
     bool split_state = false;   // stateful cells' E computed by the state pass

@@ -12,6 +12,8 @@
 #include "material.hpp"
 #include "tfsf.hpp"
 
+#include <algorithm>
+
 namespace mfd {
 
 typedef __attribute__((address_space(1))) float gf1;
@@ -194,7 +196,8 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
         for (int k = lane + 64 * ty; k <= nm; k += 64 * TY) {
             const DevMat &mt = m.mats[k];
             const float kind = (mt.flags & (MAT_DISP | MAT_PLASMA)) ? 2.0f : (mt.flags & MAT_KERR) ? 1.0f : 0.0f;
-            ctab[k] = make_float4(mt.Md1, mt.Md2, mt.Me, kind);
+            // w: the material kind with ADE/Kerr/plasma, else 1/Me for the runs without D
+            ctab[k] = make_float4(mt.Md1, mt.Md2, mt.Me, MODE == MODE_FULL ? kind : mt.iMe);
             if (MODE == MODE_FULL) ktab[k] = make_float2(mt.K2, mt.K3);
         }
         // TFSF x faces finished in this kernel: the source's scalars come into registers
@@ -257,11 +260,21 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
             const bool xcol = xtile && (ix == xb[0] - 1 || ix == xb[0] || ix == xb[1]) && iy >= xb[2] &&
                               iy <= xb[3];
             const int xside = ix == xb[1] ? 1 : 0;
+            // without stored D (plain materials, TFSF as the only source), a plain cell
+            // reads E and rebuilds D = E/Me; the TFSF face cells keep their D, which the
+            // face corrections change apart from E
+            const bool dless = MODE != MODE_FULL && m.dless;
+            auto dcell = [&](int iz) { return !dless || tfsf_touched_box(xb, ix, iy, iz); };
+            gcf1 *Dsx = G((const float *)m.F[DX]), *Dsy = G((const float *)m.F[DY]);
+            gcf1 *Dsz = G((const float *)m.F[DZ]);
+            gcf1 *Esx = G((const float *)m.F[EX]), *Esy = G((const float *)m.F[EY]);
+            gcf1 *Esz = G((const float *)m.F[EZ]);
             auto load = [&](int iz, EP &P) {
                 const unsigned i = col + sz * iz;
                 P.hx = Hx[i]; P.hy = Hy[i]; P.hz = Hz[i];
                 if (yload) { P.hxy = Hx[i + yoff]; P.hzy = Hz[i + yoff]; }
-                P.d0 = Dx[i]; P.d1 = Dy[i]; P.d2 = Dz[i];
+                const bool dc = dcell(iz);
+                P.d0 = (dc ? Dsx : Esx)[i]; P.d1 = (dc ? Dsy : Esy)[i]; P.d2 = (dc ? Dsz : Esz)[i];
                 if (eload) { P.hyE = Hy[i + xoff]; P.hzE = Hz[i + xoff]; }
                 if (MODE != MODE_VAC || INTEG) P.g = id[i];
                 if (doSa) P.sa = Sa[i];
@@ -294,7 +307,9 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
                 float dHyz = (P.hy - hym) * idz;
                 float dHzx = (P.hz - hzx) * idx;
                 float dHzy = (P.hz - P.hzy) * idy;
+                const bool dc = dcell(iz);
                 float d0 = P.d0, d1 = P.d1, d2 = P.d2;
+                if (!dc) { d0 *= cf.w; d1 *= cf.w; d2 *= cf.w; }   // D = E/Me
                 float a0, a1;
                 if (xs0) {   // src/models.jl:976-986
                     slab2(L[0], ix, ix + (unsigned)L[0].nb * (iy + (unsigned)Ny * iz), dHyx, dHzx, a0, a1);
@@ -376,7 +391,7 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
                         }
                     }
                 }
-                Dx[i] = d0; Dy[i] = d1; Dz[i] = d2;
+                if (dc) { Dx[i] = d0; Dy[i] = d1; Dz[i] = d2; }
                 if (!defer) { Ex[i] = e[0]; Ey[i] = e[1]; Ez[i] = e[2]; }
                 check_finite(e[0], m.flag, !defer);
                 if (INTEG && !defer && (!(P.g & ID_TOUCHED) || xh)) {   // src/outputs.jl:480-483
@@ -427,6 +442,29 @@ void launch_h3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, in
     else if (ty == 2) lh<2>(m, tiles, ntiles, grid, mode, s);
     else if (ty == 4) lh<4>(m, tiles, ntiles, grid, mode, s);
     else lh<8>(m, tiles, ntiles, grid, mode, s);
+}
+
+// D = E/Me (the E kernel's own rebuild) for every plain cell of a model that ran
+// without stored D, so that D reads back and later runs start from a full state.
+__global__ __launch_bounds__(256) void k_d_from_e(const DevModel m) {
+    int xb[6] = {0, -2, 0, -1, 0, -1};
+    if (m.xtfsf)
+        for (int q = 0; q < 6; q++) xb[q] = m.xtfsf->box[q];
+    const unsigned N = (unsigned)m.N;
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const int ix = (int)(i % (unsigned)m.Nx), iy = (int)((i / (unsigned)m.Nx) % (unsigned)m.Ny);
+        const int iz = (int)(i / ((unsigned)m.Nx * (unsigned)m.Ny));
+        if (tfsf_touched_box(xb, ix, iy, iz)) continue;
+        const float iMe = G(m.mats)[G(m.id)[i] & ID_MASK].iMe;
+        G(m.F[DX])[i] = G((const float *)m.F[EX])[i] * iMe;
+        G(m.F[DY])[i] = G((const float *)m.F[EY])[i] * iMe;
+        G(m.F[DZ])[i] = G((const float *)m.F[EZ])[i] * iMe;
+    }
+}
+
+void launch_d_from_e(const DevModel &m, hipStream_t s) {
+    const long long nb = std::min<long long>((m.N + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_d_from_e, dim3((unsigned)nb), dim3(256), 0, s, m);
 }
 
 void launch_e3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ,

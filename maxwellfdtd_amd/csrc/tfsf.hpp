@@ -82,6 +82,17 @@ __device__ __forceinline__ bool tfsf_xhandled_box(const int (&b)[6], int ix, int
     const bool on_z = (iz == Z1 - 1 || iz == Z1 || iz == Z2) && ix >= X1 && ix <= X2;
     return !on_y && !on_z && !stateful;
 }
+// Whether a cell lies on a TFSF face plane inside the tangential ranges: the cells a
+// face correction may write (the touched set of the host's mark_touched)
+__device__ __forceinline__ bool tfsf_touched_box(const int (&b)[6], int ix, int iy, int iz) {
+    const int X1 = b[0], X2 = b[1], Y1 = b[2], Y2 = b[3], Z1 = b[4], Z2 = b[5];
+    const bool inx = ix >= X1 && ix <= X2, iny = iy >= Y1 && iy <= Y2, inz = iz >= Z1 && iz <= Z2;
+    const bool fx = (ix == X1 - 1 || ix == X1 || ix == X2) && iny && inz;
+    const bool fy = (iy == Y1 - 1 || iy == Y1 || iy == Y2) && inx && inz;
+    const bool fz = (iz == Z1 - 1 || iz == Z1 || iz == Z2) && inx && iny;
+    return fx || fy || fz;
+}
+
 template <class SR>
 __device__ __forceinline__ bool tfsf_xhandled(SR &S, int ix, int iy, int iz, bool stateful) {
     const int b[6] = {S.box[0], S.box[1], S.box[2], S.box[3], S.box[4], S.box[5]};
