@@ -50,6 +50,7 @@ struct DevLayer {
 struct DevMat {
     float Mh, Me, Md1, Md2;     // src/materials.jl:393-400 (vacuum: src/models.jl:111-114)
     float iMe;                  // 1/Me = EPS0*eps: D of a plain cell from its E
+    float MeMd2;                // Me*Md2: the E-form update of a plain cell without D
     int flags;                  // MAT_DISP | MAT_KERR | MAT_PLASMA
     int nq;                     // ADE poles
     float K2, K3;               // Kerr Pade: Mk2*Me^2, Mk3*Me^3 (src/models.jl:1140-1149)

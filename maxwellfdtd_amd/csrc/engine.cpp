@@ -251,6 +251,7 @@ int Model::upload() {
         DevMat &v = hmats[0];
         v.Mh = (float)d.Mh; v.Me = (float)d.Me; v.Md1 = (float)d.Md1; v.Md2 = (float)d.Md2;
         v.iMe = (float)(1.0 / d.Me);
+        v.MeMd2 = (float)(d.Me * d.Md2);
     }
     for (int k = 0; k < nmat; k++) {
         const HostMat &hm = mats[k];
@@ -259,6 +260,7 @@ int Model::upload() {
         v.Mh = (float)hm.d.Mh; v.Me = (float)hm.d.Me;
         v.Md1 = (float)hm.d.Md1; v.Md2 = (float)hm.d.Md2;
         v.iMe = (float)(1.0 / hm.d.Me);
+        v.MeMd2 = (float)(hm.d.Me * hm.d.Md2);
         v.flags = (hm.d.nq > 0 ? MAT_DISP : 0) | (hm.d.iskerr ? MAT_KERR : 0) |
                   (hm.d.isplasma ? MAT_PLASMA : 0);
         v.nq = hm.d.nq;

@@ -53,7 +53,7 @@ struct HP {
 };
 
 template <int MODE, int TY>
-__global__ __launch_bounds__(64 * TY) void k_h3d_p(const DevModel *__restrict__ models,
+__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_h3d_p(const DevModel *__restrict__ models,
                                                    const Tile *__restrict__ tiles, int ntiles) {
     __shared__ float mh_tab[128];
     const int lane = threadIdx.x, ty = threadIdx.y;
@@ -196,8 +196,8 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
         for (int k = lane + 64 * ty; k <= nm; k += 64 * TY) {
             const DevMat &mt = m.mats[k];
             const float kind = (mt.flags & (MAT_DISP | MAT_PLASMA)) ? 2.0f : (mt.flags & MAT_KERR) ? 1.0f : 0.0f;
-            // w: the material kind with ADE/Kerr/plasma, else 1/Me for the runs without D
-            ctab[k] = make_float4(mt.Md1, mt.Md2, mt.Me, MODE == MODE_FULL ? kind : mt.iMe);
+            // w: the material kind with ADE/Kerr/plasma, else Me*Md2 for the runs without D
+            ctab[k] = make_float4(mt.Md1, mt.Md2, mt.Me, MODE == MODE_FULL ? kind : mt.MeMd2);
             if (MODE == MODE_FULL) ktab[k] = make_float2(mt.K2, mt.K3);
         }
         // TFSF x faces finished in this kernel: the source's scalars come into registers
@@ -261,8 +261,9 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
                               iy <= xb[3];
             const int xside = ix == xb[1] ? 1 : 0;
             // without stored D (plain materials, TFSF as the only source), a plain cell
-            // reads E and rebuilds D = E/Me; the TFSF face cells keep their D, which the
-            // face corrections change apart from E
+            // reads E and takes the update in E form, E' = Md1 (E + Me Md2 psi) + Me Md2
+            // curl H, which is Me times the D form; the TFSF face cells keep their D,
+            // which the face corrections change apart from E
             const bool dless = MODE != MODE_FULL && m.dless;
             auto dcell = [&](int iz) { return !dless || tfsf_touched_box(xb, ix, iy, iz); };
             gcf1 *Dsx = G((const float *)m.F[DX]), *Dsy = G((const float *)m.F[DY]);
@@ -300,16 +301,15 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
                 }
                 const int g = P.g & ID_MASK;
                 const float4 cf = MODE != MODE_VAC ? ctab[g] : ctab[0];
-                const float Md1 = cf.x, Md2 = cf.y;
+                const bool dc = dcell(iz);
+                const float Md1 = cf.x, Md2 = dc ? cf.y : cf.w;
                 float dHxy = (P.hx - P.hxy) * idy;
                 float dHxz = (P.hx - hxm) * idz;
                 float dHyx = (P.hy - hyx) * idx;
                 float dHyz = (P.hy - hym) * idz;
                 float dHzx = (P.hz - hzx) * idx;
                 float dHzy = (P.hz - P.hzy) * idy;
-                const bool dc = dcell(iz);
                 float d0 = P.d0, d1 = P.d1, d2 = P.d2;
-                if (!dc) { d0 *= cf.w; d1 *= cf.w; d2 *= cf.w; }   // D = E/Me
                 float a0, a1;
                 if (xs0) {   // src/models.jl:976-986
                     slab2(L[0], ix, ix + (unsigned)L[0].nb * (iy + (unsigned)Ny * iz), dHyx, dHzx, a0, a1);
@@ -342,7 +342,8 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
                 d1 = Md1 * d1 + Md2 * (dHxz - dHzx);
                 d2 = Md1 * d2 + Md2 * (dHyx - dHxy);
                 float e[3];
-                e[0] = d0 * cf.z; e[1] = d1 * cf.z; e[2] = d2 * cf.z;
+                if (dc) { e[0] = d0 * cf.z; e[1] = d1 * cf.z; e[2] = d2 * cf.z; }
+                else { e[0] = d0; e[1] = d1; e[2] = d2; }       // already E
                 bool defer = false;
                 if (MODE == MODE_FULL && cf.w != 0.0f) {
                     if (cf.w > 1.5f) {

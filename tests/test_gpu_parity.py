@@ -148,6 +148,28 @@ def test_plain_tfsf_without_stored_d():
         assert np.array_equal(hg.field(c), mid.field(c)), c
 
 
+def test_no_drift_without_stored_d():
+    """A closed, lossless box (periodic, no CPML) keeps the field for good: over 3000
+    steps the run without stored D stays within rounding of the run with stored D and of
+    the FP64 oracle.  A plain cell whose D were rebuilt as fl(E * fl(1/Me)) every step
+    would gain about 6e-8 per step in vacuum and miss the first bound."""
+    model, mons = cases.spheres_3d(n=32, steps=3000, soft=False, nspheres=6, pml=0,
+                                   monitors=False)
+    desc = model.describe()
+    a = Handle(desc, _lib.engine())
+    a.step(desc.Nt)
+    b = Handle(desc, _lib.engine())
+    b.lib.check(b.lib.model_set_options(b.h, 11, 0), "set_options")
+    b.step(desc.Nt)
+    errs = field_errors(a, b, 3)
+    assert max(errs.values()) <= 5e-5, errs
+    ho = Handle(desc, oracle("f64"))
+    ho.step(desc.Nt)
+    errs = field_errors(a, ho, 3)
+    assert max(errs.values()) <= 1e-4, errs
+    assert np.max(np.abs(ho.field("Ex"))) > 0.1
+
+
 def test_graph_and_eager_agree_bitwise():
     model, mons = cases.spheres_3d(n=24, steps=70)
     desc = model.describe(monitors=mons)
