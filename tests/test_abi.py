@@ -153,3 +153,36 @@ def test_oracle_rejects_like_engine():
     assert orc.model_create(C.byref(_desc(ndims=4)), C.byref(h)) == _lib.MFD_ERR_ARG
     assert orc.model_create(C.byref(_desc(bcs=[1, 1, 4, 1, 1, 1])), C.byref(h)) == _lib.MFD_ERR_ARG
     assert "Wrong 'bc' value" in orc.error()
+
+
+def test_batch_handle_and_option_checks(engine_so):
+    """Batch handles and their option checks are host-side: they work without a GPU."""
+    lib = _lib.engine()
+    assert lib.batch_create(None) == _lib.MFD_ERR_ARG
+    b = C.c_void_p()
+    assert lib.batch_create(C.byref(b)) == _lib.MFD_OK
+    try:
+        assert lib.batch_add(b, None) == _lib.MFD_ERR_ARG
+        assert lib.batch_set_options(b, _lib.MFD_OPT_ZCHUNK, 0) == _lib.MFD_ERR_ARG
+        assert lib.batch_set_options(b, _lib.MFD_OPT_ROWS, 3) == _lib.MFD_ERR_ARG
+        assert "rows per workgroup" in lib.error()
+        assert lib.batch_set_options(b, _lib.MFD_OPT_DFT_BLOCK, 16) == _lib.MFD_ERR_ARG
+        assert "per-model option" in lib.error()
+        assert lib.batch_set_options(b, _lib.MFD_OPT_ZCHUNK, 32) == _lib.MFD_OK
+        assert lib.batch_set_options(b, 99, 1) == _lib.MFD_ERR_ARG
+        n = C.c_int32(-1)
+        assert lib.batch_active(b, C.byref(n)) == _lib.MFD_OK and n.value == 0
+        assert lib.batch_autotune(b, 4, None, None, None, None) == _lib.MFD_ERR_STATE
+    finally:
+        assert lib.batch_destroy(b) == _lib.MFD_OK
+
+
+def test_server_needs_a_gpu(engine_so):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    lib = _lib.engine()
+    s = C.c_void_p()
+    assert lib.server_create(0, 4, None) == _lib.MFD_ERR_ARG
+    assert lib.server_create(0, 4, C.byref(s)) != _lib.MFD_OK
+    assert lib.error()
