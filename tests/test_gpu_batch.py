@@ -34,12 +34,15 @@ def alone(desc):
     return h
 
 
-def assert_same(h1, h2, ndims):
+def assert_same(h1, h2, ndims, sa_tol=0.0):
     for c in COMPS[ndims]:
         a, b = h1.field(c), h2.field(c)
         assert np.array_equal(a, b), (c, np.max(np.abs(a - b)))
     if h1.desc.integrals & 1:
-        assert np.array_equal(h1.output(0), h2.output(0))
+        if sa_tol == 0.0:
+            assert np.array_equal(h1.output(0), h2.output(0))
+        else:
+            assert rel_l2(h1.output(0), h2.output(0)) <= sa_tol
 
 
 def test_batch_2d_equals_alone_and_oracle():
@@ -151,3 +154,60 @@ def C_footprint(h):
     v = C.c_int64()
     h.lib.check(h.lib.model_footprint(h.h, C.byref(v)), "footprint")
     return v.value
+
+
+def test_batch_autotune_3d_keeps_results():
+    models = small_3d(count=3, seed=21)
+    descs = [m.describe() for m in models]
+    b = Batch(descs, options={_lib.MFD_OPT_GRAPH_STEPS: 4})
+    best = b.autotune(trial_steps=2)
+    assert best["zchunk"] in (8, 16, 32) and best["rows"] in (2, 4, 8)
+    b.run(0)
+    for d, hb in zip(descs, b.handles):
+        ha = alone(d)
+        assert_same(hb, ha, 3)
+        ha.close()
+    b.close()
+
+
+def unstable_3d(n=32, steps=200):
+    """A model that blows up: CN = 1.5 breaks the Courant limit."""
+    from maxwellfdtd_amd import Grid, Model, SoftSource
+    d = 40e-9
+    L = (n - 1) * d
+    grid = Grid(xmin=0, xmax=L, Nx=n, ymin=0, ymax=L, Ny=n, zmin=0, zmax=L, Nz=n)
+    src = SoftSource(geometry=(L / 2, L / 2, L / 2), waveform=lambda x, y, z, t: 1.0 + 0 * t,
+                     component="Ex")
+    dt = 1.5 / 3e8 / np.sqrt(3 / d ** 2)
+    return Model(grid, src, tmax=(steps - 1) * dt, CN=1.5, pml=0)
+
+
+def test_batch_and_server_isolate_a_failing_solve():
+    """A solve that goes non-finite is reported as such and drops out; the others run to
+    the end with the results they have alone.  The failing solve has a soft source, so
+    while it is in the batch the TFSF faces of the others take the per-face kernels
+    instead of the gathered pass: the fields stay bitwise equal, the Sa integral of the
+    face cells is summed in another order (round-off)."""
+    good = [m.describe() for m in small_3d(count=2, seed=31)]
+    bad = unstable_3d().describe()
+    b = Batch([good[0], bad, good[1]])
+    with pytest.raises(_lib.EngineError, match="I found (NaN|Inf) field values"):
+        b.run(0)
+    assert b.active == 0
+    assert b.handles[1].status in (_lib.MFD_ERR_NAN, _lib.MFD_ERR_INF)
+    for d, hb in ((good[0], b.handles[0]), (good[1], b.handles[2])):
+        assert hb.status == 0 and hb.it == d.Nt
+        ha = alone(d)
+        assert_same(hb, ha, 3, sa_tol=1e-6)
+        ha.close()
+    b.close()
+    srv = Server()
+    t0 = srv.submit(good[0])[0]
+    tb = srv.submit(bad)[0]
+    assert srv.wait(0, 120000) == _lib.MFD_REQ_DONE
+    st, code, _ = srv.poll(tb)
+    assert st == _lib.MFD_REQ_FAILED and code in (_lib.MFD_ERR_NAN, _lib.MFD_ERR_INF)
+    st, code, _ = srv.poll(t0)
+    assert st == _lib.MFD_REQ_DONE and code == 0
+    assert srv.stats()["failed"] == 1
+    srv.close()

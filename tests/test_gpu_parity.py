@@ -117,7 +117,7 @@ def test_slab_1d():
     assert_parity(hg, ho, desc, model)
 
 
-@pytest.mark.parametrize("kform", [2, 0])
+@pytest.mark.parametrize("kform", [2, 1, 0])
 def test_tfsf_faces_through_materials(kform):
     """TFSF as the only source (one gather launch per step; with the pipelined kernels
     the E kernel finishes x-face cells itself) with face cells holding stateless,
