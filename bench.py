@@ -111,7 +111,7 @@ def bench_3d(args, lib):
 
     h = Handle(desc, lib)
     set_opts(lib, h, {0: args.graph_steps, 1: args.zchunk, 5: args.ty, 6: args.kform,
-                      10: args.snake, 11: args.dless})
+                      10: args.snake, 11: args.dless, 12: args.pml_split})
     h.step(args.warmup)
     log(f"warmup {args.warmup} steps done")
     t0 = time.perf_counter()
@@ -145,7 +145,8 @@ def bench_3d(args, lib):
                                  f"{'off' if args.no_integrals else 'on'}",
                      "cells": N, "model_steps": desc.Nt, "graph_steps": args.graph_steps,
                      "zchunk": args.zchunk, "rows_per_workgroup": args.ty, "kernel_form": args.kform,
-                     "snake_order": args.snake, "plain_cells_without_d": args.dless}
+                     "snake_order": args.snake, "plain_cells_without_d": args.dless,
+                     "cpml_split": args.pml_split}
     out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                        "traffic": traffic,
@@ -409,6 +410,8 @@ def main():
     ap.add_argument("--kform", type=int, default=2)
     ap.add_argument("--snake", type=int, default=1)
     ap.add_argument("--dless", type=int, default=1, help="plain cells keep no D (default 1)")
+    ap.add_argument("--pml-split", type=int, default=0,
+                    help="tiles outside every CPML layer run the slab-free kernels (default 0)")
     ap.add_argument("--count", type=int, default=None, help="instances / requests")
     ap.add_argument("--step-scale", type=float, default=1.0)
     ap.add_argument("--chunk", type=int, default=200)

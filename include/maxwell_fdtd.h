@@ -202,7 +202,8 @@ enum {
     MFD_OPT_QUANTUM = 8,       /* server: steps per scheduling round (default 64)          */
     MFD_OPT_FILL_MCELLS = 9,   /* server: admit while active cells < value * 2^20 (48)     */
     MFD_OPT_SNAKE = 10,        /* E half walks the tile list backwards (default 1)         */
-    MFD_OPT_DLESS = 11         /* plain cells keep no D (rebuilt as E/Me on read; default 1) */
+    MFD_OPT_DLESS = 11,        /* plain cells keep no D (rebuilt as E/Me on read; default 1) */
+    MFD_OPT_PML_SPLIT = 12     /* tiles no CPML layer reaches run a slab-free kernel (0)   */
 };
 /* Changing an option rebuilds the launch plan; field state is kept.                   */
 int32_t mfd_model_set_options(mfd_model *m, int32_t opt, int32_t value);

@@ -155,8 +155,11 @@ constexpr int XFACE_ZC = 64;        // longest tile z range with the E kernel's 
 // launchers (defined in the .hip files)
 void launch_h3d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, hipStream_t s);
 void launch_e3d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, hipStream_t s);
-void launch_h3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int ty, hipStream_t s);
-void launch_e3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int ty, hipStream_t s);
+// pml = false: tiles that no CPML layer reaches (the kernels leave out the slab updates)
+void launch_h3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int ty, bool pml,
+                  hipStream_t s);
+void launch_e3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int ty,
+                  bool pml, hipStream_t s);
 void launch_h3d_v4(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int ty, hipStream_t s);
 void launch_e3d_v4(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int ty, hipStream_t s);
 void launch_h2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int tw, hipStream_t s);

@@ -685,26 +685,46 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     MFD_HIP(hipMalloc(&d_flags, hd.size() * sizeof(int)));
     MFD_HIP(hipHostMalloc(&h_flags, hd.size() * sizeof(int)));
     MFD_HIP(copy_sync(d_models, hd.data(), hd.size() * sizeof(DevModel), hipMemcpyHostToDevice, stream));
+    // pipelined 3D: tiles that no CPML layer reaches go to the slab-free kernels
+    std::vector<Tile> tl_pml, tl_int;
+    const bool split = ndims == 3 && kform == 2 && opt.pml_split;
+    for (const Tile &t : tiles) {
+        bool in = !split;
+        if (split) {
+            const DevModel &dm = hd[t.x];
+            const int x1 = std::min(dm.Nx, t.y + 64) - 1, y1 = std::min(dm.Ny, t.z + opt.ty) - 1;
+            const int z0 = tile_z0(t), z1 = tile_z1(t) - 1;
+            const DevLayer *L = dm.L;
+            in = (L[0].present && t.y <= L[0].ib) || (L[1].present && x1 >= L[1].ib) ||
+                 (L[2].present && t.z <= L[2].ib) || (L[3].present && y1 >= L[3].ib) ||
+                 (L[4].present && z0 <= L[4].ib) || (L[5].present && z1 >= L[5].ib);
+        }
+        (in ? tl_pml : tl_int).push_back(t);
+    }
     // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs, so block b
-    // runs on XCD b % 8.  Give every XCD one contiguous chunk of the tile list, so tiles
-    // that share halo rows and planes share that XCD's L2.
+    // runs on XCD b % 8.  Give every XCD one contiguous chunk of each tile list, so tiles
+    // that share halo rows and planes share that XCD's L2.  The E half-step walks each
+    // XCD's chunk backwards (and the lists in the opposite order), so the E kernel starts
+    // on the tiles the H kernel finished last (their H is still in the Infinity Cache),
+    // and the next H kernel, walking forwards, starts where E finished.
     const int nxcd = 8;
-    const int per = ((int)tiles.size() + nxcd - 1) / nxcd;
-    std::vector<Tile> perm((size_t)per * nxcd, make_int4(-1, 0, 0, 0));
-    for (int b = 0; b < per * nxcd; b++) {
-        const int src = (b % nxcd) * per + b / nxcd;
-        if (src < (int)tiles.size()) perm[b] = tiles[src];
-    }
+    auto order = [&](const std::vector<Tile> &in, bool backwards, std::vector<Tile> &out) {
+        if (in.empty()) return;
+        const int per = ((int)in.size() + nxcd - 1) / nxcd;
+        for (int b = 0; b < per * nxcd; b++) {
+            const int j = b / nxcd;
+            const int src = (b % nxcd) * per + (backwards ? per - 1 - j : j);
+            out.push_back(src < (int)in.size() ? in[src] : make_int4(-1, 0, 0, 0));
+        }
+    };
+    std::vector<Tile> perm, rev;
+    order(tl_pml, false, perm);
+    ntiles_pml = (int)perm.size();
+    order(tl_int, false, perm);
+    order(tl_int, opt.snake, rev);
+    order(tl_pml, opt.snake, rev);
+    if (!split) ntiles_pml = (int)perm.size();
     ntiles = (int)perm.size();
-    // E half-step order: each XCD walks its chunk backwards, so the E kernel starts on
-    // the tiles the H kernel finished last (their H is still in the Infinity Cache), and
-    // the next H kernel, walking forwards, starts where E finished
-    std::vector<Tile> rev(perm.size(), make_int4(-1, 0, 0, 0));
-    for (int b = 0; b < per * nxcd; b++) {
-        const int src = (b % nxcd) * per + (per - 1 - b / nxcd);
-        if (src < (int)tiles.size()) rev[b] = tiles[src];
-    }
-    if (!opt.snake) rev = perm;
     MFD_HIP(hipMalloc(&d_tiles, 2 * perm.size() * sizeof(Tile)));
     MFD_HIP(copy_sync(d_tiles, perm.data(), perm.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
     MFD_HIP(copy_sync(d_tiles + perm.size(), rev.data(), rev.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
@@ -716,6 +736,8 @@ int Runner::launch_step(bool timing) {
     const int nm = (int)models.size();
     int grid = ntiles;
     if (opt.persistent > 0) grid = std::min(ntiles, cu_count * opt.persistent);
+    const int np = ntiles_pml, ni = ntiles - ntiles_pml;
+    auto grid_of = [&](int n) { return opt.persistent > 0 ? std::min(n, cu_count * opt.persistent) : n; };
     if (timing && ev.empty()) {
         ev.resize(4);
         for (auto &e : ev) MFD_HIP(hipEventCreate(&e));
@@ -724,13 +746,19 @@ int Runner::launch_step(bool timing) {
     const int mode = any_full ? MODE_FULL : any_mats ? MODE_SIMPLE : MODE_VAC;
     const int hmode = h_mats ? MODE_SIMPLE : MODE_VAC;
     if (ndims == 3 && kform == 1) launch_h3d_v4(d_models, d_tiles, ntiles, grid, hmode, opt.ty, stream);
-    else if (ndims == 3 && kform == 2) launch_h3d_p(d_models, d_tiles, ntiles, grid, hmode, opt.ty, stream);
+    else if (ndims == 3 && kform == 2) {
+        launch_h3d_p(d_models, d_tiles, np, grid_of(np), hmode, opt.ty, true, stream);
+        launch_h3d_p(d_models, d_tiles + np, ni, grid_of(ni), hmode, opt.ty, false, stream);
+    }
     else if (ndims == 3) launch_h3d(d_models, d_tiles, ntiles, grid, hmode, stream);
     else if (ndims == 2) launch_h2d(d_models, d_tiles, ntiles, grid, hmode, opt.ty, stream);
     else launch_h1d(d_models, d_tiles, ntiles, grid, hmode, stream);
     if (timing) { MFD_HIP(hipEventRecord(ev[1], stream)); MFD_HIP(hipEventRecord(ev[2], stream)); }
     if (ndims == 3 && kform == 1) launch_e3d_v4(d_models, d_tiles_e, ntiles, grid, mode, any_int, opt.ty, stream);
-    else if (ndims == 3 && kform == 2) launch_e3d_p(d_models, d_tiles_e, ntiles, grid, mode, any_int, opt.ty, stream);
+    else if (ndims == 3 && kform == 2) {
+        launch_e3d_p(d_models, d_tiles_e, ni, grid_of(ni), mode, any_int, opt.ty, false, stream);
+        launch_e3d_p(d_models, d_tiles_e + ni, np, grid_of(np), mode, any_int, opt.ty, true, stream);
+    }
     else if (ndims == 3) launch_e3d(d_models, d_tiles_e, ntiles, grid, mode, any_int, stream);
     else if (ndims == 2) launch_e2d(d_models, d_tiles_e, ntiles, grid, mode, any_int, opt.ty, stream);
     else launch_e1d(d_models, d_tiles_e, ntiles, grid, mode, any_int, stream);
@@ -886,6 +914,7 @@ int set_option(RunOptions &o, Model *m, int opt, int value) {
             break;
         case MFD_OPT_SNAKE: o.snake = value != 0; break;
         case MFD_OPT_DLESS: o.dless = value != 0; break;
+        case MFD_OPT_PML_SPLIT: o.pml_split = value != 0; break;
         case MFD_OPT_STAGGER:
             if (!m || m->uploaded || value < 0) { set_error("stagger must be >= 0, set on a model before upload"); return MFD_ERR_ARG; }
             m->stagger = value;

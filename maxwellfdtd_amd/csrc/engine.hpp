@@ -131,6 +131,10 @@ struct RunOptions {
     bool kernel_timing = false; // hipEvent pairs around the H/E launches (eager mode)
     bool snake = true;          // E half walks each XCD's tile chunk backwards
     bool dless = true;          // plain cells keep no D where the run allows it
+    // pipelined 3D: tiles outside every CPML layer get launches of their own (off: with
+    // one launch, waves of CPML tiles, heavy in ALU work, share CUs with the memory-bound
+    // interior waves and the step is faster)
+    bool pml_split = false;
 };
 
 // Advances a fixed set of uploaded models together.
@@ -140,6 +144,9 @@ struct Runner {
     Tile *d_tiles = nullptr;    // H-half tile order
     Tile *d_tiles_e = nullptr;  // E-half tile order
     int ntiles = 0;
+    // pipelined 3D with the CPML split: d_tiles holds the ntiles_pml tiles some layer
+    // reaches, then the rest; d_tiles_e holds the rest, then the layer tiles
+    int ntiles_pml = 0;
     int ndims = 3;
     bool any_mats = false, any_full = false, any_int = false, any_mon = false, any_spectral = false;
     bool h_mats = false;        // some material has mu != 1: the H half reads the id

@@ -8,6 +8,10 @@
 // of loads are in flight per wave while registers stay few enough for a high wave count.
 // In-wave x neighbours come through a cross-lane shuffle.  Material coefficients come
 // from an LDS table.  All field pointers are accessed in the global address space.
+//
+// PML = false instantiations serve tiles that no CPML layer reaches: without the six
+// slab branches the kernels need about 50 fewer VGPRs (E: 118 -> 69), so a SIMD holds
+// more waves and more loads are in flight.
 #include "common.hpp"
 #include "material.hpp"
 #include "tfsf.hpp"
@@ -52,7 +56,7 @@ struct HP {
     unsigned char g;
 };
 
-template <int MODE, int TY>
+template <int MODE, int TY, bool PML>
 __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_h3d_p(const DevModel *__restrict__ models,
                                                    const Tile *__restrict__ tiles, int ntiles) {
     __shared__ float mh_tab[128];
@@ -84,7 +88,10 @@ __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(5, 8)))
             const unsigned xoff = ix + 1 < Nx ? 1u : (unsigned)(1 - Nx);
             LayP L[6];
 #pragma unroll
-            for (int f = 0; f < 6; f++) L[f] = layp(m.L[f], true);
+            for (int f = 0; f < 6; f++) {
+                L[f] = layp(m.L[f], true);
+                if (!PML) L[f].present = 0;
+            }
             const bool xs0 = L[0].present && ix <= L[0].ib, xs1 = L[1].present && ix >= L[1].ib;
             const bool ys0 = L[2].present && iy <= L[2].ib, ys1 = L[3].present && iy >= L[3].ib;
             auto load = [&](int iz, HP &P) {
@@ -176,7 +183,7 @@ struct EP {
     unsigned char g;
 };
 
-template <int MODE, bool INTEG, int TY>
+template <int MODE, bool INTEG, int TY, bool PML>
 __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ models,
                                                    const Tile *__restrict__ tiles, int ntiles) {
     // (Md1, Md2, Me, kind): kind 0 plain, 1 Kerr only, 2 ADE/plasma state (E of such
@@ -253,7 +260,10 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
             const unsigned xoff = ix > 0 ? (unsigned)-1 : (unsigned)(Nx - 1);
             LayP L[6];
 #pragma unroll
-            for (int f = 0; f < 6; f++) L[f] = layp(m.L[f], false);
+            for (int f = 0; f < 6; f++) {
+                L[f] = layp(m.L[f], false);
+                if (!PML) L[f].present = 0;
+            }
             const bool xs0 = L[0].present && ix <= L[0].ib, xs1 = L[1].present && ix >= L[1].ib;
             const bool ys0 = L[2].present && iy <= L[2].ib, ys1 = L[3].present && iy >= L[3].ib;
             // lanes on a TFSF x-face column (planes N1-1, N1, N2 of the x faces)
@@ -419,18 +429,18 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
     }
 }
 
-template <int TY>
+template <int TY, bool PML>
 static void lh(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, hipStream_t s) {
     dim3 block(64, TY);
-    if (mode == MODE_VAC) hipLaunchKernelGGL((k_h3d_p<MODE_VAC, TY>), dim3(grid), block, 0, s, m, tiles, ntiles);
-    else hipLaunchKernelGGL((k_h3d_p<MODE_SIMPLE, TY>), dim3(grid), block, 0, s, m, tiles, ntiles);
+    if (mode == MODE_VAC) hipLaunchKernelGGL((k_h3d_p<MODE_VAC, TY, PML>), dim3(grid), block, 0, s, m, tiles, ntiles);
+    else hipLaunchKernelGGL((k_h3d_p<MODE_SIMPLE, TY, PML>), dim3(grid), block, 0, s, m, tiles, ntiles);
 }
 
-template <int TY>
+template <int TY, bool PML>
 static void le(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ,
                hipStream_t s) {
     dim3 block(64, TY);
-#define EP_(MD, IN) hipLaunchKernelGGL((k_e3d_p<MD, IN, TY>), dim3(grid), block, 0, s, m, tiles, ntiles)
+#define EP_(MD, IN) hipLaunchKernelGGL((k_e3d_p<MD, IN, TY, PML>), dim3(grid), block, 0, s, m, tiles, ntiles)
     if (mode == MODE_VAC) { if (integ) EP_(MODE_VAC, true); else EP_(MODE_VAC, false); }
     else if (mode == MODE_SIMPLE) { if (integ) EP_(MODE_SIMPLE, true); else EP_(MODE_SIMPLE, false); }
     else { if (integ) EP_(MODE_FULL, true); else EP_(MODE_FULL, false); }
@@ -438,11 +448,14 @@ static void le(const DevModel *m, const Tile *tiles, int ntiles, int grid, int m
 }
 
 void launch_h3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int ty,
-                  hipStream_t s) {
-    if (ty == 1) lh<1>(m, tiles, ntiles, grid, mode, s);
-    else if (ty == 2) lh<2>(m, tiles, ntiles, grid, mode, s);
-    else if (ty == 4) lh<4>(m, tiles, ntiles, grid, mode, s);
-    else lh<8>(m, tiles, ntiles, grid, mode, s);
+                  bool pml, hipStream_t s) {
+    if (ntiles <= 0) return;
+#define LH_(T) (pml ? lh<T, true>(m, tiles, ntiles, grid, mode, s) : lh<T, false>(m, tiles, ntiles, grid, mode, s))
+    if (ty == 1) LH_(1);
+    else if (ty == 2) LH_(2);
+    else if (ty == 4) LH_(4);
+    else LH_(8);
+#undef LH_
 }
 
 // D = E/Me (the E kernel's own rebuild) for every plain cell of a model that ran
@@ -469,11 +482,15 @@ void launch_d_from_e(const DevModel &m, hipStream_t s) {
 }
 
 void launch_e3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ,
-                  int ty, hipStream_t s) {
-    if (ty == 1) le<1>(m, tiles, ntiles, grid, mode, integ, s);
-    else if (ty == 2) le<2>(m, tiles, ntiles, grid, mode, integ, s);
-    else if (ty == 4) le<4>(m, tiles, ntiles, grid, mode, integ, s);
-    else le<8>(m, tiles, ntiles, grid, mode, integ, s);
+                  int ty, bool pml, hipStream_t s) {
+    if (ntiles <= 0) return;
+#define LE_(T) (pml ? le<T, true>(m, tiles, ntiles, grid, mode, integ, s) \
+                    : le<T, false>(m, tiles, ntiles, grid, mode, integ, s))
+    if (ty == 1) LE_(1);
+    else if (ty == 2) LE_(2);
+    else if (ty == 4) LE_(4);
+    else LE_(8);
+#undef LE_
 }
 
 }  // namespace mfd

@@ -125,16 +125,20 @@ def beer_lambert_ok(Sa, z, ki, rtol=1e-2):
 
 # ---- seeded all-feature models ---------------------------------------------------------
 def spheres_3d(n=32, steps=120, seed=7, tfsf=True, soft=True, monitors=True, nspheres=5,
-               bc="periodic", pml=4):
-    """n^3 grid, CPML `pml` cells on every face, seeded dielectric spheres, a TFSF plane
-    wave and a soft point source."""
+               bc="periodic", pml=4, nx=None):
+    """n^3 grid (nx by n by n when nx is given), CPML `pml` cells on every face, seeded
+    dielectric spheres, a TFSF plane wave and a soft point source."""
     rng = np.random.default_rng(seed)
     d = 40e-9
     L = (n - 1) * d
-    grid = Grid(xmin=0, xmax=L, Nx=n, ymin=0, ymax=L, Ny=n, zmin=0, zmax=L, Nz=n)
+    Nx = n if nx is None else nx
+    Lx = (Nx - 1) * d
+    grid = Grid(xmin=0, xmax=Lx, Nx=Nx, ymin=0, ymax=L, Ny=n, zmin=0, zmax=L, Nz=n)
     mats = []
     for _ in range(nspheres):
         c = rng.uniform(0.3 * L, 0.7 * L, 3)
+        if nx is not None:
+            c[0] = c[0] * (Lx / L)
         r = rng.uniform(2, 5) * d
         eps = rng.uniform(1.5, 4.0)
         mats.append(Material(geometry=lambda x, y, z, c=c, r=r:
@@ -144,22 +148,23 @@ def spheres_3d(n=32, steps=120, seed=7, tfsf=True, soft=True, monitors=True, nsp
     srcs = []
     if tfsf:
         b0, b1 = (pml + 3) * d, L - (pml + 3) * d
-        srcs.append(TFSFSource(box=(b0, b1, b0, b1, b0, b1), waveform=w, component="Ex",
+        bx1 = Lx - (pml + 3) * d
+        srcs.append(TFSFSource(box=(b0, bx1, b0, b1, b0, b1), waveform=w, component="Ex",
                                z0=(pml + 1) * d))
     if soft:
-        srcs.append(SoftSource(geometry=(0.45 * L, 0.5 * L, 0.55 * L),
+        srcs.append(SoftSource(geometry=(0.45 * Lx, 0.5 * L, 0.55 * L),
                                waveform=lambda x, y, z, t: 1e3 * w(t) + 0 * x, component="Ey"))
     dt = 0.99 / C0 / np.sqrt(3 / d ** 2)
     model = Model(grid, srcs, tmax=(steps - 1) * dt, CN=0.99, pml=pml * d, material=mats, bc=bc)
     mons = None
     if monitors:
         w0 = 2 * np.pi * C0 / 800e-9
-        mons = [FieldMonitor((0.5 * L, 0.5 * L, 0.7 * L)),
-                FieldMonitor(lambda x, y, z: (np.abs(z - 0.7 * L) < d / 2) & (np.abs(x - 0.5 * L) < 2.5 * d)
+        mons = [FieldMonitor((0.5 * Lx, 0.5 * L, 0.7 * L)),
+                FieldMonitor(lambda x, y, z: (np.abs(z - 0.7 * L) < d / 2) & (np.abs(x - 0.5 * Lx) < 2.5 * d)
                              & (np.abs(y - 0.5 * L) < 2.5 * d), sum=True),
-                SpectralMonitor(lambda x, y, z: (np.abs(z - 0.75 * L) < d / 2) & (x > 0.3 * L) & (x < 0.7 * L)
+                SpectralMonitor(lambda x, y, z: (np.abs(z - 0.75 * L) < d / 2) & (x > 0.3 * Lx) & (x < 0.7 * Lx)
                                 & (y > 0.4 * L) & (y < 0.6 * L), w=w0 * np.linspace(0.8, 1.2, 5)),
-                SpectralMonitor((0.5 * L, 0.5 * L, 0.3 * L), w=w0 * np.linspace(0.9, 1.1, 3), sum=True)]
+                SpectralMonitor((0.5 * Lx, 0.5 * L, 0.3 * L), w=w0 * np.linspace(0.9, 1.1, 3), sum=True)]
     return model, mons
 
 

@@ -85,6 +85,21 @@ def test_spheres_3d_all_features():
     assert_parity(hg, ho, desc, model)
 
 
+def test_cpml_free_tiles():
+    """A grid long in x (160 cells) has 64-cell tiles that no CPML layer reaches; with the
+    CPML split they run the slab-free kernels.  The split changes which code serves a
+    cell, not its arithmetic: fields equal the unsplit run bitwise and the oracle within
+    the bar."""
+    model, mons = cases.spheres_3d(n=28, nx=160, steps=100)
+    hg, ho, desc = run_pair(model, mons, options={_lib.MFD_OPT_PML_SPLIT: 1})
+    assert_parity(hg, ho, desc, model)
+    whole = Handle(desc, _lib.engine())
+    whole.step(desc.Nt)
+    for c in ("Hx", "Hy", "Hz", "Ex", "Ey", "Ez", "Dx"):
+        assert np.array_equal(hg.field(c), whole.field(c)), c
+    assert np.array_equal(hg.output(0), whole.output(0))
+
+
 def test_spheres_3d_boundary_conditions():
     bc = ("dirichlet", "neumann", "neumann", "periodic", "periodic", "dirichlet")
     model, mons = cases.spheres_3d(n=24, steps=80, bc=bc, monitors=False, pml=0)
