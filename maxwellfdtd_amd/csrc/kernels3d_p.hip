@@ -288,8 +288,10 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
                 P.d0 = (dc ? Dsx : Esx)[i]; P.d1 = (dc ? Dsy : Esy)[i]; P.d2 = (dc ? Dsz : Esz)[i];
                 if (eload) { P.hyE = Hy[i + xoff]; P.hzE = Hz[i + xoff]; }
                 if (MODE != MODE_VAC || INTEG) P.g = id[i];
-                if (doSa) P.sa = Sa[i];
-                if (doE2) P.e2 = E2[i];
+                // the integrals are read and written once per step and not touched by the
+                // H half: streamed past the caches (nontemporal), leaving them to H and E
+                if (doSa) P.sa = __builtin_nontemporal_load(&Sa[i]);
+                if (doE2) P.e2 = __builtin_nontemporal_load(&E2[i]);
             };
             float hxm = 0.0f, hym = 0.0f;     // H(z-1)
             bool zm_own = false;
@@ -409,9 +411,9 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
                     if (doSa) {
                         const float a = e[1] * hzI - e[2] * hyI, b = e[2] * P.hx - e[0] * hzI;
                         const float cc = e[0] * hyI - e[1] * P.hx;
-                        Sa[i] = P.sa + sqrtf(a * a + b * b + cc * cc) * dtf;
+                        __builtin_nontemporal_store(P.sa + sqrtf(a * a + b * b + cc * cc) * dtf, &Sa[i]);
                     }
-                    if (doE2) E2[i] = P.e2 + (e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) * dtf;
+                    if (doE2) __builtin_nontemporal_store(P.e2 + (e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) * dtf, &E2[i]);
                 }
                 hxm = P.hx; hym = P.hy;
             };
