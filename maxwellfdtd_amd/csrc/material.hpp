@@ -51,13 +51,14 @@ __device__ __forceinline__ void material_e(MR &mt, int s, const float (&d)[NC],
             const float A = G(mt.Aq)[q], B = G(mt.Bq)[q], Cc = G(mt.Cq)[q];
 #pragma unroll
             for (int c = 0; c < NC; c++) {
+                // pole state is read and written once per step: streamed past the caches
                 gp<v2f> pp = (gp<v2f>)(mt.P + (size_t)(q * NC + c) * cnt + s);
-                const v2f p = *pp;
+                const v2f p = __builtin_nontemporal_load(pp);
                 const float pn = A * p.x + B * p.y + Cc * e[c];
                 v2f np;
                 np.x = pn;
                 np.y = p.x;
-                *pp = np;
+                __builtin_nontemporal_store(np, pp);
                 sumP[c] += pn;
             }
         }
@@ -69,18 +70,18 @@ __device__ __forceinline__ void material_e(MR &mt, int s, const float (&d)[NC],
         for (int c = 0; c < NC; c++) E2 += (double)e[c] * (double)e[c];
         const double II = mt.ksi * E2;
         gp<v2d> prd = (gp<v2d>)(mt.rhod + s);
-        v2d rd = *prd;
+        v2d rd = __builtin_nontemporal_load(prd);
         const float cp = (float)(mt.Cprho0 * rd.x);
         float J[NC];
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             gp<v2f> pp = (gp<v2f>)(mt.Pp + (size_t)c * cnt + s);
-            const v2f p = *pp;
+            const v2f p = __builtin_nontemporal_load(pp);
             const float pn = mt.Ap * p.x + mt.Bp * p.y + cp * e[c];
             v2f np;
             np.x = pn;
             np.y = p.x;
-            *pp = np;
+            __builtin_nontemporal_store(np, pp);
             sumP[c] += pn;
             J[c] = (pn - p.y) / (2.0f * dtf);
         }
@@ -89,8 +90,8 @@ __device__ __forceinline__ void material_e(MR &mt, int s, const float (&d)[NC],
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             gp<float> pa = G(mt.Pa) + (size_t)c * cnt + s;
-            const float v = *pa + ca * e[c];
-            *pa = v;
+            const float v = __builtin_nontemporal_load(pa) + ca * e[c];
+            __builtin_nontemporal_store(v, pa);
             sumP[c] += v;
         }
         const double R1 = ionrate(mt, II);
@@ -104,11 +105,11 @@ __device__ __forceinline__ void material_e(MR &mt, int s, const float (&d)[NC],
         }
         rd.x = rho;
         rd.y = R1 * (1.0 - rho);
-        *prd = rd;
+        __builtin_nontemporal_store(rd, prd);
         float je = 0.0f;
 #pragma unroll
         for (int c = 0; c < NC; c++) je += J[c] * e[c];
-        G(mt.JE)[s] += je * dtf;
+        __builtin_nontemporal_store(__builtin_nontemporal_load(&G(mt.JE)[s]) + je * dtf, &G(mt.JE)[s]);
     }
     const float Me = mt.Me;
     if (flags & MAT_KERR) {
