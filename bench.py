@@ -322,7 +322,7 @@ def bench_served_3d(args, lib):
     log(f"{len(reqs)} requests built in {time.perf_counter()-t0:.1f}s: {work/1e12:.3f} T cell-updates, "
         f"arrivals over {reqs[-1][0]:.2f} s")
 
-    srv = Server(options={8: args.quantum, 9: args.fill})
+    srv = Server(options={8: args.quantum, 9: args.fill, 13: args.sched})
     t0 = time.perf_counter()
     replay(srv, reqs, describe=lambda d: d)
     wall_s = time.perf_counter() - t0
@@ -355,7 +355,8 @@ def bench_served_3d(args, lib):
                                  "sphere, TFSF 800 nm, steps 1000..4000"
                                  + (f" x {args.step_scale}" if args.step_scale != 1.0 else ""),
                      "requests": len(reqs), "cell_updates": work, "quantum_steps": args.quantum,
-                     "fill_mcells": args.fill}
+                     "fill_mcells": args.fill,
+                     "server_admission": "least work first" if args.sched else "arrival order"}
     out["served"] = {"gcells_per_s": round(st["gcells_per_s"], 3), "p50_ms": round(st["p50_ms"], 1),
                      "p99_ms": round(st["p99_ms"], 1), "mean_ms": round(st["mean_ms"], 1),
                      "done": st["done"], "failed": st["failed"], "busy_s": round(st["busy_s"], 3)}
@@ -364,6 +365,7 @@ def bench_served_3d(args, lib):
                     "throughput": "cell-updates / (last completion - first arrival), "
                                   "as for the server"}
     out["speedup_vs_plain"] = round(st["gcells_per_s"] / g_plain, 3) if g_plain else None
+    out["latency_gain_vs_plain"] = {"p50": round(p50_p / st["p50_ms"], 3), "p99": round(p99_p / st["p99_ms"], 3)}
     out["roofline"] = None
     return out
 
@@ -417,7 +419,9 @@ def main():
     ap.add_argument("--chunk", type=int, default=200)
     ap.add_argument("--plain-count", type=int, default=0)
     ap.add_argument("--quantum", type=int, default=64)
-    ap.add_argument("--fill", type=int, default=48, help="server: active cells bound, Mi-cells")
+    ap.add_argument("--fill", type=int, default=16, help="server: active cells bound, Mi-cells")
+    ap.add_argument("--sched", type=int, default=1,
+                    help="server admission: 0 arrival order, 1 least work first (default 1)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU z-slab sharding is "

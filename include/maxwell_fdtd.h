@@ -200,10 +200,11 @@ enum {
     MFD_OPT_KERNEL_FORM = 6,   /* 3D kernel form: 0 scalar, 1 4-cell vector, 2 pipelined   */
     MFD_OPT_STAGGER = 7,       /* byte offset between field arrays (before upload)         */
     MFD_OPT_QUANTUM = 8,       /* server: steps per scheduling round (default 64)          */
-    MFD_OPT_FILL_MCELLS = 9,   /* server: admit while active cells < value * 2^20 (48)     */
+    MFD_OPT_FILL_MCELLS = 9,   /* server: admit while active cells < value * 2^20 (16)     */
     MFD_OPT_SNAKE = 10,        /* E half walks the tile list backwards (default 1)         */
     MFD_OPT_DLESS = 11,        /* plain cells keep no D (rebuilt as E/Me on read; default 1) */
-    MFD_OPT_PML_SPLIT = 12     /* tiles no CPML layer reaches run a slab-free kernel (0)   */
+    MFD_OPT_PML_SPLIT = 12,    /* tiles no CPML layer reaches run a slab-free kernel (0)   */
+    MFD_OPT_SCHED = 13         /* server: admit 0 in arrival order, 1 least work first (1) */
 };
 /* Changing an option rebuilds the launch plan; field state is kept.                   */
 int32_t mfd_model_set_options(mfd_model *m, int32_t opt, int32_t value);
@@ -242,9 +243,10 @@ int32_t mfd_batch_destroy(mfd_batch *b);
 
 /* ---- serving loop (no counterpart in the reference) ----------------------------------
  * A server owns a worker thread and a stream.  mfd_server_submit queues a built,
- * not-yet-stepped model; the worker admits queued models in arrival order while their
- * device footprint fits the HBM budget (and free HBM) and the active set holds fewer
- * than MFD_OPT_FILL_MCELLS * 2^20 cells (enough to fill the GPU), uploads them, and
+ * not-yet-stepped model; the worker admits queued models (least work, cells x Nt, first;
+ * or in arrival order with MFD_OPT_SCHED = 0) while their device footprint fits the HBM
+ * budget (and free HBM) and the active set holds fewer than MFD_OPT_FILL_MCELLS * 2^20
+ * cells (enough to fill the GPU), uploads them, and
  * advances the active set together in rounds of MFD_OPT_QUANTUM steps, so requests that
  * arrive while others run join between rounds.  A model that finishes leaves the tile list before it
  * is reported done; its device state stays until the caller destroys the model.
@@ -261,7 +263,7 @@ typedef struct mfd_server_stats {
 typedef struct mfd_server mfd_server;
 /* hbm_budget <= 0: free HBM minus 2 GiB.  max_active bounds the active set.           */
 int32_t mfd_server_create(int64_t hbm_budget, int32_t max_active, mfd_server **out);
-/* MFD_OPT_QUANTUM, FILL_MCELLS, GRAPH_STEPS, ZCHUNK, PERSISTENT, ROWS, KERNEL_FORM;
+/* MFD_OPT_QUANTUM, FILL_MCELLS, SCHED, GRAPH_STEPS, ZCHUNK, PERSISTENT, ROWS, KERNEL_FORM;
  * before the first submit.                                                            */
 int32_t mfd_server_set_options(mfd_server *s, int32_t opt, int32_t value);
 int32_t mfd_server_submit(mfd_server *s, mfd_model *m, int64_t *ticket);

@@ -224,7 +224,8 @@ struct mfd_server {
     int64_t budget = 0;
     int max_active = 64;
     int quantum = 64;
-    long long fill_cells = 48LL << 20;   // admit while the active set is below this size
+    long long fill_cells = 16LL << 20;   // admit while the active set is below this size
+    int sched = 1;                       // 0: arrival order, 1: least work (cells x steps) first
     long long active_cells = 0;
     int ndims = 0;
     RunOptions opt;
@@ -287,9 +288,16 @@ void mfd_server::loop() {
             hipMemGetInfo(&free_b, &total_b);
             int64_t reserve = (int64_t)1 << 30;
             while (!queue.empty() && (int)(active.size() + admitted.size()) < max_active) {
-                Request *r = queue.front();
+                // the next request: the first to arrive, or the one with the least work
+                // (a queued request has not started, so its work is cells x Nt)
+                auto pick = queue.begin();
+                if (sched == 1)
+                    for (auto it = queue.begin(); it != queue.end(); ++it)
+                        if ((double)(*it)->h->m.N * (*it)->h->m.d.Nt < (double)(*pick)->h->m.N * (*pick)->h->m.d.Nt)
+                            pick = it;
+                Request *r = *pick;
                 if (r->fp > budget) {
-                    queue.pop_front();
+                    queue.erase(pick);
                     in_use += r->fp;       // finish() subtracts it
                     lk.unlock();
                     finish(r, MFD_REQ_FAILED, MFD_ERR_NOMEM, "request exceeds the server's HBM budget");
@@ -300,7 +308,7 @@ void mfd_server::loop() {
                 // enough cells in flight to fill the GPU: later requests wait, so that
                 // requests finish close to arrival order instead of sharing the GPU
                 if (!(active.empty() && admitted.empty()) && active_cells + r->h->m.N > fill_cells) break;
-                queue.pop_front();
+                queue.erase(pick);
                 in_use += r->fp;
                 active_cells += r->h->m.N;
                 free_b -= (size_t)std::min<int64_t>((int64_t)free_b, r->fp);
@@ -402,6 +410,11 @@ int32_t mfd_server_set_options(mfd_server *s, int32_t opt, int32_t value) {
     if (opt == MFD_OPT_QUANTUM) {
         if (value < 1) { set_error("quantum must be >= 1"); return MFD_ERR_ARG; }
         s->quantum = value;
+        return MFD_OK;
+    }
+    if (opt == MFD_OPT_SCHED) {
+        if (value != 0 && value != 1) { set_error("schedule must be 0 (arrival order) or 1 (least work first)"); return MFD_ERR_ARG; }
+        s->sched = value;
         return MFD_OK;
     }
     if (opt == MFD_OPT_FILL_MCELLS) {
