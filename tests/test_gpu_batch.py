@@ -211,3 +211,25 @@ def test_batch_and_server_isolate_a_failing_solve():
     assert st == _lib.MFD_REQ_DONE and code == 0
     assert srv.stats()["failed"] == 1
     srv.close()
+
+
+def test_server_admits_least_work_first():
+    """With room for one request at a time (fill of 1 Mi cells, requests of 81^3 cells),
+    requests queued behind a running one are admitted least work first: the short
+    requests finish while the long one, submitted before them, has not."""
+    import cases
+    mk = lambda steps: cases.spheres_3d(n=81, steps=steps, monitors=False)[0].describe()
+    blocker, long_, short1, short2 = mk(6000), mk(400), mk(60), mk(120)
+    srv = Server(options={_lib.MFD_OPT_FILL_MCELLS: 1})
+    tb = srv.submit(blocker)[0]
+    tl = srv.submit(long_)[0]
+    t1 = srv.submit(short1)[0]
+    t2 = srv.submit(short2)[0]
+    assert srv.poll(tb)[0] == _lib.MFD_REQ_RUNNING     # the others were queued behind it
+    assert srv.wait(t1, 120000) == _lib.MFD_REQ_DONE
+    assert srv.poll(tl)[0] != _lib.MFD_REQ_DONE
+    assert srv.wait(t2, 120000) == _lib.MFD_REQ_DONE
+    assert srv.poll(tl)[0] != _lib.MFD_REQ_DONE
+    assert srv.wait(0, 120000) == _lib.MFD_REQ_DONE
+    assert srv.stats()["done"] == 4
+    srv.close()
