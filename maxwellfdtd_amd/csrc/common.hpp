@@ -40,6 +40,7 @@ struct DevLayer {
     int nb;
     int pad;
     const float *K, *A, *B;
+    const float *Ki;        // 1/K (rounded once from FP64), for the pipelined 3D kernels
     float *psiE0, *psiE1;   // H half-step psi (x: Eyx,Ezx  y: Exy,Ezy  z: Exz,Eyz)
     float *psiH0, *psiH1;   // E half-step psi (x: Hyx,Hzx  y: Hxy,Hzy  z: Hxz,Hyz)
 };

@@ -62,7 +62,7 @@ int64_t Model::footprint() const {
     }
     if (stateful) b += N * 4;
     for (int f = 0; f < 6; f++)
-        if (L[f].present) b += (int64_t)4 * psi_len(*this, f, L[f].nb) * 4 + 3 * L[f].nb * 4;
+        if (L[f].present) b += (int64_t)4 * psi_len(*this, f, L[f].nb) * 4 + 4 * L[f].nb * 4;
     for (const auto &s : srcs) {
         if (s.type == 0) b += (int64_t)s.cells.size() * (4 + 4LL * d.Nt);
         else {
@@ -314,6 +314,9 @@ int Model::upload() {
         l.nb = L[f].nb;
         float *p;
         UP(upload_vec(arena, &p, to_f32(L[f].K), io)); l.K = p;
+        std::vector<float> ki(L[f].K.size());
+        for (size_t q = 0; q < ki.size(); q++) ki[q] = (float)(1.0 / L[f].K[q]);
+        UP(upload_vec(arena, &p, ki, io)); l.Ki = p;
         UP(upload_vec(arena, &p, to_f32(L[f].A), io)); l.A = p;
         UP(upload_vec(arena, &p, to_f32(L[f].B), io)); l.B = p;
         const size_t n = psi_len(*this, f, l.nb);

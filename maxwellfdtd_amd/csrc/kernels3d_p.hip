@@ -26,13 +26,13 @@ typedef const __attribute__((address_space(1))) unsigned char gcu1;
 
 struct LayP {
     int present, ib, nb;
-    const float *K, *A, *B;
+    const float *Ki, *A, *B;     // 1/K, A, B
     float *p0, *p1;
 };
 __device__ __forceinline__ LayP layp(const DevLayer &l, bool h) {
     LayP r;
     r.present = l.present; r.ib = l.ib; r.nb = l.nb;
-    r.K = l.K; r.A = l.A; r.B = l.B;
+    r.Ki = l.Ki; r.A = l.A; r.B = l.B;
     r.p0 = h ? l.psiE0 : l.psiH0;
     r.p1 = h ? l.psiE1 : l.psiH1;
     return r;
@@ -41,13 +41,13 @@ __device__ __forceinline__ LayP layp(const DevLayer &l, bool h) {
 // one CPML slab update of a cell: psi = B psi + A d for the derivative pair, returns psi
 __device__ __forceinline__ void slab2(const LayP &L, int p, unsigned j, float &d0, float &d1,
                                       float &a0, float &a1) {
-    const float B = G(L.B)[p], A = G(L.A)[p], K = G(L.K)[p];
+    const float B = G(L.B)[p], A = G(L.A)[p], Ki = G(L.Ki)[p];
     a0 = B * G(L.p0)[j] + A * d0;
     a1 = B * G(L.p1)[j] + A * d1;
     G(L.p0)[j] = a0;
     G(L.p1)[j] = a1;
-    d0 = d0 / K;
-    d1 = d1 / K;
+    d0 = d0 * Ki;      // d / K, with 1/K rounded once on the host
+    d1 = d1 * Ki;
 }
 
 // ---------------------------------- H half ---------------------------------------------

@@ -222,10 +222,14 @@ def test_server_admits_least_work_first():
     blocker, long_, short1, short2 = mk(6000), mk(400), mk(60), mk(120)
     srv = Server(options={_lib.MFD_OPT_FILL_MCELLS: 1})
     tb = srv.submit(blocker)[0]
+    t0 = time.monotonic()
+    while srv.poll(tb)[0] == _lib.MFD_REQ_QUEUED and time.monotonic() - t0 < 10:
+        time.sleep(0.001)
+    assert srv.poll(tb)[0] == _lib.MFD_REQ_RUNNING
     tl = srv.submit(long_)[0]
     t1 = srv.submit(short1)[0]
     t2 = srv.submit(short2)[0]
-    assert srv.poll(tb)[0] == _lib.MFD_REQ_RUNNING     # the others were queued behind it
+    assert srv.poll(tb)[0] == _lib.MFD_REQ_RUNNING     # the others queue behind it
     assert srv.wait(t1, 120000) == _lib.MFD_REQ_DONE
     assert srv.poll(tl)[0] != _lib.MFD_REQ_DONE
     assert srv.wait(t2, 120000) == _lib.MFD_REQ_DONE
