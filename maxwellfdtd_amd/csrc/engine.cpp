@@ -663,13 +663,15 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     if (xface)
         for (size_t k = 0; k < ms.size(); k++) hd[k].xtfsf = ms[k]->dm.src;
     dless_models.clear();
-    // plain cells without stored D: pipelined 3D kernels, no ADE/Kerr/plasma in the run,
-    // TFSF (finished by the x-face path) or no source at all, and a state the caller has
-    // not written
+    // plain cells without stored D, for a state the caller has not written.  3D: the
+    // pipelined kernels, no ADE/Kerr/plasma in the run, TFSF (finished by the x-face
+    // path) or no source at all.  2D: any model; the kernel keeps D where the id says
+    // Kerr, ADE, plasma or source cell.
     for (size_t k = 0; k < ms.size(); k++) {
         Model *m = ms[k];
-        const bool ok = opt.dless && ndims == 3 && kform == 2 && !any_full && !m->user_set &&
-                        ((xface && m->srcs.size() == 1) || m->srcs.empty());
+        const bool ok3 = ndims == 3 && kform == 2 && !any_full &&
+                         ((xface && m->srcs.size() == 1) || m->srcs.empty());
+        const bool ok = opt.dless && !m->user_set && (ok3 || ndims == 2);
         if (!ok) {
             int st = m->materialize_d();
             if (st != MFD_OK) return st;

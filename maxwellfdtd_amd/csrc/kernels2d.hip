@@ -16,11 +16,12 @@ typedef __attribute__((address_space(1))) float gf1;
 typedef __attribute__((address_space(1))) const float gcf1;
 typedef const __attribute__((address_space(1))) unsigned char gcu1;
 
-// CPML slab update of one derivative: psi = B psi + A d, d /= K (src/models.jl:463-520)
+// CPML slab update of one derivative: psi = B psi + A d, d /= K (src/models.jl:463-520),
+// the division as a product with 1/K rounded once on the host
 __device__ __forceinline__ float slab1(const DevLayer &l, int p, unsigned j, float *psi, float &d) {
     const float a = G(l.B)[p] * G(psi)[j] + G(l.A)[p] * d;
     G(psi)[j] = a;
-    d = d / G(l.K)[p];
+    d = d * G(l.Ki)[p];
     return a;
 }
 
@@ -116,9 +117,10 @@ struct E2P {
 template <int MODE, bool INTEG, int TW>
 __global__ __launch_bounds__(64 * TW) void k_e2d(const DevModel *__restrict__ models,
                                                  const Tile *__restrict__ tiles, int ntiles) {
-    // (Md1, Md2, Me, kind): kind 0 plain, 1 Kerr only, 2 ADE/plasma state
+    // (Md1, Md2, Me, kind): kind 0 plain, 1 Kerr only, 2 ADE/plasma state; cw = Me Md2
     __shared__ float4 ctab[128];
     __shared__ float2 ktab[128];
+    __shared__ float cw[128];
     const int lane = threadIdx.x, w = threadIdx.y;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile tl = tiles[t];
@@ -130,6 +132,7 @@ __global__ __launch_bounds__(64 * TW) void k_e2d(const DevModel *__restrict__ mo
             const DevMat &mt = m.mats[k];
             const float kind = (mt.flags & (MAT_DISP | MAT_PLASMA)) ? 2.0f : (mt.flags & MAT_KERR) ? 1.0f : 0.0f;
             ctab[k] = make_float4(mt.Md1, mt.Md2, mt.Me, kind);
+            cw[k] = mt.MeMd2;
             if (MODE == MODE_FULL) ktab[k] = make_float2(mt.K2, mt.K3);
         }
         __syncthreads();
@@ -150,12 +153,18 @@ __global__ __launch_bounds__(64 * TW) void k_e2d(const DevModel *__restrict__ mo
             const unsigned xoff = ix > 0 ? (unsigned)-1 : (unsigned)(Nx - 1);
             const DevLayer &L0 = m.L[0], &L1 = m.L[1], &L4 = m.L[4], &L5 = m.L[5];
             const bool xs0 = L0.present && ix <= L0.ib, xs1 = L1.present && ix >= L1.ib;
+            // without stored D, a plain cell that no source writes reads E and takes the
+            // update in E form, E' = Md1 (E + Me Md2 psi) + Me Md2 curl H (Me times the D
+            // form); Kerr, ADE and source cells keep their D, loaded once the id is known
+            const bool dless = m.dless;
+            gcf1 *Dsx = G((const float *)m.F[DX]), *Dsz = G((const float *)m.F[DZ]);
+            gcf1 *Esx = G((const float *)m.F[EX]), *Esz = G((const float *)m.F[EZ]);
             auto load = [&](int iz, E2P &P) {
                 const unsigned i = ix + sz * iz;
                 P.hy = Hy[i];
-                P.d0 = Dx[i]; P.d1 = Dz[i];
+                P.d0 = (dless ? Esx : Dsx)[i]; P.d1 = (dless ? Esz : Dsz)[i];
                 if (eload) P.hyE = Hy[i + xoff];
-                if (MODE != MODE_VAC || INTEG) P.g = id[i];
+                if (MODE != MODE_VAC || INTEG || dless) P.g = id[i];
                 if (doSa) P.sa = Sa[i];
                 if (doE2) P.e2 = E2[i];
             };
@@ -175,10 +184,12 @@ __global__ __launch_bounds__(64 * TW) void k_e2d(const DevModel *__restrict__ mo
                 }
                 const int g = P.g & ID_MASK;
                 const float4 cf = MODE != MODE_VAC ? ctab[g] : ctab[0];
-                const float Md1 = cf.x, Md2 = cf.y;
+                const bool dc = !dless || (P.g & ID_TOUCHED) || (MODE == MODE_FULL && cf.w != 0.0f);
+                float d0 = P.d0, d1 = P.d1;
+                if (dless && dc) { d0 = Dsx[i]; d1 = Dsz[i]; }
+                const float Md1 = cf.x, Md2 = dc ? cf.y : (MODE != MODE_VAC ? cw[g] : cw[0]);
                 float dHyx = (P.hy - hyx) * idx;
                 float dHyz = (P.hy - hym) * idz;
-                float d0 = P.d0, d1 = P.d1;
                 if (xs0) d1 += Md2 * slab1(L0, ix, ix + (unsigned)L0.nb * iz, L0.psiH0, dHyx);
                 if (xs1) {
                     const int p = ix - L1.ib;
@@ -191,7 +202,7 @@ __global__ __launch_bounds__(64 * TW) void k_e2d(const DevModel *__restrict__ mo
                 }
                 d0 = Md1 * d0 + Md2 * (0.0f - dHyz);
                 d1 = Md1 * d1 + Md2 * (dHyx - 0.0f);
-                float e0 = d0 * cf.z, e1 = d1 * cf.z;
+                float e0 = dc ? d0 * cf.z : d0, e1 = dc ? d1 * cf.z : d1;
                 bool defer = false;
                 if (MODE == MODE_FULL && cf.w != 0.0f) {
                     if (cf.w > 1.5f) {
@@ -204,7 +215,7 @@ __global__ __launch_bounds__(64 * TW) void k_e2d(const DevModel *__restrict__ mo
                              (1.0f + 2.0f * kk.x * d1 + 3.0f * kk.y * (d1 * d1)) * d1 * cf.z;
                     }
                 }
-                Dx[i] = d0; Dz[i] = d1;
+                if (dc) { Dx[i] = d0; Dz[i] = d1; }
                 if (!defer) { Ex[i] = e0; Ez[i] = e1; }
                 check_finite(e0, m.flag, !defer);
                 if (INTEG && !defer && !(P.g & ID_TOUCHED)) {   // src/outputs.jl:322-325

@@ -478,9 +478,25 @@ __global__ __launch_bounds__(256) void k_d_from_e(const DevModel m) {
     }
 }
 
+// 2D: the cells that keep D are those with Kerr, ADE or plasma state and those a source
+// writes (the id's touched bit); every other cell gets D = E/Me
+__global__ __launch_bounds__(256) void k_d_from_e_2d(const DevModel m) {
+    const unsigned N = (unsigned)m.N;
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const unsigned char g = G(m.id)[i];
+        if (g & ID_TOUCHED) continue;
+        const DevMat &mt = m.mats[g & ID_MASK];
+        if (mt.flags & (MAT_DISP | MAT_PLASMA | MAT_KERR)) continue;
+        const float iMe = mt.iMe;
+        G(m.F[DX])[i] = G((const float *)m.F[EX])[i] * iMe;
+        G(m.F[DZ])[i] = G((const float *)m.F[EZ])[i] * iMe;
+    }
+}
+
 void launch_d_from_e(const DevModel &m, hipStream_t s) {
     const long long nb = std::min<long long>((m.N + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_d_from_e, dim3((unsigned)nb), dim3(256), 0, s, m);
+    if (m.ndims == 2) hipLaunchKernelGGL(k_d_from_e_2d, dim3((unsigned)nb), dim3(256), 0, s, m);
+    else hipLaunchKernelGGL(k_d_from_e, dim3((unsigned)nb), dim3(256), 0, s, m);
 }
 
 void launch_e3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ,

@@ -163,6 +163,26 @@ def test_plain_tfsf_without_stored_d():
         assert np.array_equal(hg.field(c), mid.field(c)), c
 
 
+def test_2d_without_stored_d():
+    """2D keeps no D for plain cells that no source writes (D = E/Me, rebuilt on read);
+    Lorentz rods, TFSF and soft-source cells keep theirs.  Parity against the oracle on
+    H, D and E, the run with stored D agrees to rounding, and reading D mid-run does not
+    change the result."""
+    model, mons = cases.lorentz_2d()
+    hg, ho, desc = run_pair(model, mons)
+    assert_parity(hg, ho, desc, model)
+    full = Handle(desc, _lib.engine())
+    full.lib.check(full.lib.model_set_options(full.h, _lib.MFD_OPT_DLESS, 0), "set_options")
+    full.step(desc.Nt)
+    mid = Handle(desc, _lib.engine())
+    mid.step(desc.Nt // 2)
+    mid.field("Dz")
+    mid.step(desc.Nt - desc.Nt // 2)
+    for c in ("Hy", "Dx", "Dz", "Ex", "Ez"):
+        assert rel_l2(hg.field(c), full.field(c)) <= 1e-5, c
+        assert np.array_equal(hg.field(c), mid.field(c)), c
+
+
 def test_no_drift_without_stored_d():
     """A closed, lossless box (periodic, no CPML) keeps the field for good: over 3000
     steps the run without stored D stays within rounding of the run with stored D and of
