@@ -663,22 +663,19 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     if (xface)
         for (size_t k = 0; k < ms.size(); k++) hd[k].xtfsf = ms[k]->dm.src;
     dless_models.clear();
-    // plain cells without stored D, for a state the caller has not written.  3D: the
-    // pipelined kernels, no ADE/Kerr/plasma in the run, TFSF (finished by the x-face
-    // path) or no source at all.  2D: any model; the kernel keeps D where the id says
-    // Kerr, ADE, plasma or source cell.
+    // plain cells without stored D, for a state the caller has not written, on the
+    // pipelined 3D kernels and the 2D kernels: D stays stored where the id says Kerr,
+    // ADE, plasma or source cell (and, in 3D, on the TFSF faces the x-face path finishes)
     for (size_t k = 0; k < ms.size(); k++) {
         Model *m = ms[k];
-        const bool ok3 = ndims == 3 && kform == 2 && !any_full &&
-                         ((xface && m->srcs.size() == 1) || m->srcs.empty());
-        const bool ok = opt.dless && !m->user_set && (ok3 || ndims == 2);
+        const bool ok = opt.dless && !m->user_set && ((ndims == 3 && kform == 2) || ndims == 2);
         if (!ok) {
             int st = m->materialize_d();
             if (st != MFD_OK) return st;
             continue;
         }
         hd[k].dless = 1;
-        m->d_tfsf = !m->srcs.empty();
+        m->d_tfsf = hd[k].xtfsf != nullptr;
         dless_models.push_back(m);
     }
     src_kind_mask.assign(maxsrc, 0);

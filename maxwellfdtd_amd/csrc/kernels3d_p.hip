@@ -190,6 +190,7 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
     // cells is finished by the state pass, k_e_state)
     __shared__ float4 ctab[128];
     __shared__ float2 ktab[128];    // Kerr Pade (K2, K3)
+    __shared__ float cw[128];       // Me*Md2 (the E-form update without D), full mode
     // incident values of the TFSF x faces for this tile's rows and planes:
     // [side][H_u, H_v, E_u, E_v][row][plane - z0]
     __shared__ float xinc[2][4][TY][XFACE_ZC];
@@ -205,7 +206,7 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
             const float kind = (mt.flags & (MAT_DISP | MAT_PLASMA)) ? 2.0f : (mt.flags & MAT_KERR) ? 1.0f : 0.0f;
             // w: the material kind with ADE/Kerr/plasma, else Me*Md2 for the runs without D
             ctab[k] = make_float4(mt.Md1, mt.Md2, mt.Me, MODE == MODE_FULL ? kind : mt.MeMd2);
-            if (MODE == MODE_FULL) ktab[k] = make_float2(mt.K2, mt.K3);
+            if (MODE == MODE_FULL) { ktab[k] = make_float2(mt.K2, mt.K3); cw[k] = mt.MeMd2; }
         }
         // TFSF x faces finished in this kernel: the source's scalars come into registers
         // once per tile and the incident values of the tile's x-face cells into LDS, so
@@ -270,11 +271,12 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
             const bool xcol = xtile && (ix == xb[0] - 1 || ix == xb[0] || ix == xb[1]) && iy >= xb[2] &&
                               iy <= xb[3];
             const int xside = ix == xb[1] ? 1 : 0;
-            // without stored D (plain materials, TFSF as the only source), a plain cell
-            // reads E and takes the update in E form, E' = Md1 (E + Me Md2 psi) + Me Md2
-            // curl H, which is Me times the D form; the TFSF face cells keep their D,
-            // which the face corrections change apart from E
-            const bool dless = MODE != MODE_FULL && m.dless;
+            // without stored D, a plain cell reads E and takes the update in E form,
+            // E' = Md1 (E + Me Md2 psi) + Me Md2 curl H, which is Me times the D form.
+            // Cells that keep their D: the TFSF face cells, whose D the face corrections
+            // change apart from E (known from the plane, so D is loaded ahead), and Kerr,
+            // ADE, plasma and point-source cells (known from the id: D loaded on use)
+            const bool dless = m.dless;
             auto dcell = [&](int iz) { return !dless || tfsf_touched_box(xb, ix, iy, iz); };
             gcf1 *Dsx = G((const float *)m.F[DX]), *Dsy = G((const float *)m.F[DY]);
             gcf1 *Dsz = G((const float *)m.F[DZ]);
@@ -287,7 +289,7 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
                 const bool dc = dcell(iz);
                 P.d0 = (dc ? Dsx : Esx)[i]; P.d1 = (dc ? Dsy : Esy)[i]; P.d2 = (dc ? Dsz : Esz)[i];
                 if (eload) { P.hyE = Hy[i + xoff]; P.hzE = Hz[i + xoff]; }
-                if (MODE != MODE_VAC || INTEG) P.g = id[i];
+                if (MODE != MODE_VAC || INTEG || dless) P.g = id[i];
                 // the integrals are read and written once per step and not touched by the
                 // H half: streamed past the caches (nontemporal), leaving them to H and E
                 if (doSa) P.sa = __builtin_nontemporal_load(&Sa[i]);
@@ -313,8 +315,12 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
                 }
                 const int g = P.g & ID_MASK;
                 const float4 cf = MODE != MODE_VAC ? ctab[g] : ctab[0];
-                const bool dc = dcell(iz);
-                const float Md1 = cf.x, Md2 = dc ? cf.y : cf.w;
+                bool dc = dcell(iz);
+                if (dless && !dc && ((P.g & ID_TOUCHED) || (MODE == MODE_FULL && cf.w != 0.0f))) {
+                    dc = true;
+                    P.d0 = Dsx[i]; P.d1 = Dsy[i]; P.d2 = Dsz[i];
+                }
+                const float Md1 = cf.x, Md2 = dc ? cf.y : (MODE == MODE_FULL ? cw[g] : cf.w);
                 float dHxy = (P.hx - P.hxy) * idy;
                 float dHxz = (P.hx - hxm) * idz;
                 float dHyx = (P.hy - hyx) * idx;
@@ -471,7 +477,11 @@ __global__ __launch_bounds__(256) void k_d_from_e(const DevModel m) {
         const int ix = (int)(i % (unsigned)m.Nx), iy = (int)((i / (unsigned)m.Nx) % (unsigned)m.Ny);
         const int iz = (int)(i / ((unsigned)m.Nx * (unsigned)m.Ny));
         if (tfsf_touched_box(xb, ix, iy, iz)) continue;
-        const float iMe = G(m.mats)[G(m.id)[i] & ID_MASK].iMe;
+        const unsigned char g = G(m.id)[i];
+        if (g & ID_TOUCHED) continue;
+        const DevMat &mt = m.mats[g & ID_MASK];
+        if (mt.flags & (MAT_DISP | MAT_PLASMA | MAT_KERR)) continue;
+        const float iMe = mt.iMe;
         G(m.F[DX])[i] = G((const float *)m.F[EX])[i] * iMe;
         G(m.F[DY])[i] = G((const float *)m.F[EY])[i] * iMe;
         G(m.F[DZ])[i] = G((const float *)m.F[EZ])[i] * iMe;

@@ -183,6 +183,32 @@ def test_2d_without_stored_d():
         assert np.array_equal(hg.field(c), mid.field(c)), c
 
 
+@pytest.mark.parametrize("case", ["dispersive", "nonlinear", "soft"])
+def test_3d_without_stored_d(case):
+    """3D keeps no D for plain cells: Drude/Lorentz/Debye, Kerr + plasma and point-source
+    cells keep theirs (chosen by the id, D loaded on use), TFSF face cells too.  Parity
+    against the oracle, agreement with the stored-D run to rounding, and a mid-run read
+    of D that does not change the result."""
+    if case == "dispersive":
+        model, mons = cases.dispersive_3d(), None
+    elif case == "nonlinear":
+        model, mons = cases.nonlinear_3d(), None
+    else:
+        model, mons = cases.spheres_3d(n=32, steps=100)
+    hg, ho, desc = run_pair(model, mons)
+    assert_parity(hg, ho, desc, model)
+    full = Handle(desc, _lib.engine())
+    full.lib.check(full.lib.model_set_options(full.h, _lib.MFD_OPT_DLESS, 0), "set_options")
+    full.step(desc.Nt)
+    mid = Handle(desc, _lib.engine())
+    mid.step(desc.Nt // 2)
+    mid.field("Dy")
+    mid.step(desc.Nt - desc.Nt // 2)
+    for c in ("Hx", "Hz", "Dx", "Dy", "Dz", "Ex", "Ey", "Ez"):
+        assert rel_l2(hg.field(c), full.field(c)) <= 1e-5, c
+        assert np.array_equal(hg.field(c), mid.field(c)), c
+
+
 def test_no_drift_without_stored_d():
     """A closed, lossless box (periodic, no CPML) keeps the field for good: over 3000
     steps the run without stored D stays within rounding of the run with stored D and of
