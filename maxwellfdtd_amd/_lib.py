@@ -14,6 +14,15 @@ ENGINE_LIB = os.path.join(HERE, "libmaxwellfdtd_amd.so")
 
 MFD_OK, MFD_ERR_ARG, MFD_ERR_STATE, MFD_ERR_HIP = 0, -1, -2, -3
 MFD_ERR_NOMEM, MFD_ERR_NAN, MFD_ERR_INF, MFD_ERR_NODEV = -4, -5, -6, -7
+# field components, boundary conditions, CPML faces, sources, ionisation rates, integrals
+MFD_HX, MFD_HY, MFD_HZ, MFD_DX, MFD_DY, MFD_DZ, MFD_EX, MFD_EY, MFD_EZ = range(9)
+MFD_BC_PERIODIC, MFD_BC_DIRICHLET, MFD_BC_NEUMANN = 1, 2, 3
+MFD_FACE_XL, MFD_FACE_XR, MFD_FACE_YL, MFD_FACE_YR, MFD_FACE_ZL, MFD_FACE_ZR = range(6)
+MFD_SRC_SOFT, MFD_SRC_HARD = 0, 1
+MFD_TFSF_TABLES, MFD_TFSF_AUX1D = 0, 1
+MFD_RATE_NONE, MFD_RATE_POWER, MFD_RATE_LOGLOG = 0, 1, 2
+MFD_INT_SA, MFD_INT_E2 = 1, 2
+MFD_OUT_SA, MFD_OUT_E2, MFD_OUT_JE, MFD_OUT_RHO = 0, 1, 2, 3
 NAN_MSG = "Something went wrong. I found NaN field values."
 INF_MSG = "Something went wrong. I found Inf field values."
 

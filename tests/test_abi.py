@@ -169,6 +169,8 @@ def test_batch_handle_and_option_checks(engine_so):
         assert lib.batch_set_options(b, _lib.MFD_OPT_DFT_BLOCK, 16) == _lib.MFD_ERR_ARG
         assert "per-model option" in lib.error()
         assert lib.batch_set_options(b, _lib.MFD_OPT_ZCHUNK, 32) == _lib.MFD_OK
+        assert lib.batch_set_options(b, _lib.MFD_OPT_PML_SPLIT, 1) == _lib.MFD_OK
+        assert lib.batch_set_options(b, _lib.MFD_OPT_SCHED, 1) == _lib.MFD_ERR_ARG   # server only
         assert lib.batch_set_options(b, 99, 1) == _lib.MFD_ERR_ARG
         n = C.c_int32(-1)
         assert lib.batch_active(b, C.byref(n)) == _lib.MFD_OK and n.value == 0
@@ -186,3 +188,26 @@ def test_server_needs_a_gpu(engine_so):
     assert lib.server_create(0, 4, None) == _lib.MFD_ERR_ARG
     assert lib.server_create(0, 4, C.byref(s)) != _lib.MFD_OK
     assert lib.error()
+
+
+def test_header_constants_match_binding():
+    """Every MFD_* enumerator the header declares has the same value in the Python
+    binding (the ctypes twin of the Julia shim)."""
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "maxwell_fdtd.h")).read(),
+                  flags=re.S)
+    found = {}
+    for body in re.findall(r"enum\s*\{(.*?)\}", text, flags=re.S):
+        nxt = 0
+        for item in body.split(","):
+            item = item.strip()
+            if not item:
+                continue
+            name, _, val = item.partition("=")
+            nxt = int(val) if val.strip() else nxt
+            found[name.strip()] = nxt
+            nxt += 1
+    assert len(found) > 50
+    missing = [k for k in found if not hasattr(_lib, k)]
+    assert not missing, missing
+    wrong = {k: (v, getattr(_lib, k)) for k, v in found.items() if getattr(_lib, k) != v}
+    assert not wrong, wrong
