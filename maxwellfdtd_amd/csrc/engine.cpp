@@ -598,7 +598,7 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     MFD_HIP(hipGetDevice(&dev));
     MFD_HIP(hipDeviceGetAttribute(&cu_count, hipDeviceAttributeMultiprocessorCount, dev));
     std::vector<DevModel> hd;
-    std::vector<Tile> tiles;
+    std::vector<Tile> tiles, tiles_h;
     any_mats = any_full = any_int = any_mon = any_spectral = h_mats = false;
     maxtouched = 1;
     maxsrc = 0;
@@ -617,6 +617,7 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
         if (st != MFD_OK) return st;
         hd.push_back(m->dm);
         make_tiles(*m, (int)k, opt.zchunk, opt.ty, kform, tiles);
+        if (ndims == 3 && kform == 2 && opt.zchunk_h > 0) make_tiles(*m, (int)k, opt.zchunk_h, opt.ty, kform, tiles_h);
         any_mats |= m->has_mats;
         for (const auto &hm : m->mats) {
             any_full |= hm.d.nq > 0 || hm.d.iskerr || hm.d.isplasma;
@@ -688,21 +689,28 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     MFD_HIP(hipHostMalloc(&h_flags, hd.size() * sizeof(int)));
     MFD_HIP(copy_sync(d_models, hd.data(), hd.size() * sizeof(DevModel), hipMemcpyHostToDevice, stream));
     // pipelined 3D: tiles that no CPML layer reaches go to the slab-free kernels
-    std::vector<Tile> tl_pml, tl_int;
     const bool split = ndims == 3 && kform == 2 && opt.pml_split;
-    for (const Tile &t : tiles) {
-        bool in = !split;
-        if (split) {
-            const DevModel &dm = hd[t.x];
-            const int x1 = std::min(dm.Nx, t.y + 64) - 1, y1 = std::min(dm.Ny, t.z + opt.ty) - 1;
-            const int z0 = tile_z0(t), z1 = tile_z1(t) - 1;
-            const DevLayer *L = dm.L;
-            in = (L[0].present && t.y <= L[0].ib) || (L[1].present && x1 >= L[1].ib) ||
-                 (L[2].present && t.z <= L[2].ib) || (L[3].present && y1 >= L[3].ib) ||
-                 (L[4].present && z0 <= L[4].ib) || (L[5].present && z1 >= L[5].ib);
+    auto classify = [&](const std::vector<Tile> &all, std::vector<Tile> &tl_pml, std::vector<Tile> &tl_int) {
+        for (const Tile &t : all) {
+            bool in = !split;
+            if (split) {
+                const DevModel &dm = hd[t.x];
+                const int x1 = std::min(dm.Nx, t.y + 64) - 1, y1 = std::min(dm.Ny, t.z + opt.ty) - 1;
+                const int z0 = tile_z0(t), z1 = tile_z1(t) - 1;
+                const DevLayer *L = dm.L;
+                in = (L[0].present && t.y <= L[0].ib) || (L[1].present && x1 >= L[1].ib) ||
+                     (L[2].present && t.z <= L[2].ib) || (L[3].present && y1 >= L[3].ib) ||
+                     (L[4].present && z0 <= L[4].ib) || (L[5].present && z1 >= L[5].ib);
+            }
+            (in ? tl_pml : tl_int).push_back(t);
         }
-        (in ? tl_pml : tl_int).push_back(t);
-    }
+    };
+    // the H half may march its own z chunk (MFD_OPT_ZCHUNK_H); both lists cover the grid
+    // in the same z-major order, so the snake order still pairs E tiles with the H
+    // tiles finished last
+    std::vector<Tile> tl_pml, tl_int, th_pml, th_int;
+    classify(tiles, tl_pml, tl_int);
+    classify(tiles_h.empty() ? tiles : tiles_h, th_pml, th_int);
     // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs, so block b
     // runs on XCD b % 8.  Give every XCD one contiguous chunk of each tile list, so tiles
     // that share halo rows and planes share that XCD's L2.  The E half-step walks each
@@ -720,14 +728,15 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
         }
     };
     std::vector<Tile> perm, rev;
-    order(tl_pml, false, perm);
-    ntiles_pml = (int)perm.size();
-    order(tl_int, false, perm);
+    order(th_pml, false, perm);
+    ntiles_pml_h = (int)perm.size();
+    order(th_int, false, perm);
     order(tl_int, opt.snake, rev);
+    ntiles_int_e = (int)rev.size();
     order(tl_pml, opt.snake, rev);
-    if (!split) ntiles_pml = (int)perm.size();
-    ntiles = (int)perm.size();
-    MFD_HIP(hipMalloc(&d_tiles, 2 * perm.size() * sizeof(Tile)));
+    ntiles_h = (int)perm.size();
+    ntiles = (int)rev.size();
+    MFD_HIP(hipMalloc(&d_tiles, (perm.size() + rev.size()) * sizeof(Tile)));
     MFD_HIP(copy_sync(d_tiles, perm.data(), perm.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
     MFD_HIP(copy_sync(d_tiles + perm.size(), rev.data(), rev.size() * sizeof(Tile), hipMemcpyHostToDevice, stream));
     d_tiles_e = d_tiles + perm.size();
@@ -738,7 +747,11 @@ int Runner::launch_step(bool timing) {
     const int nm = (int)models.size();
     int grid = ntiles;
     if (opt.persistent > 0) grid = std::min(ntiles, cu_count * opt.persistent);
-    const int np = ntiles_pml, ni = ntiles - ntiles_pml;
+    // pipelined 3D: H list = ntiles_pml_h CPML tiles then the rest (ntiles_h in all);
+    // E list = ntiles_int_e CPML-free tiles then the rest (ntiles in all); without the
+    // CPML split every tile counts as a CPML tile
+    const int nph = ntiles_pml_h, nih = ntiles_h - ntiles_pml_h;
+    const int nie = ntiles_int_e, npe = ntiles - ntiles_int_e;
     auto grid_of = [&](int n) { return opt.persistent > 0 ? std::min(n, cu_count * opt.persistent) : n; };
     if (timing && ev.empty()) {
         ev.resize(4);
@@ -749,8 +762,8 @@ int Runner::launch_step(bool timing) {
     const int hmode = h_mats ? MODE_SIMPLE : MODE_VAC;
     if (ndims == 3 && kform == 1) launch_h3d_v4(d_models, d_tiles, ntiles, grid, hmode, opt.ty, stream);
     else if (ndims == 3 && kform == 2) {
-        launch_h3d_p(d_models, d_tiles, np, grid_of(np), hmode, opt.ty, true, stream);
-        launch_h3d_p(d_models, d_tiles + np, ni, grid_of(ni), hmode, opt.ty, false, stream);
+        launch_h3d_p(d_models, d_tiles, nph, grid_of(nph), hmode, opt.ty, true, stream);
+        launch_h3d_p(d_models, d_tiles + nph, nih, grid_of(nih), hmode, opt.ty, false, stream);
     }
     else if (ndims == 3) launch_h3d(d_models, d_tiles, ntiles, grid, hmode, stream);
     else if (ndims == 2) launch_h2d(d_models, d_tiles, ntiles, grid, hmode, opt.ty, stream);
@@ -758,8 +771,8 @@ int Runner::launch_step(bool timing) {
     if (timing) { MFD_HIP(hipEventRecord(ev[1], stream)); MFD_HIP(hipEventRecord(ev[2], stream)); }
     if (ndims == 3 && kform == 1) launch_e3d_v4(d_models, d_tiles_e, ntiles, grid, mode, any_int, opt.ty, stream);
     else if (ndims == 3 && kform == 2) {
-        launch_e3d_p(d_models, d_tiles_e, ni, grid_of(ni), mode, any_int, opt.ty, false, stream);
-        launch_e3d_p(d_models, d_tiles_e + ni, np, grid_of(np), mode, any_int, opt.ty, true, stream);
+        launch_e3d_p(d_models, d_tiles_e, nie, grid_of(nie), mode, any_int, opt.ty, false, stream);
+        launch_e3d_p(d_models, d_tiles_e + nie, npe, grid_of(npe), mode, any_int, opt.ty, true, stream);
     }
     else if (ndims == 3) launch_e3d(d_models, d_tiles_e, ntiles, grid, mode, any_int, stream);
     else if (ndims == 2) launch_e2d(d_models, d_tiles_e, ntiles, grid, mode, any_int, opt.ty, stream);
@@ -899,6 +912,10 @@ int set_option(RunOptions &o, Model *m, int opt, int value) {
         case MFD_OPT_ZCHUNK:
             if (value < 1 || value > 65535) { set_error("bad z chunk"); return MFD_ERR_ARG; }
             o.zchunk = value;
+            break;
+        case MFD_OPT_ZCHUNK_H:
+            if (value < 0 || value > 65535) { set_error("bad H z chunk"); return MFD_ERR_ARG; }
+            o.zchunk_h = value;
             break;
         case MFD_OPT_PERSISTENT: o.persistent = std::max(0, value); break;
         case MFD_OPT_KERNEL_TIMING: o.kernel_timing = value != 0; break;

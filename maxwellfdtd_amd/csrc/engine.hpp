@@ -125,6 +125,7 @@ struct Model {
 struct RunOptions {
     int graph_steps = 16;       // steps captured per hipGraph replay (0 = eager launches)
     int zchunk = 16;            // z planes per tile (3D/2D)
+    int zchunk_h = 8;           // pipelined 3D H half: its own z planes per tile (0 = zchunk)
     int ty = 4;                 // rows per workgroup of the 3D vector kernels (1, 2, 4, 8)
     int vec = 2;                // 3D kernel form: 0 scalar, 1 vector (Nx % 4 == 0), 2 pipelined
     int persistent = 0;         // 0: one workgroup per tile; >0: that many workgroups per CU
@@ -144,9 +145,10 @@ struct Runner {
     Tile *d_tiles = nullptr;    // H-half tile order
     Tile *d_tiles_e = nullptr;  // E-half tile order
     int ntiles = 0;
-    // pipelined 3D with the CPML split: d_tiles holds the ntiles_pml tiles some layer
-    // reaches, then the rest; d_tiles_e holds the rest, then the layer tiles
-    int ntiles_pml = 0;
+    // pipelined 3D: d_tiles (ntiles_h) holds the ntiles_pml_h tiles some CPML layer
+    // reaches, then the rest; d_tiles_e (ntiles) holds the ntiles_int_e CPML-free
+    // tiles, then the layer tiles (without the CPML split, all tiles are layer tiles)
+    int ntiles_h = 0, ntiles_pml_h = 0, ntiles_int_e = 0;
     int ndims = 3;
     bool any_mats = false, any_full = false, any_int = false, any_mon = false, any_spectral = false;
     bool h_mats = false;        // some material has mu != 1: the H half reads the id
