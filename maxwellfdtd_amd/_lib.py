@@ -145,6 +145,7 @@ ENGINE_EXTRA = {
     "model_set_options": (I32, [P, I32, I32]),
     "model_timing": (I32, [P, I32, DP, IP]),
     "stream_copy": (I32, [C.c_int64, I32, DP]),
+    "stream_copy_fast": (I32, [C.c_int64, I32, DP]),
     "stream_multi": (I32, [C.c_int64, I32, I32, I32, I32, DP]),
     "batch_create": (I32, [C.POINTER(P)]),
     "batch_add": (I32, [P, P]),

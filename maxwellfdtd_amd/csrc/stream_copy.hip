@@ -16,9 +16,40 @@ __global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src,
         dst[i] = src[i];
 }
 
+// 4 float4 per lane in flight over block-contiguous chunks, nontemporal both ways
+__global__ __launch_bounds__(256) void k_copy_fast(const float4 *__restrict__ src,
+                                                   float4 *__restrict__ dst, long long n4) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(1))) v4 gv4;
+    const long long chunk = 4LL * blockDim.x;
+    for (long long base = blockIdx.x * chunk; base < n4; base += (long long)gridDim.x * chunk) {
+        v4 r[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const long long i = base + u * blockDim.x + threadIdx.x;
+            if (i < n4) r[u] = __builtin_nontemporal_load(&((const gv4 *)src)[i]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const long long i = base + u * blockDim.x + threadIdx.x;
+            if (i < n4) __builtin_nontemporal_store(r[u], &((gv4 *)dst)[i]);
+        }
+    }
+}
+
+static int32_t time_copy(int64_t bytes, int32_t iters, double *ms_per_iter, bool fast);
+
 }  // namespace mfd
 
 extern "C" int32_t mfd_stream_copy(int64_t bytes, int32_t iters, double *ms_per_iter) {
+    return mfd::time_copy(bytes, iters, ms_per_iter, false);
+}
+
+extern "C" int32_t mfd_stream_copy_fast(int64_t bytes, int32_t iters, double *ms_per_iter) {
+    return mfd::time_copy(bytes, iters, ms_per_iter, true);
+}
+
+static int32_t mfd::time_copy(int64_t bytes, int32_t iters, double *ms_per_iter, bool fast) {
     using namespace mfd;
     if (bytes < 1024 || iters < 1 || !ms_per_iter) { set_error("bad arguments"); return MFD_ERR_ARG; }
     const long long n4 = bytes / 16;
@@ -30,13 +61,17 @@ extern "C" int32_t mfd_stream_copy(int64_t bytes, int32_t iters, double *ms_per_
     int dev = 0, cus = 256;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const int grid = cus * 8;
+    const int grid = cus * (fast ? 16 : 8);
+    auto launch = [&] {
+        if (fast) hipLaunchKernelGGL(k_copy_fast, dim3(grid), dim3(256), 0, 0, a, b, n4);
+        else hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, a, b, n4);
+    };
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
-    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, a, b, n4);
+    launch();
     hipEventRecord(e0, 0);
-    for (int i = 0; i < iters; i++) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, a, b, n4);
+    for (int i = 0; i < iters; i++) launch();
     hipEventRecord(e1, 0);
     hipEventSynchronize(e1);
     float ms = 0;

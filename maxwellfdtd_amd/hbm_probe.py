@@ -16,6 +16,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--iters", type=int, default=5)
+    ap.add_argument("--plain-only", action="store_true",
+                    help="only the plain copy (the byte-counter calibration run)")
     a = ap.parse_args()
     lib = _lib.engine()
     nbytes = a.mib << 20
@@ -23,6 +25,10 @@ def main():
     lib.check(lib.stream_copy(nbytes, a.iters, C.byref(ms)), "stream_copy")
     print(f"copy {nbytes} bytes: read {nbytes} + write {nbytes} per launch, "
           f"{ms.value * 1e3:.1f} us, {2 * nbytes / ms.value / 1e6:.0f} GB/s", flush=True)
+    if not a.plain_only:
+        lib.check(lib.stream_copy_fast(nbytes, a.iters, C.byref(ms)), "stream_copy_fast")
+        print(f"fast copy (4 x float4 per lane, nontemporal) {nbytes} bytes: "
+              f"{ms.value * 1e3:.1f} us, {2 * nbytes / ms.value / 1e6:.0f} GB/s", flush=True)
 
 
 if __name__ == "__main__":
