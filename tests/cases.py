@@ -245,6 +245,32 @@ def lorentz_2d(nx=96, nz=80, steps=200, seed=3, tfsf=True):
     return model, mons
 
 
+def nonlinear_2d(nx=96, nz=80, steps=180, I0=1e17):
+    """2D (x, z): a Kerr + plasma half-space, a Kerr-only rod and a plain slab under an
+    intense TFSF pulse, plus a soft point source."""
+    d = 40e-9
+    grid = Grid(xmin=0, xmax=(nx - 1) * d, Nx=nx, zmin=0, zmax=(nz - 1) * d, Nz=nz)
+    Lx, Lz = (nx - 1) * d, (nz - 1) * d
+    n0, n2 = 1.45, 2.5e-20
+    chi3 = 4 / 3 * n0 ** 2 * EPS0 * C0 * n2
+    w0 = 2 * np.pi * C0 / 800e-9
+    plasma = Plasma(ionrate=PowerLawRate(sigma=1e-90, K=6), rho0=2.1e28, nuc=1e15, frequency=w0,
+                    Uiev=9.0, mr=0.64)
+    mats = [Material(geometry=lambda x, z: (z > 0.45 * Lz) + 0 * x > 0, eps=n0 ** 2, chi3=chi3,
+                     plasma=plasma),
+            Material(geometry=lambda x, z: (x - 0.5 * Lx) ** 2 + (z - 0.3 * Lz) ** 2 <= (5 * d) ** 2,
+                     eps=2.0, chi3=chi3),
+            Material(geometry=lambda x, z: (z > 0.2 * Lz) & (z < 0.25 * Lz) & (x < 0.4 * Lx), eps=3.0)]
+    E0 = np.sqrt(I0 / (n0 * EPS0 * C0 / 2))
+    w = lambda t: E0 * np.exp(-(t - 10e-15) ** 2 / (4e-15) ** 2) * np.cos(w0 * (t - 10e-15))
+    srcs = [TFSFSource(box=(8 * d, Lx - 8 * d, 8 * d, Lz - 8 * d), waveform=w, component="Ex",
+                       z0=6 * d),
+            SoftSource(geometry=(0.7 * Lx, 0.35 * Lz), waveform=lambda x, z, t: 1e-3 * w(t) + 0 * x,
+                       component="Ez")]
+    dt = 0.99 / C0 / np.sqrt(2 / d ** 2)
+    return Model(grid, srcs, tmax=(steps - 1) * dt, CN=0.99, pml=5 * d, material=mats)
+
+
 def slab_1d(nz=600, steps=900, dispersive=True):
     """1D: CPML, a Lorentz + conductive slab and a soft source, plus monitors."""
     d = 40e-9

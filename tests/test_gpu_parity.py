@@ -121,6 +121,15 @@ def test_nonlinear_3d(rate):
     assert_parity(hg, ho, desc, model)
 
 
+def test_nonlinear_2d():
+    """2D Kerr + plasma and Kerr-only cells keep their D (the E kernel loads it once the
+    id is known); the plain cells around them run without it."""
+    model = cases.nonlinear_2d()
+    hg, ho, desc = run_pair(model)
+    assert_parity(hg, ho, desc, model)
+    assert np.max(ho.output(3)) > 0          # the pulse ionised the medium
+
+
 def test_lorentz_2d():
     model, mons = cases.lorentz_2d()
     hg, ho, desc = run_pair(model, mons)
