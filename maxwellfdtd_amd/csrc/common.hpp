@@ -22,6 +22,11 @@ __device__ __forceinline__ gp<T> G(T *p) { return (gp<T>)p; }
 template <class T>
 __device__ __forceinline__ gp<const T> G(const T *p) { return (gp<const T>)p; }
 
+// |E x H| for the Sa integral (src/outputs.jl:182-191,318-327,475-485): the hardware
+// square root (v_sqrt_f32, within 1 ulp) rather than the correctly rounded library
+// sequence, which costs about 15 more instructions per cell and step
+__device__ __forceinline__ float sa_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // Two-element vectors usable through address-space-1 pointers (HIP's float2/double2
 // classes only bind to generic references).
 typedef float v2f __attribute__((ext_vector_type(2)));

@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void k_e1d(const DevModel *__restrict__ models
             const float dt = m.dtf;
             if (m.Sa) {
                 const float a = e[0] * hy;
-                m.Sa[iz] += sqrtf(a * a) * dt;
+                m.Sa[iz] += sa_sqrt(a * a) * dt;
             }
             if (m.E2) m.E2[iz] += e[0] * e[0] * dt;
         }

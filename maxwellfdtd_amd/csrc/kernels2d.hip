@@ -221,7 +221,7 @@ __global__ __launch_bounds__(64 * TW) void k_e2d(const DevModel *__restrict__ mo
                 if (INTEG && !defer && !(P.g & ID_TOUCHED)) {   // src/outputs.jl:322-325
                     if (doSa) {
                         const float a = -e1 * P.hy, b = e0 * P.hy;
-                        Sa[i] = P.sa + sqrtf(a * a + b * b) * dtf;
+                        Sa[i] = P.sa + sa_sqrt(a * a + b * b) * dtf;
                     }
                     if (doE2) E2[i] = P.e2 + (e0 * e0 + e1 * e1) * dtf;
                 }

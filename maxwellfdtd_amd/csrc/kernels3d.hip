@@ -315,7 +315,7 @@ __global__ __launch_bounds__(256) void k_e3d(const DevModel *__restrict__ models
                 if (m.Sa) {
                     const float a = e[1] * hz - e[2] * hy, b = e[2] * hx - e[0] * hz;
                     const float c = e[0] * hy - e[1] * hx;
-                    m.Sa[i] += sqrtf(a * a + b * b + c * c) * dt;
+                    m.Sa[i] += sa_sqrt(a * a + b * b + c * c) * dt;
                 }
                 if (m.E2) m.E2[i] += (e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) * dt;
             }

@@ -417,7 +417,7 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
                     if (doSa) {
                         const float a = e[1] * hzI - e[2] * hyI, b = e[2] * P.hx - e[0] * hzI;
                         const float cc = e[0] * hyI - e[1] * P.hx;
-                        __builtin_nontemporal_store(P.sa + sqrtf(a * a + b * b + cc * cc) * dtf, &Sa[i]);
+                        __builtin_nontemporal_store(P.sa + sa_sqrt(a * a + b * b + cc * cc) * dtf, &Sa[i]);
                     }
                     if (doE2) __builtin_nontemporal_store(P.e2 + (e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) * dtf, &E2[i]);
                 }

@@ -465,7 +465,7 @@ __device__ __forceinline__ void e_compute(const DevModel &m, const ECtx &c, int 
                 const float a = e1[k] * P.hz[k] - e2[k] * P.hy[k];
                 const float b = e2[k] * P.hx[k] - e0[k] * P.hz[k];
                 const float cc = e0[k] * P.hy[k] - e1[k] * P.hx[k];
-                P.sa[k] += sqrtf(a * a + b * b + cc * cc) * dtf;
+                P.sa[k] += sa_sqrt(a * a + b * b + cc * cc) * dtf;
             }
             st4(c.Sa, i, P.sa);
         }

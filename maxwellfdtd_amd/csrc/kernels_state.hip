@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void k_e_state(const DevModel *__restrict__ mo
                 const float ex = e[0], ey = e[1 % NC], ez = e[2 % NC];
                 if (m.Sa) {
                     const float a = ey * hz - ez * hy, b = ez * hx - ex * hz, c = ex * hy - ey * hx;
-                    G(m.Sa)[i] += sqrtf(a * a + b * b + c * c) * dt;
+                    G(m.Sa)[i] += sa_sqrt(a * a + b * b + c * c) * dt;
                 }
                 if (m.E2) G(m.E2)[i] += (ex * ex + ey * ey + ez * ez) * dt;
             } else if (NC == 2) {
@@ -57,7 +57,7 @@ __global__ __launch_bounds__(256) void k_e_state(const DevModel *__restrict__ mo
                 const float ex = e[0], ez = e[1 % NC];
                 if (m.Sa) {
                     const float a = -ez * hy, b = ex * hy;
-                    G(m.Sa)[i] += sqrtf(a * a + b * b) * dt;
+                    G(m.Sa)[i] += sa_sqrt(a * a + b * b) * dt;
                 }
                 if (m.E2) G(m.E2)[i] += (ex * ex + ez * ez) * dt;
             } else {
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void k_e_state(const DevModel *__restrict__ mo
                 const float ex = e[0];
                 if (m.Sa) {
                     const float a = ex * hy;
-                    G(m.Sa)[i] += sqrtf(a * a) * dt;
+                    G(m.Sa)[i] += sa_sqrt(a * a) * dt;
                 }
                 if (m.E2) G(m.E2)[i] += ex * ex * dt;
             }

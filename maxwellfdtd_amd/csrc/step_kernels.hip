@@ -350,7 +350,7 @@ __global__ __launch_bounds__(256) void k_tfsf_gather(const DevModel *__restrict_
                 if (m.Sa) {
                     const float a = f[EY] * f[HZ] - f[EZ] * f[HY], b = f[EZ] * f[HX] - f[EX] * f[HZ];
                     const float cc = f[EX] * f[HY] - f[EY] * f[HX];
-                    G(m.Sa)[i] += sqrtf(a * a + b * b + cc * cc) * dt;
+                    G(m.Sa)[i] += sa_sqrt(a * a + b * b + cc * cc) * dt;
                 }
                 if (m.E2) G(m.E2)[i] += (f[EX] * f[EX] + f[EY] * f[EY] + f[EZ] * f[EZ]) * dt;
             }
@@ -535,21 +535,21 @@ __global__ __launch_bounds__(256) void k_integrals_touched(const DevModel *__res
             const float ex = m.F[EX][i], ey = m.F[EY][i], ez = m.F[EZ][i];
             if (Sa) {
                 const float a = ey * hz - ez * hy, b = ez * hx - ex * hz, c = ex * hy - ey * hx;
-                Sa[i] += sqrtf(a * a + b * b + c * c) * dt;
+                Sa[i] += sa_sqrt(a * a + b * b + c * c) * dt;
             }
             if (E2) E2[i] += (ex * ex + ey * ey + ez * ez) * dt;
         } else if (ndims == 2) {
             const float hy = m.F[HY][i], ex = m.F[EX][i], ez = m.F[EZ][i];
             if (Sa) {
                 const float a = -ez * hy, b = ex * hy;
-                Sa[i] += sqrtf(a * a + b * b) * dt;
+                Sa[i] += sa_sqrt(a * a + b * b) * dt;
             }
             if (E2) E2[i] += (ex * ex + ez * ez) * dt;
         } else {
             const float hy = m.F[HY][i], ex = m.F[EX][i];
             if (Sa) {
                 const float a = ex * hy;
-                Sa[i] += sqrtf(a * a) * dt;
+                Sa[i] += sa_sqrt(a * a) * dt;
             }
             if (E2) E2[i] += ex * ex * dt;
         }
