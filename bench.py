@@ -426,7 +426,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU z-slab sharding is "
-                                   "a design only (see DESIGN.md), so --gpus must be 1"}))
+                                   "a design only, so --gpus must be 1"}))
         sys.exit(2)
     args.steps_given = args.steps is not None
     if args.steps is None:

@@ -3,6 +3,7 @@ backend is called (grids, time grid, boundary codes, CPML layers, material and s
 coefficients, output times).  Expected values come from the reference's formulas and
 messages (files cited per test) or from closed forms."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -209,3 +210,15 @@ def test_bytes_per_cell_update():
     bh, be = bytes_per_cell_update(m)
     assert math.isclose(bh, 37 + 16 * s) and math.isclose(be, 49 + 16 * s)
     assert math.isclose(s, 6 * 11 / 64)      # 11-point layers on six faces of 64^2
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU: --gpus above 1 exits with a message (no GPU needed)."""
+    import json
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
