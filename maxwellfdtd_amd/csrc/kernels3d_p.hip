@@ -297,6 +297,7 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
             };
             float hxm = 0.0f, hym = 0.0f;     // H(z-1)
             bool zm_own = false;
+            int nonfin = 0;                   // 1 NaN, 2 Inf seen in this lane's Ex
             if (z0 > 0) { hxm = Hx[col + sz * (z0 - 1)]; hym = Hy[col + sz * (z0 - 1)]; }
             else if (bcz == BC_PERIODIC) { hxm = Hx[col + sz * (Nz - 1)]; hym = Hy[col + sz * (Nz - 1)]; }
             else if (bcz == BC_NEUMANN) zm_own = true;
@@ -412,7 +413,8 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
                 }
                 if (dc) { Dx[i] = d0; Dy[i] = d1; Dz[i] = d2; }
                 if (!defer) { Ex[i] = e[0]; Ey[i] = e[1]; Ez[i] = e[2]; }
-                check_finite(e[0], m.flag, !defer);
+                // non-finite guard on Ex: collected per lane, reported once per tile
+                if (!defer && !isfinite(e[0])) nonfin |= isnan(e[0]) ? 1 : 2;
                 if (INTEG && !defer && (!(P.g & ID_TOUCHED) || xh)) {   // src/outputs.jl:480-483
                     if (doSa) {
                         const float a = e[1] * hzI - e[2] * hyI, b = e[2] * P.hx - e[0] * hzI;
@@ -432,6 +434,7 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
                 if (iz + 2 < z1) load(iz + 2, A);
                 compute(iz + 1, B);
             }
+            if (__any(nonfin != 0) && nonfin) atomicOr(m.flag, nonfin);
         }
         __syncthreads();
     }
