@@ -718,8 +718,13 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     // on the tiles the H kernel finished last (their H is still in the Infinity Cache),
     // and the next H kernel, walking forwards, starts where E finished.
     const int nxcd = 8;
-    auto order = [&](const std::vector<Tile> &in, bool backwards, std::vector<Tile> &out) {
+    auto order = [&](const std::vector<Tile> &in, bool backwards, std::vector<Tile> &out,
+                     bool chunked = true) {
         if (in.empty()) return;
+        if (!chunked) {
+            for (size_t b = 0; b < in.size(); b++) out.push_back(in[backwards ? in.size() - 1 - b : b]);
+            return;
+        }
         const int per = ((int)in.size() + nxcd - 1) / nxcd;
         for (int b = 0; b < per * nxcd; b++) {
             const int j = b / nxcd;
@@ -727,10 +732,14 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
             out.push_back(src < (int)in.size() ? in[src] : make_int4(-1, 0, 0, 0));
         }
     };
+    // 3D H tiles run in grid order unless MFD_OPT_XCD_H asks for chunks: at 256^3 that
+    // takes H from 155 to 148 us (the 2D kernels keep the chunks: a lone 2D solve runs
+    // 20 % slower in grid order)
     std::vector<Tile> perm, rev;
-    order(th_pml, false, perm);
+    const bool hchunk = opt.xcd_h || ndims != 3;
+    order(th_pml, false, perm, hchunk);
     ntiles_pml_h = (int)perm.size();
-    order(th_int, false, perm);
+    order(th_int, false, perm, hchunk);
     order(tl_int, opt.snake, rev);
     ntiles_int_e = (int)rev.size();
     order(tl_pml, opt.snake, rev);
@@ -753,6 +762,9 @@ int Runner::launch_step(bool timing) {
     const int nph = ntiles_pml_h, nih = ntiles_h - ntiles_pml_h;
     const int nie = ntiles_int_e, npe = ntiles - ntiles_int_e;
     auto grid_of = [&](int n) { return opt.persistent > 0 ? std::min(n, cu_count * opt.persistent) : n; };
+    // the H list (d_tiles, ntiles_h entries) and the E list (d_tiles_e, ntiles) differ in
+    // length when only one of them is padded to whole XCD chunks
+    const int grid_h = grid_of(ntiles_h);
     if (timing && ev.empty()) {
         ev.resize(4);
         for (auto &e : ev) MFD_HIP(hipEventCreate(&e));
@@ -760,14 +772,14 @@ int Runner::launch_step(bool timing) {
     if (timing) MFD_HIP(hipEventRecord(ev[0], stream));
     const int mode = any_full ? MODE_FULL : any_mats ? MODE_SIMPLE : MODE_VAC;
     const int hmode = h_mats ? MODE_SIMPLE : MODE_VAC;
-    if (ndims == 3 && kform == 1) launch_h3d_v4(d_models, d_tiles, ntiles, grid, hmode, opt.ty, stream);
+    if (ndims == 3 && kform == 1) launch_h3d_v4(d_models, d_tiles, ntiles_h, grid_h, hmode, opt.ty, stream);
     else if (ndims == 3 && kform == 2) {
         launch_h3d_p(d_models, d_tiles, nph, grid_of(nph), hmode, opt.ty, true, stream);
         launch_h3d_p(d_models, d_tiles + nph, nih, grid_of(nih), hmode, opt.ty, false, stream);
     }
-    else if (ndims == 3) launch_h3d(d_models, d_tiles, ntiles, grid, hmode, stream);
-    else if (ndims == 2) launch_h2d(d_models, d_tiles, ntiles, grid, hmode, opt.ty, stream);
-    else launch_h1d(d_models, d_tiles, ntiles, grid, hmode, stream);
+    else if (ndims == 3) launch_h3d(d_models, d_tiles, ntiles_h, grid_h, hmode, stream);
+    else if (ndims == 2) launch_h2d(d_models, d_tiles, ntiles_h, grid_h, hmode, opt.ty, stream);
+    else launch_h1d(d_models, d_tiles, ntiles_h, grid_h, hmode, stream);
     if (timing) { MFD_HIP(hipEventRecord(ev[1], stream)); MFD_HIP(hipEventRecord(ev[2], stream)); }
     if (ndims == 3 && kform == 1) launch_e3d_v4(d_models, d_tiles_e, ntiles, grid, mode, any_int, opt.ty, stream);
     else if (ndims == 3 && kform == 2) {
@@ -934,6 +946,7 @@ int set_option(RunOptions &o, Model *m, int opt, int value) {
         case MFD_OPT_SNAKE: o.snake = value != 0; break;
         case MFD_OPT_DLESS: o.dless = value != 0; break;
         case MFD_OPT_PML_SPLIT: o.pml_split = value != 0; break;
+        case MFD_OPT_XCD_H: o.xcd_h = value != 0; break;
         case MFD_OPT_STAGGER:
             if (!m || m->uploaded || value < 0) { set_error("stagger must be >= 0, set on a model before upload"); return MFD_ERR_ARG; }
             m->stagger = value;

@@ -136,6 +136,7 @@ struct RunOptions {
     // one launch, waves of CPML tiles, heavy in ALU work, share CUs with the memory-bound
     // interior waves and the step is faster)
     bool pml_split = false;
+    bool xcd_h = false;         // 3D H tile list in per-XCD contiguous chunks (else grid order)
 };
 
 // Advances a fixed set of uploaded models together.
