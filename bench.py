@@ -112,7 +112,7 @@ def bench_3d(args, lib):
     h = Handle(desc, lib)
     set_opts(lib, h, {0: args.graph_steps, 1: args.zchunk, 5: args.ty, 6: args.kform,
                       10: args.snake, 11: args.dless, 12: args.pml_split, 14: args.zchunk_h,
-                      15: args.xcd_h})
+                      15: args.xcd_h, 16: args.xcd_block})
     h.step(args.warmup)
     log(f"warmup {args.warmup} steps done")
     t0 = time.perf_counter()
@@ -147,7 +147,7 @@ def bench_3d(args, lib):
                      "cells": N, "model_steps": desc.Nt, "graph_steps": args.graph_steps,
                      "zchunk": args.zchunk, "rows_per_workgroup": args.ty, "kernel_form": args.kform,
                      "snake_order": args.snake, "plain_cells_without_d": args.dless,
-                     "cpml_split": args.pml_split, "zchunk_h": args.zchunk_h, "xcd_chunked_h": args.xcd_h}
+                     "cpml_split": args.pml_split, "zchunk_h": args.zchunk_h, "xcd_chunked_h": args.xcd_h, "xcd_block": args.xcd_block}
     out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                        "traffic": traffic,
@@ -180,7 +180,8 @@ def bench_dispersive_3d(args, lib):
     log(f"model built in {time.perf_counter()-t0:.1f}s: {desc.dims}, f_metal={f_metal:.4f}, "
         f"f_sub={f_sub:.4f}, B_cell={b_cell:.2f}")
     h = Handle(desc, lib)
-    set_opts(lib, h, {0: args.graph_steps, 1: args.zchunk, 5: args.ty, 6: args.kform})
+    set_opts(lib, h, {0: args.graph_steps, 1: args.zchunk, 5: args.ty, 6: args.kform, 14: args.zchunk_h,
+                      15: args.xcd_h, 16: args.xcd_block})
     t0 = time.perf_counter()
     h.step(steps)
     wall = time.perf_counter() - t0
@@ -410,6 +411,8 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=16)
     ap.add_argument("--zchunk", type=int, default=16)
     ap.add_argument("--zchunk-h", type=int, default=8, help="H half z planes per tile (0 = --zchunk)")
+    ap.add_argument("--xcd-block", type=int, default=64,
+                    help="tiles per block dealt to the XCDs (0: one chunk per XCD; default 64)")
     ap.add_argument("--xcd-h", type=int, default=0,
                     help="H tiles in per-XCD contiguous chunks (1) or in grid order (0)")
     ap.add_argument("--ty", type=int, default=4)

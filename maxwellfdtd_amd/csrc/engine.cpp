@@ -725,10 +725,18 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
             for (size_t b = 0; b < in.size(); b++) out.push_back(in[backwards ? in.size() - 1 - b : b]);
             return;
         }
-        const int per = ((int)in.size() + nxcd - 1) / nxcd;
+        // K = 0: one contiguous chunk per XCD; K > 0 (3D): blocks of K contiguous tiles
+        // dealt to the XCDs in turn (XCD x runs blocks x, x + 8, ...).  At 256^3, 64-tile
+        // blocks take E from 258 to 251 us and the dispersive step from 28.9 to 30.5
+        // Gcell/s; a lone 2D solve runs 20 % slower with them, so 2D keeps the chunks.
+        const int K = ndims == 3 ? opt.xcd_block : 0;
+        const int per = K > 0 ? ((int)in.size() + nxcd * K - 1) / (nxcd * K) * K
+                              : ((int)in.size() + nxcd - 1) / nxcd;
         for (int b = 0; b < per * nxcd; b++) {
-            const int j = b / nxcd;
-            const int src = (b % nxcd) * per + (backwards ? per - 1 - j : j);
+            int j = b / nxcd;
+            if (backwards) j = per - 1 - j;
+            const int x = b % nxcd;
+            const int src = K > 0 ? (j / K) * nxcd * K + x * K + j % K : x * per + j;
             out.push_back(src < (int)in.size() ? in[src] : make_int4(-1, 0, 0, 0));
         }
     };
@@ -947,6 +955,10 @@ int set_option(RunOptions &o, Model *m, int opt, int value) {
         case MFD_OPT_DLESS: o.dless = value != 0; break;
         case MFD_OPT_PML_SPLIT: o.pml_split = value != 0; break;
         case MFD_OPT_XCD_H: o.xcd_h = value != 0; break;
+        case MFD_OPT_XCD_BLOCK:
+            if (value < 0) { set_error("XCD block must be >= 0"); return MFD_ERR_ARG; }
+            o.xcd_block = value;
+            break;
         case MFD_OPT_STAGGER:
             if (!m || m->uploaded || value < 0) { set_error("stagger must be >= 0, set on a model before upload"); return MFD_ERR_ARG; }
             m->stagger = value;

@@ -137,6 +137,7 @@ struct RunOptions {
     // interior waves and the step is faster)
     bool pml_split = false;
     bool xcd_h = false;         // 3D H tile list in per-XCD contiguous chunks (else grid order)
+    int xcd_block = 64;         // 3D: tiles per block dealt to the XCDs (0: one chunk per XCD)
 };
 
 // Advances a fixed set of uploaded models together.

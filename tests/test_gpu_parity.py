@@ -256,16 +256,18 @@ def test_graph_and_eager_agree_bitwise():
 
 @pytest.mark.parametrize("kform", [2, 0])
 def test_h_tile_list_is_bitwise_neutral(kform):
-    """The H half marching its own z chunk, and its tiles in XCD chunks or in grid order
-    (then the H list and the E list differ in length), change no bit."""
+    """The H half marching its own z chunk, its tiles in XCD chunks or in grid order (then
+    the H list and the E list differ in length), and the tiles dealt to the XCDs in
+    blocks of any size, change no bit."""
     model, mons = cases.spheres_3d(n=40, nx=144, steps=60)
     desc = model.describe(monitors=mons)
     outs = []
-    for zh, xcd in ((0, 1), (8, 0), (4, 1), (32, 0)):
+    for zh, xcd, blk in ((0, 1, 0), (8, 0, 64), (4, 1, 7), (32, 0, 1)):
         h = Handle(desc, _lib.engine())
         h.lib.check(h.lib.model_set_options(h.h, _lib.MFD_OPT_KERNEL_FORM, kform), "set_options")
         h.lib.check(h.lib.model_set_options(h.h, _lib.MFD_OPT_ZCHUNK_H, zh), "set_options")
         h.lib.check(h.lib.model_set_options(h.h, _lib.MFD_OPT_XCD_H, xcd), "set_options")
+        h.lib.check(h.lib.model_set_options(h.h, _lib.MFD_OPT_XCD_BLOCK, blk), "set_options")
         h.step(model.Nt)
         outs.append([h.field(c) for c in ("Hx", "Hy", "Hz", "Ex", "Ey", "Ez")] + [h.output(0)])
         h.close()
