@@ -413,7 +413,7 @@ def main():
     ap.add_argument("--zchunk-h", type=int, default=8, help="H half z planes per tile (0 = --zchunk)")
     ap.add_argument("--xcd-block", type=int, default=64,
                     help="tiles per block dealt to the XCDs (0: one chunk per XCD; default 64)")
-    ap.add_argument("--xcd-h", type=int, default=0,
+    ap.add_argument("--xcd-h", type=int, default=1,
                     help="H tiles in per-XCD contiguous chunks (1) or in grid order (0)")
     ap.add_argument("--ty", type=int, default=4)
     ap.add_argument("--kform", type=int, default=2)

@@ -206,7 +206,7 @@ enum {
     MFD_OPT_PML_SPLIT = 12,    /* tiles no CPML layer reaches run a slab-free kernel (0)   */
     MFD_OPT_SCHED = 13,        /* server: admit 0 in arrival order, 1 least work first (1) */
     MFD_OPT_ZCHUNK_H = 14,     /* pipelined 3D H half z planes per tile (8; 0 = ZCHUNK)    */
-    MFD_OPT_XCD_H = 15,        /* 3D H tiles in per-XCD chunks (1) or grid order (0)       */
+    MFD_OPT_XCD_H = 15,        /* 3D H tiles dealt to the XCDs like E's (1) or grid order  */
     MFD_OPT_XCD_BLOCK = 16     /* 3D tiles per block dealt to the XCDs (64; 0: one per XCD) */
 };
 /* Changing an option rebuilds the launch plan; field state is kept.                   */

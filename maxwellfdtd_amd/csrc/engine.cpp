@@ -740,9 +740,8 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
             out.push_back(src < (int)in.size() ? in[src] : make_int4(-1, 0, 0, 0));
         }
     };
-    // 3D H tiles run in grid order unless MFD_OPT_XCD_H asks for chunks: at 256^3 that
-    // takes H from 155 to 148 us (the 2D kernels keep the chunks: a lone 2D solve runs
-    // 20 % slower in grid order)
+    // the H list is dealt to the XCDs like the E list (MFD_OPT_XCD_H = 0: 3D H tiles in
+    // plain grid order, which beats one chunk per XCD but not 64-tile blocks)
     std::vector<Tile> perm, rev;
     const bool hchunk = opt.xcd_h || ndims != 3;
     order(th_pml, false, perm, hchunk);

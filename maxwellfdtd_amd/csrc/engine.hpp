@@ -136,7 +136,7 @@ struct RunOptions {
     // one launch, waves of CPML tiles, heavy in ALU work, share CUs with the memory-bound
     // interior waves and the step is faster)
     bool pml_split = false;
-    bool xcd_h = false;         // 3D H tile list in per-XCD contiguous chunks (else grid order)
+    bool xcd_h = true;          // 3D H tile list dealt to the XCDs like E's (else grid order)
     int xcd_block = 64;         // 3D: tiles per block dealt to the XCDs (0: one chunk per XCD)
 };
 
