@@ -725,11 +725,12 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
             for (size_t b = 0; b < in.size(); b++) out.push_back(in[backwards ? in.size() - 1 - b : b]);
             return;
         }
-        // K = 0: one contiguous chunk per XCD; K > 0 (3D): blocks of K contiguous tiles
-        // dealt to the XCDs in turn (XCD x runs blocks x, x + 8, ...).  At 256^3, 64-tile
+        // K = 0: one contiguous chunk per XCD; K > 0: blocks of K contiguous tiles dealt
+        // to the XCDs in turn (XCD x runs blocks x, x + 8, ...).  At 256^3, 64-tile
         // blocks take E from 258 to 251 us and the dispersive step from 28.9 to 30.5
-        // Gcell/s; a lone 2D solve runs 20 % slower with them, so 2D keeps the chunks.
-        const int K = ndims == 3 ? opt.xcd_block : 0;
+        // Gcell/s.  A lone 2D solve runs 20 % slower with them and keeps the chunks; a
+        // batch of 2D solves takes the blocks.
+        const int K = (ndims == 3 || models.size() > 1) ? opt.xcd_block : 0;
         const int per = K > 0 ? ((int)in.size() + nxcd * K - 1) / (nxcd * K) * K
                               : ((int)in.size() + nxcd - 1) / nxcd;
         for (int b = 0; b < per * nxcd; b++) {
