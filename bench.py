@@ -335,7 +335,7 @@ def bench_served_3d(args, lib):
     # plain: one request at a time in arrival order (a FIFO queue in front of a single
     # solver); service times measured, waiting derived from the arrival times
     clock, busy, lat_p = 0.0, 0.0, []
-    for t_arr, d in reqs:
+    for t_arr, d in ([] if args.no_plain else reqs):
         h = Handle(d, lib)
         h.lib.check(h.lib.model_upload(h.h), "upload")
         t1 = time.perf_counter()
@@ -346,11 +346,6 @@ def bench_served_3d(args, lib):
         clock = start + svc
         busy += svc
         lat_p.append((clock - t_arr) * 1e3)
-    span = clock - reqs[0][0]
-    g_plain = work / span / 1e9
-    p50_p, p99_p = percentiles(lat_p)
-    log(f"plain: {busy:.2f} s busy over a {span:.2f} s span, {g_plain:.2f} Gcell/s, "
-        f"p50 {p50_p:.0f} ms p99 {p99_p:.0f} ms")
     out = base_record(args, st["gcells_per_s"], wall_s, 0)
     out["config"] = {"workload": f"served_3d: first {len(reqs)} requests of the Poisson trace "
                                  "(20 req/s), N^3 with N in 64..320, Kerr + 6-photon plasma "
@@ -362,13 +357,20 @@ def bench_served_3d(args, lib):
     out["served"] = {"gcells_per_s": round(st["gcells_per_s"], 3), "p50_ms": round(st["p50_ms"], 1),
                      "p99_ms": round(st["p99_ms"], 1), "mean_ms": round(st["mean_ms"], 1),
                      "done": st["done"], "failed": st["failed"], "busy_s": round(st["busy_s"], 3)}
+    out["roofline"] = None
+    if args.no_plain:
+        return out
+    span = clock - reqs[0][0]
+    g_plain = work / span / 1e9
+    p50_p, p99_p = percentiles(lat_p)
+    log(f"plain: {busy:.2f} s busy over a {span:.2f} s span, {g_plain:.2f} Gcell/s, "
+        f"p50 {p50_p:.0f} ms p99 {p99_p:.0f} ms")
     out["plain"] = {"gcells_per_s": round(g_plain, 3), "p50_ms": round(p50_p, 1),
                     "p99_ms": round(p99_p, 1), "busy_s": round(busy, 3), "span_s": round(span, 3),
                     "throughput": "cell-updates / (last completion - first arrival), "
                                   "as for the server"}
     out["speedup_vs_plain"] = round(st["gcells_per_s"] / g_plain, 3) if g_plain else None
     out["latency_gain_vs_plain"] = {"p50": round(p50_p / st["p50_ms"], 3), "p99": round(p99_p / st["p99_ms"], 3)}
-    out["roofline"] = None
     return out
 
 
@@ -425,6 +427,8 @@ def main():
     ap.add_argument("--step-scale", type=float, default=1.0)
     ap.add_argument("--chunk", type=int, default=200)
     ap.add_argument("--plain-count", type=int, default=0)
+    ap.add_argument("--no-plain", action="store_true",
+                    help="served_3d: skip the one-request-at-a-time comparison")
     ap.add_argument("--quantum", type=int, default=64)
     ap.add_argument("--fill", type=int, default=16, help="server: active cells bound, Mi-cells")
     ap.add_argument("--sched", type=int, default=1,
