@@ -134,7 +134,9 @@ int32_t mfd_device_info(int64_t *free_bytes, int64_t *total_bytes, int32_t *cu_c
 /* ---- model construction ------------------------------------------------------------ */
 typedef struct mfd_model mfd_model;
 
-/* Model(...) (src/models.jl:57-122).  Host-side only until the first step/upload.    */
+/* Model(...) (src/models.jl:57-122).  Host-side only until the first step/upload.
+ * Limits: Nz <= 65535 and Nx*Ny*Nz <= 2^31 - 1 cells (cell indices are 32-bit);
+ * MFD_ERR_ARG ("bad model descriptor") beyond them.                                  */
 int32_t mfd_model_create(const mfd_model_desc *desc, mfd_model **out);
 /* geometry Int array, 0 = vacuum, k = k-th material (src/models.jl:85-98).          */
 int32_t mfd_model_set_geometry(mfd_model *m, const int32_t *ids);

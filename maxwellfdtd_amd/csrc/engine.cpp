@@ -1032,6 +1032,11 @@ int32_t mfd_model_create(const mfd_model_desc *desc, mfd_model **out) {
         set_error("bad model descriptor");
         return MFD_ERR_ARG;
     }
+    // the kernels and the cell lists index cells with 32-bit integers
+    if ((long long)desc->Nx * desc->Ny * desc->Nz > 2147483647LL) {
+        set_error("bad model descriptor: more than 2^31 - 1 cells");
+        return MFD_ERR_ARG;
+    }
     for (int i = 0; i < 6; i++)
         if (desc->bcs[i] < 1 || desc->bcs[i] > 3) {
             set_error("Wrong 'bc' value. It can be either ':periodic', ':dirichlet', or ':neumann'.");

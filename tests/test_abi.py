@@ -109,7 +109,8 @@ def _desc(**kw):
 
 
 @pytest.mark.parametrize("bad", [dict(ndims=4), dict(Nz=1), dict(ndims=1),
-                                 dict(ndims=2), dict(Nt=0), dict(dt=0.0), dict(Nz=70000)])
+                                 dict(ndims=2), dict(Nt=0), dict(dt=0.0), dict(Nz=70000),
+                                 dict(Nx=2048, Ny=2048, Nz=512)])
 def test_engine_rejects_bad_descriptor(engine_so, bad):
     lib = _lib.engine()
     h = C.c_void_p()
