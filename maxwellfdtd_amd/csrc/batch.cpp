@@ -192,10 +192,10 @@ int32_t mfd_batch_active(mfd_batch *b, int32_t *nactive) {
 
 int32_t mfd_batch_destroy(mfd_batch *b) {
     if (!b) return MFD_OK;
-    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     b->runner.release();
     for (mfd_model *h : b->models) h->owner = nullptr;
-    if (b->stream) hipStreamDestroy(b->stream);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return MFD_OK;
 }
@@ -275,7 +275,7 @@ void mfd_server::finish(Request *r, int state, int status, const std::string &er
 }
 
 void mfd_server::loop() {
-    hipSetDevice(device);
+    const hipError_t dev_err = hipSetDevice(device);
     bool changed = false;
     for (;;) {
         std::vector<Request *> admitted;
@@ -285,7 +285,20 @@ void mfd_server::loop() {
                 cv_work.wait(lk, [&] { return stop || !queue.empty(); });
             if (stop && queue.empty() && active.empty()) return;
             size_t free_b = 0, total_b = 0;
-            hipMemGetInfo(&free_b, &total_b);
+            const hipError_t me = dev_err != hipSuccess ? dev_err : hipMemGetInfo(&free_b, &total_b);
+            if (me != hipSuccess) {
+                // the device cannot be used or queried: the queued requests fail with
+                // the HIP error instead of waiting for memory that is never reported
+                const std::string e = std::string("server device: ") + hipGetErrorString(me);
+                while (!queue.empty()) {
+                    Request *r = queue.front();
+                    queue.pop_front();
+                    in_use += r->fp;       // finish() subtracts it
+                    lk.unlock();
+                    finish(r, MFD_REQ_FAILED, MFD_ERR_HIP, e);
+                    lk.lock();
+                }
+            }
             int64_t reserve = (int64_t)1 << 30;
             while (!queue.empty() && (int)(active.size() + admitted.size()) < max_active) {
                 // the next request: the first to arrive, or the one with the least work
@@ -517,7 +530,7 @@ int32_t mfd_server_destroy(mfd_server *s) {
     }
     if (s->worker.joinable()) s->worker.join();
     s->runner.release();
-    if (s->stream) hipStreamDestroy(s->stream);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
     for (Request *r : s->all) {
         if (r->h && r->h->owner == s) r->h->owner = nullptr;
         delete r;

@@ -15,7 +15,7 @@ void set_error(const std::string &msg) { g_error = msg; }
 
 Arena::~Arena() { release(); }
 void Arena::release() {
-    for (void *p : ptrs) hipFree(p);
+    for (void *p : ptrs) (void)hipFree(p);
     ptrs.clear();
     bytes = 0;
 }
@@ -508,9 +508,9 @@ int Model::materialize_d() {
 int Model::sync() { return MFD_OK; }
 
 void Model::release() {
-    if (io) hipStreamSynchronize(io);
+    if (io) (void)hipStreamSynchronize(io);
     arena.release();
-    if (io) { hipStreamDestroy(io); io = nullptr; }
+    if (io) { (void)hipStreamDestroy(io); io = nullptr; }
     uploaded = false;
 }
 
@@ -553,14 +553,14 @@ int Model::write_field(int comp, const double *in) {
 Runner::~Runner() { release(); }
 
 void Runner::release() {
-    if (gexec) { hipGraphExecDestroy(gexec); gexec = nullptr; }
-    if (d_models) { hipFree(d_models); d_models = nullptr; }
-    if (d_tiles) { hipFree(d_tiles); d_tiles = nullptr; d_tiles_e = nullptr; }
-    if (d_chunks) { hipFree(d_chunks); d_chunks = nullptr; }
-    if (d_flags) { hipFree(d_flags); d_flags = nullptr; }
-    if (h_flags) { hipHostFree(h_flags); h_flags = nullptr; }
+    if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+    if (d_models) { (void)hipFree(d_models); d_models = nullptr; }
+    if (d_tiles) { (void)hipFree(d_tiles); d_tiles = nullptr; d_tiles_e = nullptr; }
+    if (d_chunks) { (void)hipFree(d_chunks); d_chunks = nullptr; }
+    if (d_flags) { (void)hipFree(d_flags); d_flags = nullptr; }
+    if (h_flags) { (void)hipHostFree(h_flags); h_flags = nullptr; }
     nchunks = 0;
-    for (auto e : ev) hipEventDestroy(e);
+    for (auto e : ev) (void)hipEventDestroy(e);
     ev.clear();
 }
 
@@ -823,7 +823,7 @@ int Runner::launch_step(bool timing) {
 }
 
 int Runner::build_graph(int steps) {
-    if (gexec) { hipGraphExecDestroy(gexec); gexec = nullptr; }
+    if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
     hipGraph_t g;
     MFD_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     int st = MFD_OK;
@@ -836,7 +836,7 @@ int Runner::build_graph(int steps) {
     if (st != MFD_OK) return st;
     if (e != hipSuccess) { set_error(std::string("graph capture: ") + hipGetErrorString(e)); return MFD_ERR_HIP; }
     e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     if (e != hipSuccess) { set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return MFD_ERR_HIP; }
     gsteps = steps;
     return MFD_OK;
@@ -988,8 +988,13 @@ static int check_device() {
     }
     hipDeviceProp_t p;
     int dev = 0;
-    hipGetDevice(&dev);
-    if (hipGetDeviceProperties(&p, dev) != hipSuccess || std::strncmp(p.gcnArchName, "gfx950", 6) != 0) {
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) {
+        set_error("cannot read the HIP device's properties");
+        ok = MFD_ERR_NODEV;
+        return ok;
+    }
+    p.gcnArchName[sizeof(p.gcnArchName) - 1] = '\0';
+    if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0) {
         set_error(std::string("device is not gfx950: ") + p.gcnArchName);
         ok = MFD_ERR_NODEV;
         return ok;
@@ -1380,10 +1385,10 @@ int32_t mfd_model_get_output(mfd_model *h, int32_t which, double *out) {
 int32_t mfd_model_destroy(mfd_model *h) {
     if (!h) return MFD_OK;
     if (h->owner) { set_error("model belongs to a batch or server"); return MFD_ERR_STATE; }
-    if (h->stream) hipStreamSynchronize(h->stream);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
     h->runner.release();
     h->m.release();
-    if (h->stream) hipStreamDestroy(h->stream);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return MFD_OK;
 }

@@ -11,8 +11,6 @@
 
 namespace mfd {
 
-constexpr double MU0_ = 1.25663706212e-6;
-
 // ---- soft / hard sources: add_source_kernel! (src/sources.jl:77-119,190-232) ---------
 __global__ __launch_bounds__(256) void k_point_sources(const DevModel *__restrict__ models,
                                                        int slot) {
@@ -496,7 +494,7 @@ __global__ __launch_bounds__(256) void k_dft_flush(const DevModel *__restrict__ 
                         for (int j = threadIdx.x; j < M.n; j += blockDim.x)
                             acc += (double)M.samp[((size_t)kk * M.ncomp + c) * M.n + j];
                         const double Fs = block_sum(acc, sh);
-                        if (threadIdx.x < nwc) {
+                        if ((int)threadIdx.x < nwc) {
                             const double2 p = ph[k * WCH + threadIdx.x];
                             double2 *Sp = M.S + (size_t)c * M.nw + w0 + threadIdx.x;
                             double2 sv = *Sp;

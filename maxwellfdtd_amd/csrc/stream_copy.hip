@@ -49,39 +49,72 @@ extern "C" int32_t mfd_stream_copy_fast(int64_t bytes, int32_t iters, double *ms
     return mfd::time_copy(bytes, iters, ms_per_iter, true);
 }
 
+namespace mfd {
+// device buffers and events of one probe, released on every return path
+struct ProbeRes {
+    std::vector<void *> mem;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~ProbeRes() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        for (void *p : mem) (void)hipFree(p);
+    }
+    template <class T>
+    int alloc(T **p, size_t bytes) {
+        if (hipMalloc(p, bytes) != hipSuccess) { set_error("out of memory"); return MFD_ERR_NOMEM; }
+        mem.push_back(*p);
+        return MFD_OK;
+    }
+};
+
+#define PROBE_TRY(x)                                     \
+    do {                                                 \
+        const int st_ = (x);                             \
+        if (st_ != MFD_OK) return st_;                   \
+    } while (0)
+
+// one warm-up launch, then `iters` timed launches between two events on the null stream
+template <class F>
+static int32_t time_launches(ProbeRes &r, int32_t iters, F launch, double *ms_per_iter) {
+    MFD_HIP(hipEventCreate(&r.e0));
+    MFD_HIP(hipEventCreate(&r.e1));
+    launch();
+    MFD_HIP(hipGetLastError());
+    MFD_HIP(hipEventRecord(r.e0, 0));
+    for (int i = 0; i < iters; i++) launch();
+    MFD_HIP(hipEventRecord(r.e1, 0));
+    MFD_HIP(hipEventSynchronize(r.e1));
+    float ms = 0;
+    MFD_HIP(hipEventElapsedTime(&ms, r.e0, r.e1));
+    *ms_per_iter = ms / iters;
+    return MFD_OK;
+}
+
+static int32_t cu_count_of_device(int *cus) {
+    int dev = 0;
+    MFD_HIP(hipGetDevice(&dev));
+    MFD_HIP(hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev));
+    return MFD_OK;
+}
+}  // namespace mfd
+
 static int32_t mfd::time_copy(int64_t bytes, int32_t iters, double *ms_per_iter, bool fast) {
     using namespace mfd;
     if (bytes < 1024 || iters < 1 || !ms_per_iter) { set_error("bad arguments"); return MFD_ERR_ARG; }
     const long long n4 = bytes / 16;
+    ProbeRes r;
     float4 *a = nullptr, *b = nullptr;
-    MFD_HIP(hipMalloc(&a, n4 * 16));
-    if (hipMalloc(&b, n4 * 16) != hipSuccess) { hipFree(a); set_error("out of memory"); return MFD_ERR_NOMEM; }
-    hipMemset(a, 0, n4 * 16);
-    hipMemset(b, 0, n4 * 16);
-    int dev = 0, cus = 256;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    PROBE_TRY(r.alloc(&a, n4 * 16));
+    PROBE_TRY(r.alloc(&b, n4 * 16));
+    MFD_HIP(hipMemset(a, 0, n4 * 16));
+    MFD_HIP(hipMemset(b, 0, n4 * 16));
+    int cus = 0;
+    PROBE_TRY(cu_count_of_device(&cus));
     const int grid = cus * (fast ? 16 : 8);
-    auto launch = [&] {
+    return time_launches(r, iters, [&] {
         if (fast) hipLaunchKernelGGL(k_copy_fast, dim3(grid), dim3(256), 0, 0, a, b, n4);
         else hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, a, b, n4);
-    };
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    launch();
-    hipEventRecord(e0, 0);
-    for (int i = 0; i < iters; i++) launch();
-    hipEventRecord(e1, 0);
-    hipEventSynchronize(e1);
-    float ms = 0;
-    hipEventElapsedTime(&ms, e0, e1);
-    *ms_per_iter = ms / iters;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    hipFree(a);
-    hipFree(b);
-    return MFD_OK;
+    }, ms_per_iter);
 }
 
 // Multi-stream variant: NIN arrays read, NOUT arrays written per element, as the
@@ -117,22 +150,23 @@ extern "C" int32_t mfd_stream_multi(int64_t bytes_per_array, int32_t nin, int32_
         set_error("need nin == nout in 1..6, blocks_per_cu >= 1, bytes >= 1024");
         return MFD_ERR_ARG;
     }
+    if (!ms_per_iter) { set_error("bad arguments"); return MFD_ERR_ARG; }
     const long long n4 = bytes_per_array / 16;
+    ProbeRes r;
     std::vector<float4 *> bufs(nin + nout);
     for (auto &b : bufs) {
-        if (hipMalloc(&b, n4 * 16) != hipSuccess) { set_error("oom"); return MFD_ERR_NOMEM; }
-        hipMemset(b, 0, n4 * 16);
+        PROBE_TRY(r.alloc(&b, n4 * 16));
+        MFD_HIP(hipMemset(b, 0, n4 * 16));
     }
     float4 **d_in, **d_out;
-    MFD_HIP(hipMalloc(&d_in, nin * sizeof(float4 *)));
-    MFD_HIP(hipMalloc(&d_out, nout * sizeof(float4 *)));
+    PROBE_TRY(r.alloc(&d_in, nin * sizeof(float4 *)));
+    PROBE_TRY(r.alloc(&d_out, nout * sizeof(float4 *)));
     MFD_HIP(hipMemcpy(d_in, bufs.data(), nin * sizeof(float4 *), hipMemcpyHostToDevice));
     MFD_HIP(hipMemcpy(d_out, bufs.data() + nin, nout * sizeof(float4 *), hipMemcpyHostToDevice));
-    int dev = 0, cus = 256;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    int cus = 0;
+    PROBE_TRY(cu_count_of_device(&cus));
     const int grid = cus * blocks_per_cu;
-    auto launch = [&]() {
+    return time_launches(r, iters, [&]() {
         switch (nin) {
             case 1: hipLaunchKernelGGL((k_multi<1, 1>), dim3(grid), dim3(256), 0, 0, d_in, d_out, n4); break;
             case 2: hipLaunchKernelGGL((k_multi<2, 2>), dim3(grid), dim3(256), 0, 0, d_in, d_out, n4); break;
@@ -141,22 +175,5 @@ extern "C" int32_t mfd_stream_multi(int64_t bytes_per_array, int32_t nin, int32_
             case 5: hipLaunchKernelGGL((k_multi<5, 5>), dim3(grid), dim3(256), 0, 0, d_in, d_out, n4); break;
             default: hipLaunchKernelGGL((k_multi<6, 6>), dim3(grid), dim3(256), 0, 0, d_in, d_out, n4); break;
         }
-    };
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    launch();
-    hipEventRecord(e0, 0);
-    for (int i = 0; i < iters; i++) launch();
-    hipEventRecord(e1, 0);
-    hipEventSynchronize(e1);
-    float ms = 0;
-    hipEventElapsedTime(&ms, e0, e1);
-    *ms_per_iter = ms / iters;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    for (auto b : bufs) hipFree(b);
-    hipFree(d_in);
-    hipFree(d_out);
-    return MFD_OK;
+    }, ms_per_iter);
 }
