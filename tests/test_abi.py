@@ -119,6 +119,19 @@ def test_engine_rejects_bad_descriptor(engine_so, bad):
     assert "descriptor" in lib.error()
 
 
+def test_hbm_probes_check_arguments(engine_so):
+    """The copy probes refuse bad sizes, counts and a missing output before any device
+    call, so these run without a GPU."""
+    lib = _lib.engine()
+    ms = C.c_double(0.0)
+    assert lib.stream_copy(512, 1, C.byref(ms)) == _lib.MFD_ERR_ARG
+    assert lib.stream_copy_fast(1 << 20, 0, C.byref(ms)) == _lib.MFD_ERR_ARG
+    assert lib.stream_copy(1 << 20, 1, None) == _lib.MFD_ERR_ARG
+    assert lib.stream_multi(1 << 20, 2, 3, 16, 1, C.byref(ms)) == _lib.MFD_ERR_ARG
+    assert lib.stream_multi(1 << 20, 7, 7, 16, 1, C.byref(ms)) == _lib.MFD_ERR_ARG
+    assert lib.stream_multi(1 << 20, 2, 2, 16, 1, None) == _lib.MFD_ERR_ARG
+
+
 def test_engine_rejects_bad_bc(engine_so):
     lib = _lib.engine()
     h = C.c_void_p()
