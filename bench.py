@@ -60,7 +60,8 @@ def cpu_leg(desc, seconds, N, label, gpu_lib):
     """The oracle (FP64) for a bounded number of steps, then the GPU engine for the same
     steps: CPU rate and GPU-vs-FP64 error."""
     sys.path.insert(0, os.path.join(HERE, "tests"))
-    from oracle_util import field_errors, oracle
+    from oracle_util import field_errors, oracle, rel_l2
+    from maxwellfdtd_amd import _lib
     from maxwellfdtd_amd.models import Handle
     orc = oracle("f64")
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
@@ -80,6 +81,10 @@ def cpu_leg(desc, seconds, N, label, gpu_lib):
     hg = Handle(desc, gpu_lib)
     hg.step(ns)
     errs = field_errors(hg, ho, desc.ndims)
+    if desc.integrals & 1:
+        errs["Sa"] = rel_l2(hg.output(_lib.MFD_OUT_SA), ho.output(_lib.MFD_OUT_SA))
+    if desc.integrals & 2:
+        errs["E2"] = rel_l2(hg.output(_lib.MFD_OUT_E2), ho.output(_lib.MFD_OUT_E2))
     parity = {"steps": ns, "rel_l2_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()}}
     log(f"parity after {ns} steps: {errs}")
     hg.close()
