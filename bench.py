@@ -68,10 +68,14 @@ def cpu_leg(desc, seconds, N, label, gpu_lib):
     threads = orc.set_threads(threads)
     ho = Handle(desc, orc)
     t0 = time.perf_counter()
+    t_note = t0 + 60.0
     ns = 0
     while time.perf_counter() - t0 < seconds and ns < desc.Nt:
         ho.step(1)
         ns += 1
+        if time.perf_counter() > t_note:    # long samples report as they go
+            log(f"cpu baseline: {ns} of {desc.Nt} steps after {time.perf_counter() - t0:.0f} s")
+            t_note += 60.0
     cw = time.perf_counter() - t0
     cpu = {"value": round(N * ns / cw / 1e9, 4), "unit": "Gcell-updates/s", "cores": threads,
            "kind": "port",
