@@ -50,6 +50,17 @@ __device__ __forceinline__ void slab2(const LayP &L, int p, unsigned j, float &d
     d1 = d1 * Ki;
 }
 
+// the same update with the old psi pair and the profile values already in registers
+__device__ __forceinline__ void slab2v(const LayP &L, unsigned j, float psi0, float psi1, float B,
+                                       float A, float Ki, float &d0, float &d1, float &a0, float &a1) {
+    a0 = B * psi0 + A * d0;
+    a1 = B * psi1 + A * d1;
+    G(L.p0)[j] = a0;
+    G(L.p1)[j] = a1;
+    d0 = d0 * Ki;
+    d1 = d1 * Ki;
+}
+
 // ---------------------------------- H half ---------------------------------------------
 struct HP {
     float ez, exn, eyn, exy, ezy, hx, hy, hz, eyE, ezE;
@@ -301,7 +312,41 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
             if (z0 > 0) { hxm = Hx[col + sz * (z0 - 1)]; hym = Hy[col + sz * (z0 - 1)]; }
             else if (bcz == BC_PERIODIC) { hxm = Hx[col + sz * (Nz - 1)]; hym = Hy[col + sz * (Nz - 1)]; }
             else if (bcz == BC_NEUMANN) zm_own = true;
-            auto compute = [&](int iz, EP &P) {
+            // x- and y-slab CPML: the profile values depend only on the lane (x) or the row
+            // (y) and are loaded once.  Each plane's x and y psi pairs are issued before the
+            // next plane's prefetch (slab_pre in the march below): vector loads complete in
+            // order, so a wait for a psi value issued after the prefetch would drain it.
+            float xKi = 0.0f, xA = 0.0f, xB = 0.0f, yKi = 0.0f, yA = 0.0f, yB = 0.0f;
+            if (xs0) { xKi = G(L[0].Ki)[ix]; xA = G(L[0].A)[ix]; xB = G(L[0].B)[ix]; }
+            else if (xs1) {
+                const int p = ix - L[1].ib;
+                xKi = G(L[1].Ki)[p]; xA = G(L[1].A)[p]; xB = G(L[1].B)[p];
+            }
+            if (ys0) { yKi = G(L[2].Ki)[iy]; yA = G(L[2].A)[iy]; yB = G(L[2].B)[iy]; }
+            else if (ys1) {
+                const int p = iy - L[3].ib;
+                yKi = G(L[3].Ki)[p]; yA = G(L[3].A)[p]; yB = G(L[3].B)[p];
+            }
+            struct SP { float x0, x1, y0, y1; };
+            auto slab_pre = [&](int iz, SP &S) {
+                if (xs0) {
+                    const unsigned j = ix + (unsigned)L[0].nb * (iy + (unsigned)Ny * iz);
+                    S.x0 = G((const float *)L[0].p0)[j]; S.x1 = G((const float *)L[0].p1)[j];
+                } else if (xs1) {
+                    const int p = ix - L[1].ib;
+                    const unsigned j = p + (unsigned)L[1].nb * (iy + (unsigned)Ny * iz);
+                    S.x0 = G((const float *)L[1].p0)[j]; S.x1 = G((const float *)L[1].p1)[j];
+                }
+                if (ys0) {
+                    const unsigned j = ix + (unsigned)Nx * (iy + (unsigned)L[2].nb * iz);
+                    S.y0 = G((const float *)L[2].p0)[j]; S.y1 = G((const float *)L[2].p1)[j];
+                } else if (ys1) {
+                    const int p = iy - L[3].ib;
+                    const unsigned j = ix + (unsigned)Nx * (p + (unsigned)L[3].nb * iz);
+                    S.y0 = G((const float *)L[3].p0)[j]; S.y1 = G((const float *)L[3].p1)[j];
+                }
+            };
+            auto compute = [&](int iz, EP &P, const SP &S) {
                 const unsigned i = col + sz * iz;
                 if (zm_own) { hxm = P.hx; hym = P.hy; zm_own = false; }
                 if (!yload) {
@@ -331,21 +376,27 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
                 float d0 = P.d0, d1 = P.d1, d2 = P.d2;
                 float a0, a1;
                 if (xs0) {   // src/models.jl:976-986
-                    slab2(L[0], ix, ix + (unsigned)L[0].nb * (iy + (unsigned)Ny * iz), dHyx, dHzx, a0, a1);
+                    slab2v(L[0], ix + (unsigned)L[0].nb * (iy + (unsigned)Ny * iz), S.x0, S.x1, xB, xA, xKi,
+                           dHyx, dHzx, a0, a1);
                     d1 -= Md2 * a1; d2 += Md2 * a0;
                 }
                 if (xs1) {   // src/models.jl:987-997
                     const int p = ix - L[1].ib;
-                    slab2(L[1], p, p + (unsigned)L[1].nb * (iy + (unsigned)Ny * iz), dHyx, dHzx, a0, a1);
+                    const unsigned j = p + (unsigned)L[1].nb * (iy + (unsigned)Ny * iz);
+                    if (!xs0) slab2v(L[1], j, S.x0, S.x1, xB, xA, xKi, dHyx, dHzx, a0, a1);
+                    else slab2(L[1], p, j, dHyx, dHzx, a0, a1);     // both x layers reach the cell
                     d1 -= Md2 * a1; d2 += Md2 * a0;
                 }
                 if (ys0) {   // src/models.jl:998-1006
-                    slab2(L[2], iy, ix + (unsigned)Nx * (iy + (unsigned)L[2].nb * iz), dHxy, dHzy, a0, a1);
+                    slab2v(L[2], ix + (unsigned)Nx * (iy + (unsigned)L[2].nb * iz), S.y0, S.y1, yB, yA, yKi,
+                           dHxy, dHzy, a0, a1);
                     d0 += Md2 * a1; d2 -= Md2 * a0;
                 }
                 if (ys1) {   // src/models.jl:1007-1015
                     const int p = iy - L[3].ib;
-                    slab2(L[3], p, ix + (unsigned)Nx * (p + (unsigned)L[3].nb * iz), dHxy, dHzy, a0, a1);
+                    const unsigned j = ix + (unsigned)Nx * (p + (unsigned)L[3].nb * iz);
+                    if (!ys0) slab2v(L[3], j, S.y0, S.y1, yB, yA, yKi, dHxy, dHzy, a0, a1);
+                    else slab2(L[3], p, j, dHxy, dHzy, a0, a1);     // both y layers reach the cell
                     d0 += Md2 * a1; d2 -= Md2 * a0;
                 }
                 if (L[4].present && iz <= L[4].ib) {   // src/models.jl:1016-1025
@@ -428,11 +479,15 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
             EP A, B;
             load(z0, A);
             for (int iz = z0; iz < z1; iz += 2) {
+                SP S0 = {0.0f, 0.0f, 0.0f, 0.0f};
+                slab_pre(iz, S0);
                 if (iz + 1 < z1) load(iz + 1, B);
-                compute(iz, A);
+                compute(iz, A, S0);
                 if (iz + 1 >= z1) break;
+                SP S1 = {0.0f, 0.0f, 0.0f, 0.0f};
+                slab_pre(iz + 1, S1);
                 if (iz + 2 < z1) load(iz + 2, A);
-                compute(iz + 1, B);
+                compute(iz + 1, B, S1);
             }
             if (__any(nonfin != 0) && nonfin) atomicOr(m.flag, nonfin);
         }
