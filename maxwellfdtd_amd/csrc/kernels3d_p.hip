@@ -327,8 +327,21 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
                 const int p = iy - L[3].ib;
                 yKi = G(L[3].Ki)[p]; yA = G(L[3].A)[p]; yB = G(L[3].B)[p];
             }
-            struct SP { float x0, x1, y0, y1; };
+            struct SP { float x0, x1, y0, y1, z0, z1, zKi, zA, zB; };
+            auto zs0 = [&](int iz) { return L[4].present && iz <= L[4].ib; };
+            auto zs1 = [&](int iz) { return L[5].present && iz >= L[5].ib; };
             auto slab_pre = [&](int iz, SP &S) {
+                // z slabs: the plane's psi pair and its profile values (uniform per plane)
+                if (zs0(iz)) {
+                    const unsigned j = col + sz * iz;
+                    S.z0 = G((const float *)L[4].p0)[j]; S.z1 = G((const float *)L[4].p1)[j];
+                    S.zKi = G(L[4].Ki)[iz]; S.zA = G(L[4].A)[iz]; S.zB = G(L[4].B)[iz];
+                } else if (zs1(iz)) {
+                    const int p = iz - L[5].ib;
+                    const unsigned j = col + sz * p;
+                    S.z0 = G((const float *)L[5].p0)[j]; S.z1 = G((const float *)L[5].p1)[j];
+                    S.zKi = G(L[5].Ki)[p]; S.zA = G(L[5].A)[p]; S.zB = G(L[5].B)[p];
+                }
                 if (xs0) {
                     const unsigned j = ix + (unsigned)L[0].nb * (iy + (unsigned)Ny * iz);
                     S.x0 = G((const float *)L[0].p0)[j]; S.x1 = G((const float *)L[0].p1)[j];
@@ -399,13 +412,14 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
                     else slab2(L[3], p, j, dHxy, dHzy, a0, a1);     // both y layers reach the cell
                     d0 += Md2 * a1; d2 -= Md2 * a0;
                 }
-                if (L[4].present && iz <= L[4].ib) {   // src/models.jl:1016-1025
-                    slab2(L[4], iz, col + sz * iz, dHxz, dHyz, a0, a1);
+                if (zs0(iz)) {   // src/models.jl:1016-1025
+                    slab2v(L[4], col + sz * iz, S.z0, S.z1, S.zB, S.zA, S.zKi, dHxz, dHyz, a0, a1);
                     d0 -= Md2 * a1; d1 += Md2 * a0;
                 }
-                if (L[5].present && iz >= L[5].ib) {   // src/models.jl:1026-1035
+                if (zs1(iz)) {   // src/models.jl:1026-1035
                     const int p = iz - L[5].ib;
-                    slab2(L[5], p, col + sz * p, dHxz, dHyz, a0, a1);
+                    if (!zs0(iz)) slab2v(L[5], col + sz * p, S.z0, S.z1, S.zB, S.zA, S.zKi, dHxz, dHyz, a0, a1);
+                    else slab2(L[5], p, col + sz * p, dHxz, dHyz, a0, a1);     // both z layers reach the cell
                     d0 -= Md2 * a1; d1 += Md2 * a0;
                 }
                 d0 = Md1 * d0 + Md2 * (dHzy - dHyz);
@@ -479,12 +493,12 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
             EP A, B;
             load(z0, A);
             for (int iz = z0; iz < z1; iz += 2) {
-                SP S0 = {0.0f, 0.0f, 0.0f, 0.0f};
+                SP S0 = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
                 slab_pre(iz, S0);
                 if (iz + 1 < z1) load(iz + 1, B);
                 compute(iz, A, S0);
                 if (iz + 1 >= z1) break;
-                SP S1 = {0.0f, 0.0f, 0.0f, 0.0f};
+                SP S1 = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
                 slab_pre(iz + 1, S1);
                 if (iz + 2 < z1) load(iz + 2, A);
                 compute(iz + 1, B, S1);
