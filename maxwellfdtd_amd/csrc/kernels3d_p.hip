@@ -116,7 +116,26 @@ __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(5, 8)))
                 if (MODE != MODE_VAC) P.g = id[i];
             };
             float ex = Ex[col + sz * z0], ey = Ey[col + sz * z0];
-            auto compute = [&](int iz, HP &P) {
+            // x-slab CPML as in the E half: profile values once per tile, each plane's psi
+            // pair issued before the next plane's prefetch
+            float xKi = 0.0f, xA = 0.0f, xB = 0.0f;
+            if (xs0) { xKi = G(L[0].Ki)[ix]; xA = G(L[0].A)[ix]; xB = G(L[0].B)[ix]; }
+            else if (xs1) {
+                const int p = ix - L[1].ib;
+                xKi = G(L[1].Ki)[p]; xA = G(L[1].A)[p]; xB = G(L[1].B)[p];
+            }
+            struct SP { float x0, x1; };
+            auto slab_pre = [&](int iz, SP &S) {
+                if (xs0) {
+                    const unsigned j = ix + (unsigned)L[0].nb * (iy + (unsigned)Ny * iz);
+                    S.x0 = G((const float *)L[0].p0)[j]; S.x1 = G((const float *)L[0].p1)[j];
+                } else if (xs1) {
+                    const int p = ix - L[1].ib;
+                    const unsigned j = p + (unsigned)L[1].nb * (iy + (unsigned)Ny * iz);
+                    S.x0 = G((const float *)L[1].p0)[j]; S.x1 = G((const float *)L[1].p1)[j];
+                }
+            };
+            auto compute = [&](int iz, HP &P, const SP &S) {
                 const unsigned i = col + sz * iz;
                 if (iz + 1 >= Nz && bcz != BC_PERIODIC) {
                     P.exn = bcz == BC_DIRICHLET ? 0.0f : ex;
@@ -142,12 +161,15 @@ __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(5, 8)))
                 float hx = P.hx, hy = P.hy, hz = P.hz;
                 float a0, a1;
                 if (xs0) {   // src/models.jl:805-815
-                    slab2(L[0], ix, ix + (unsigned)L[0].nb * (iy + (unsigned)Ny * iz), dEyx, dEzx, a0, a1);
+                    slab2v(L[0], ix + (unsigned)L[0].nb * (iy + (unsigned)Ny * iz), S.x0, S.x1, xB, xA, xKi,
+                           dEyx, dEzx, a0, a1);
                     hy += Mh * a1; hz -= Mh * a0;
                 }
                 if (xs1) {   // src/models.jl:816-826
                     const int p = ix - L[1].ib;
-                    slab2(L[1], p, p + (unsigned)L[1].nb * (iy + (unsigned)Ny * iz), dEyx, dEzx, a0, a1);
+                    const unsigned j = p + (unsigned)L[1].nb * (iy + (unsigned)Ny * iz);
+                    if (!xs0) slab2v(L[1], j, S.x0, S.x1, xB, xA, xKi, dEyx, dEzx, a0, a1);
+                    else slab2(L[1], p, j, dEyx, dEzx, a0, a1);     // both x layers reach the cell
                     hy += Mh * a1; hz -= Mh * a0;
                 }
                 if (ys0) {   // src/models.jl:827-837
@@ -177,11 +199,15 @@ __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(5, 8)))
             HP A, B;
             load(z0, A);
             for (int iz = z0; iz < z1; iz += 2) {
+                SP S0 = {0.0f, 0.0f};
+                slab_pre(iz, S0);
                 if (iz + 1 < z1) load(iz + 1, B);
-                compute(iz, A);
+                compute(iz, A, S0);
                 if (iz + 1 >= z1) break;
+                SP S1 = {0.0f, 0.0f};
+                slab_pre(iz + 1, S1);
                 if (iz + 2 < z1) load(iz + 2, A);
-                compute(iz + 1, B);
+                compute(iz + 1, B, S1);
             }
         }
         __syncthreads();
