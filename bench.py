@@ -56,16 +56,45 @@ def kernel_timing(lib, h, steps):
     return h_ms, e_ms, cnt.value
 
 
-def cpu_leg(desc, seconds, N, label, gpu_lib):
-    """The oracle (FP64) for a bounded number of steps, then the GPU engine for the same
-    steps: CPU rate and GPU-vs-FP64 error."""
-    sys.path.insert(0, os.path.join(HERE, "tests"))
-    from oracle_util import field_errors, oracle, rel_l2
-    from maxwellfdtd_amd import _lib
+def host_cpu_info():
+    """What the CPU baseline ran on: logical CPUs of the host, CPUs this process may run
+    on, the cgroup CPU quota (if any), OMP_NUM_THREADS and the CPU model."""
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            info["cgroup_cpus"] = None if q == "max" else round(int(q) / int(per), 2)
+    except (OSError, ValueError):
+        info["cgroup_cpus"] = None
+    info["model"] = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    info["model"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return info
+
+
+def baseline_threads():
+    """All CPUs this process may use (sched_getaffinity), bounded by OMP_NUM_THREADS when
+    it is set (the CPU share a shared host grants) and by the cgroup quota."""
+    n = len(os.sched_getaffinity(0))
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if omp > 0:
+        n = min(n, omp)
+    q = host_cpu_info().get("cgroup_cpus")
+    if q:
+        n = min(n, max(1, int(q)))
+    return max(1, n)
+
+
+def oracle_rate(orc, desc, seconds, N, label=""):
+    """Steps the FP64 oracle for about `seconds`; returns (steps, wall, handle)."""
     from maxwellfdtd_amd.models import Handle
-    orc = oracle("f64")
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    threads = orc.set_threads(threads)
     ho = Handle(desc, orc)
     t0 = time.perf_counter()
     t_note = t0 + 60.0
@@ -74,14 +103,37 @@ def cpu_leg(desc, seconds, N, label, gpu_lib):
         ho.step(1)
         ns += 1
         if time.perf_counter() > t_note:    # long samples report as they go
-            log(f"cpu baseline: {ns} of {desc.Nt} steps after {time.perf_counter() - t0:.0f} s")
+            log(f"cpu baseline{label}: {ns} of {desc.Nt} steps after {time.perf_counter() - t0:.0f} s")
             t_note += 60.0
-    cw = time.perf_counter() - t0
+    return ns, time.perf_counter() - t0, ho
+
+
+def cpu_leg(desc, seconds, N, label, gpu_lib, seconds_1t=5.0):
+    """The oracle (FP64) on all usable host cores for a bounded number of steps, then on
+    one thread for a shorter sample, then the GPU engine for the first sample's steps:
+    CPU rates and GPU-vs-FP64 error."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    from oracle_util import field_errors, oracle, rel_l2
+    from maxwellfdtd_amd import _lib
+    from maxwellfdtd_amd.models import Handle
+    orc = oracle("f64")
+    info = host_cpu_info()
+    threads = orc.set_threads(baseline_threads())
+    ns, cw, ho = oracle_rate(orc, desc, seconds, N)
+    one = None
+    if seconds_1t > 0:
+        orc.set_threads(1)
+        n1, w1, h1 = oracle_rate(orc, desc, seconds_1t, N, " (1 thread)")
+        h1.close()
+        one = {"value": round(N * n1 / w1 / 1e9, 5), "steps": n1}
+        orc.set_threads(threads)
     cpu = {"value": round(N * ns / cw / 1e9, 4), "unit": "Gcell-updates/s", "cores": threads,
            "kind": "port",
            "sample": f"first {ns} steps of the same {label} model, FP64, {threads} OpenMP "
-                     f"threads (oracle/fdtd_oracle.c)"}
-    log(f"cpu baseline: {ns} steps in {cw:.1f}s -> {cpu['value']} Gcell/s")
+                     f"threads (oracle/fdtd_oracle.c, the reference CPU algorithm restated in C)",
+           "one_thread": one, "host": info}
+    log(f"cpu baseline: {ns} steps in {cw:.1f}s on {threads} threads -> {cpu['value']} Gcell/s; "
+        f"1 thread: {one}; host {info}")
     hg = Handle(desc, gpu_lib)
     hg.step(ns)
     errs = field_errors(hg, ho, desc.ndims)
@@ -94,6 +146,19 @@ def cpu_leg(desc, seconds, N, label, gpu_lib):
     hg.close()
     ho.close()
     return cpu, parity
+
+
+def graph_steps_for(k, default):
+    """Steps per graph replay for K timed steps: K itself up to 64, else the largest
+    divisor of K from 32 down to 8, else `default` (eager steps cover the remainder)."""
+    if default <= 0:
+        return default
+    if k <= 64:
+        return k
+    for g in range(32, 7, -1):
+        if k % g == 0:
+            return g
+    return default
 
 
 def base_record(args, value, wall, steps, dtype="f32"):
@@ -119,11 +184,15 @@ def bench_3d(args, lib):
         f"materials={len(desc.materials)}, integrals={desc.integrals}")
 
     h = Handle(desc, lib)
-    set_opts(lib, h, {0: args.graph_steps, 1: args.zchunk, 5: args.ty, 6: args.kform,
+    gsteps = graph_steps_for(args.steps, args.graph_steps)
+    set_opts(lib, h, {0: gsteps, 1: args.zchunk, 5: args.ty, 6: args.kform,
                       10: args.snake, 11: args.dless, 12: args.pml_split, 14: args.zchunk_h,
                       15: args.xcd_h, 16: args.xcd_block})
+    lib.check(lib.model_upload(h.h), "upload")
+    # the first step call builds, instantiates and uploads the step graph (whatever the
+    # warm-up length), so the timed steps are whole graph replays
     h.step(args.warmup)
-    log(f"warmup {args.warmup} steps done")
+    log(f"warmup {args.warmup} steps done (graph of {gsteps} steps built)")
     t0 = time.perf_counter()
     h.step(args.steps)
     wall = time.perf_counter() - t0
@@ -153,7 +222,7 @@ def bench_3d(args, lib):
     out["config"] = {"workload": f"plane_wave_spheres_3d: {args.n}^3 cells, 10-cell CPML, TFSF "
                                  f"plane wave, 40 dielectric spheres, Sa/E2 integrals "
                                  f"{'off' if args.no_integrals else 'on'}",
-                     "cells": N, "model_steps": desc.Nt, "graph_steps": args.graph_steps,
+                     "cells": N, "model_steps": desc.Nt, "graph_steps": gsteps,
                      "zchunk": args.zchunk, "rows_per_workgroup": args.ty, "kernel_form": args.kform,
                      "snake_order": args.snake, "plain_cells_without_d": args.dless,
                      "cpml_split": args.pml_split, "zchunk_h": args.zchunk_h, "xcd_chunked_h": args.xcd_h, "xcd_block": args.xcd_block}

@@ -254,14 +254,15 @@ struct mfd_server {
     void finish(Request *r, int state, int status, const std::string &err);
 };
 
+// Hands a request back to the caller.  From here on the caller may destroy its model at
+// any time, so the request forgets the handle: nothing of the server touches it again
+// (mfd_server_destroy included).
 void mfd_server::finish(Request *r, int state, int status, const std::string &err) {
     std::lock_guard<std::mutex> lk(mu);
     r->state_before_finish = r->state;
-    r->state = state;
     r->status = status;
     r->err = err;
     r->t_done = now_s();
-    r->h->owner = nullptr;
     in_use -= r->fp;
     if (r->state_before_finish == MFD_REQ_RUNNING) active_cells -= r->h->m.N;
     if (state == MFD_REQ_DONE) {
@@ -271,6 +272,9 @@ void mfd_server::finish(Request *r, int state, int status, const std::string &er
     } else {
         nfailed++;
     }
+    r->h->owner = nullptr;
+    r->h = nullptr;
+    r->state = state;
     cv_done.notify_all();
 }
 
@@ -299,7 +303,7 @@ void mfd_server::loop() {
                     lk.lock();
                 }
             }
-            int64_t reserve = (int64_t)1 << 30;
+            const int64_t reserve = (int64_t)1 << 30;
             while (!queue.empty() && (int)(active.size() + admitted.size()) < max_active) {
                 // the next request: the first to arrive, or the one with the least work
                 // (a queued request has not started, so its work is cells x Nt)
@@ -314,6 +318,17 @@ void mfd_server::loop() {
                     in_use += r->fp;       // finish() subtracts it
                     lk.unlock();
                     finish(r, MFD_REQ_FAILED, MFD_ERR_NOMEM, "request exceeds the server's HBM budget");
+                    lk.lock();
+                    continue;
+                }
+                if (r->fp + reserve > (int64_t)free_b && active.empty() && admitted.empty()) {
+                    // nothing of this server holds device memory, so waiting frees none:
+                    // the request does not fit what the device has free
+                    queue.erase(pick);
+                    in_use += r->fp;       // finish() subtracts it
+                    lk.unlock();
+                    finish(r, MFD_REQ_FAILED, MFD_ERR_NOMEM,
+                           "request does not fit the free device memory (footprint + 1 GiB reserve)");
                     lk.lock();
                     continue;
                 }
@@ -380,6 +395,10 @@ void mfd_server::loop() {
             active = keep;
             changed = true;
             runner.release();
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                nactive = (int)active.size();
+            }
             for (Request *r : done) {
                 Model &m = r->h->m;
                 m.sync();

@@ -131,6 +131,7 @@ struct DevModel {
     float *Sa, *E2;
     int *it;                    // device step counter: completed steps
     int *flag;                  // non-finite Ex: 1 NaN, 2 Inf
+    int *dftdone;               // steps already contracted into the DFT accumulators
     int nsrc, nmon;
     const DevSource *src;
     const DevMon *mon;
@@ -182,7 +183,11 @@ void launch_tfsf_gather(const DevModel *m, int nmodels, int slot, int maxn, bool
 void launch_tfsf_aux_table(const DevSource *S, int Nz, int Nt, int bcl, int bcr, float *tab,
                            hipStream_t s);
 void launch_monitors(const DevModel *m, int nmodels, int maxn, hipStream_t s);
-void launch_dft_flush(const DevModel *m, int nmodels, int maxn, int nk, hipStream_t s);
+// force = false: models whose DFT block completes with this step's sample (launched
+// after the monitors, before launch_finish); force = true: every model's pending partial
+// block, to be followed by launch_dft_commit
+void launch_dft_flush(const DevModel *m, int nmodels, int maxn, bool force, hipStream_t s);
+void launch_dft_commit(const DevModel *m, int nmodels, hipStream_t s);
 void launch_finish(const DevModel *m, int nmodels, hipStream_t s);
 void launch_collect_flags(const DevModel *m, int nmodels, int *out, hipStream_t s);
 // D = E/Me written back into the D arrays of a model that ran without stored D

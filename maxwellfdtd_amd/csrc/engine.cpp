@@ -476,10 +476,11 @@ int Model::upload() {
     m.dft_block = dft_block;
 
     int *ctr;
-    UP(arena.alloc(&ctr, 2));
-    UP(hipMemsetAsync(ctr, 0, 2 * sizeof(int), io));
+    UP(arena.alloc(&ctr, 4));
+    UP(hipMemsetAsync(ctr, 0, 4 * sizeof(int), io));
     m.it = ctr;
     m.flag = ctr + 1;
+    m.dftdone = ctr + 2;
     // the fills above run on the model's copy stream; runners use other streams, so
     // finish them before any kernel can read this model
     UP(hipStreamSynchronize(io));
@@ -553,7 +554,14 @@ int Model::write_field(int comp, const double *in) {
 Runner::~Runner() { release(); }
 
 void Runner::release() {
+    // work in flight on the runner's stream (a graph replay, the last step's kernels)
+    // reads the graph, the descriptors, the tile lists and the flags: wait for it before
+    // any of them is destroyed or freed, the graph first
+    if (stream) (void)hipStreamSynchronize(stream);
+    stream = nullptr;
     if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+    gsteps = 0;
+    gmask = 0;
     if (d_models) { (void)hipFree(d_models); d_models = nullptr; }
     if (d_tiles) { (void)hipFree(d_tiles); d_tiles = nullptr; d_tiles_e = nullptr; }
     if (d_chunks) { (void)hipFree(d_chunks); d_chunks = nullptr; }
@@ -760,7 +768,7 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     return MFD_OK;
 }
 
-int Runner::launch_step(bool timing) {
+int Runner::launch_step(bool timing, bool dft_flush) {
     const int nm = (int)models.size();
     int grid = ntiles;
     if (opt.persistent > 0) grid = std::min(ntiles, cu_count * opt.persistent);
@@ -809,6 +817,7 @@ int Runner::launch_step(bool timing) {
         }
     }
     if (any_mon) launch_monitors(d_models, nm, maxmon_n, stream);
+    if (dft_flush) launch_dft_flush(d_models, nm, maxmon_n, false, stream);
     if (any_int && !tfsf_gather) launch_integrals_touched(d_models, nm, maxtouched, ndims, stream);
     launch_finish(d_models, nm, stream);
     MFD_HIP(hipGetLastError());
@@ -822,82 +831,112 @@ int Runner::launch_step(bool timing) {
     return MFD_OK;
 }
 
-int Runner::build_graph(int steps) {
-    if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+// Host mirror of the device step bookkeeping (k_finish): the step counter, and the DFT
+// block position of a model with spectral monitors.
+static void host_advance(Model *m, int dft_block) {
+    if (m->it >= m->d.Nt) return;
+    m->it++;
+    if (m->it - m->dft_done == dft_block) m->dft_done = m->it;
+}
+
+// Steps k (0-based) of the next `steps` steps at which some model's DFT block completes:
+// k_dft_flush is launched there.  With steps a multiple of the block length, the same
+// steps are due on every replay until a model finishes or a read-back flushes a partial
+// block.
+uint64_t Runner::dft_mask(int steps) const {
+    uint64_t mask = 0;
+    if (!any_spectral) return 0;
+    for (const Model *m : models) {
+        if (!m->has_spectral) continue;
+        int it = m->it, done = m->dft_done;
+        for (int k = 0; k < steps && k < 64 && it < m->d.Nt; k++) {
+            if (it + 1 - done == dft_block) { mask |= 1ull << k; done = it + 1; }
+            it++;
+        }
+    }
+    return mask;
+}
+
+int Runner::build_graph(int steps, uint64_t mask) {
+    if (gexec) {
+        MFD_HIP(hipStreamSynchronize(stream));
+        (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+    }
     hipGraph_t g;
     MFD_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     int st = MFD_OK;
-    for (int k = 0; k < steps && st == MFD_OK; k++) {
-        st = launch_step(false);
-        if (st == MFD_OK && any_spectral && (k + 1) % dft_block == 0)
-            launch_dft_flush(d_models, (int)models.size(), maxmon_n, dft_block, stream);
-    }
+    for (int k = 0; k < steps && st == MFD_OK; k++) st = launch_step(false, (mask >> k) & 1);
     hipError_t e = hipStreamEndCapture(stream, &g);
-    if (st != MFD_OK) return st;
+    if (st != MFD_OK) {
+        if (e == hipSuccess) (void)hipGraphDestroy(g);
+        return st;
+    }
     if (e != hipSuccess) { set_error(std::string("graph capture: ") + hipGetErrorString(e)); return MFD_ERR_HIP; }
     e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) { set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return MFD_ERR_HIP; }
+    // the first replay then finds the graph on the device already
+    e = hipGraphUpload(gexec, stream);
+    if (e != hipSuccess) { set_error(std::string("graph upload: ") + hipGetErrorString(e)); return MFD_ERR_HIP; }
+    MFD_HIP(hipStreamSynchronize(stream));
     gsteps = steps;
+    gmask = mask;
     return MFD_OK;
 }
 
-// One step of every model; models that already finished are skipped on the device.
+int Runner::graph_len() const {
+    int G = opt.graph_steps;
+    if (any_spectral && G > 0) G = dft_block;   // due steps then repeat on every replay
+    return std::min(G, 64);
+}
+
+// Builds the step graph ahead of the first run that replays it, so that no capture or
+// instantiation happens inside a caller's timed steps.
+int Runner::prepare() {
+    const int G = graph_len();
+    if (G <= 0 || opt.kernel_timing) return MFD_OK;
+    const uint64_t need = dft_mask(G);
+    if (gexec && gsteps == G && (need & ~gmask) == 0) return MFD_OK;
+    return build_graph(G, need);
+}
+
+// nsteps steps of every model; models that already finished are skipped on the device.
 int Runner::run(int nsteps) {
     for (Model *m : dless_models) m->d_stale = true;
     int done = 0;
     const bool timing = opt.kernel_timing;
-    int G = opt.graph_steps;
-    if (any_spectral && G > 0) G = dft_block;
-    auto step_eager = [&]() -> int {
-        int st = launch_step(timing);
-        if (st != MFD_OK) return st;
-        for (Model *m : models) if (m->it < m->d.Nt) m->it++;
-        if (any_spectral) {
-            bool due = false;
-            for (Model *m : models) due |= m->has_spectral && (m->it - m->dft_done) == dft_block;
-            if (due) {
-                launch_dft_flush(d_models, (int)models.size(), maxmon_n, dft_block, stream);
-                for (Model *m : models) if (m->has_spectral && m->it - m->dft_done == dft_block) m->dft_done = m->it;
-            }
-        }
-        return MFD_OK;
-    };
+    const int G = graph_len();
     while (done < nsteps) {
-        bool aligned = true;
-        if (any_spectral)
-            for (Model *m : models) aligned &= (m->it - m->dft_done) % dft_block == 0;
-        if (G > 0 && !timing && aligned && nsteps - done >= G) {
-            if (!gexec || gsteps != G) {
-                int st = build_graph(G);
-                if (st != MFD_OK) return st;
-            }
+        if (G > 0 && !timing && nsteps - done >= G) {
+            int st = prepare();
+            if (st != MFD_OK) return st;
             MFD_HIP(hipGraphLaunch(gexec, stream));
-            for (Model *m : models) {
-                const int adv = std::min(G, m->d.Nt - m->it);
-                m->it += adv;
-                if (m->has_spectral) m->dft_done += (adv / dft_block) * dft_block;
-            }
+            for (int k = 0; k < G; k++)
+                for (Model *m : models) host_advance(m, dft_block);
             done += G;
         } else {
-            int st = step_eager();
+            int st = launch_step(timing, (dft_mask(1) & 1) != 0);
             if (st != MFD_OK) return st;
+            for (Model *m : models) host_advance(m, dft_block);
             done += 1;
         }
     }
     return check_flags();
 }
 
+// Contracts every model's partial DFT block (read-back of spectral monitors).
 int Runner::flush_dft_partial() {
     if (!any_spectral) return MFD_OK;
-    for (size_t k = 0; k < models.size(); k++) {
-        Model *m = models[k];
-        const int nk = m->it - m->dft_done;
-        if (!m->has_spectral || nk <= 0) continue;
-        launch_dft_flush(d_models + k, 1, m->maxmon_n, nk, stream);
-        m->dft_done = m->it;
-    }
+    bool any = false;
+    for (Model *m : models) any |= m->has_spectral && m->it > m->dft_done;
+    if (!any) return MFD_OK;
+    const int nm = (int)models.size();
+    launch_dft_flush(d_models, nm, maxmon_n, true, stream);
+    launch_dft_commit(d_models, nm, stream);
+    MFD_HIP(hipGetLastError());
     MFD_HIP(hipStreamSynchronize(stream));
+    for (Model *m : models) m->dft_done = m->it;
     return MFD_OK;
 }
 
@@ -1266,6 +1305,7 @@ int32_t mfd_model_step(mfd_model *h, int32_t nsteps) {
     if (h->owner) { set_error("model belongs to a batch or server"); return MFD_ERR_STATE; }
     if (h->m.it + nsteps > h->m.d.Nt) { set_error("model already ran all Nt steps"); return MFD_ERR_STATE; }
     int st = ensure_runner(h);
+    if (st == MFD_OK) st = h->runner.prepare();
     if (st != MFD_OK) return st;
     return h->runner.run(nsteps);
 }

@@ -172,6 +172,7 @@ struct Runner {
     hipStream_t stream = nullptr;
     hipGraphExec_t gexec = nullptr;
     int gsteps = 0;
+    uint64_t gmask = 0;         // steps of the graph that launch the DFT flush
     int cu_count = 256;
     std::vector<hipEvent_t> ev;
     double t_h_ms = 0, t_e_ms = 0;
@@ -180,8 +181,11 @@ struct Runner {
     ~Runner();
     int init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_t s);
     void release();
-    int launch_step(bool timing);     // one step of every model, on `stream`
-    int build_graph(int steps);
+    int launch_step(bool timing, bool dft_flush);   // one step of every model, on `stream`
+    uint64_t dft_mask(int steps) const;
+    int graph_len() const;
+    int build_graph(int steps, uint64_t mask);
+    int prepare();                    // build the step graph before the first replay
     int run(int nsteps);              // nsteps steps (models that finish are skipped)
     int flush_dft_partial();
     int check_flags();

@@ -432,18 +432,36 @@ void launch_monitors(const DevModel *m, int nmodels, int maxn, hipStream_t s) {
 }
 
 // ---- time-blocked DFT (src/dft.jl:27-51): S += sum_k F_k * 2 exp(-i w (t_k - tshift)) dt
-// The nk buffered samples of the last time block are contracted against an FP64 phase
-// table staged in LDS; runs after the step counter advanced past the block.
+// The buffered samples of steps dftdone .. itn-1 are contracted against an FP64 phase
+// table staged in LDS.  Each model keeps its own dftdone on the device, so models of one
+// run need not be aligned (a batch of uneven Nt, a server's staggered arrivals):
+//  - in the step (force = 0, after the monitors and before the step counter advances): a
+//    model whose block is complete with this step's sample (it + 1 - dftdone ==
+//    dft_block) is contracted; k_finish then moves its dftdone to it + 1;
+//  - at read-back (force = 1): every model's pending partial block (it - dftdone steps),
+//    after which k_dft_commit moves dftdone to it.
+// A model that has finished (it >= Nt) is never contracted in the step.
 constexpr int WCH = 16;     // frequencies per pass: 32 FP64 accumulators per lane
 constexpr int KMAX = 64;
 
-__global__ __launch_bounds__(256) void k_dft_flush(const DevModel *__restrict__ models, int nk) {
+__global__ __launch_bounds__(256) void k_dft_flush(const DevModel *__restrict__ models, int force) {
     __shared__ double2 ph[KMAX * WCH];
     __shared__ double sh[256];
     const DevModel &m = models[blockIdx.y];
-    const int itn = *m.it;
-    const int base = itn - nk;
-    if (base < 0 || nk <= 0) return;
+    if (m.nmon == 0) return;
+    const int it0 = *m.it;
+    const int base = *m.dftdone;
+    int itn;
+    if (force) {
+        itn = it0;
+    } else {
+        if (it0 >= m.Nt) return;
+        itn = it0 + 1;
+        if (itn - base != m.dft_block) return;
+    }
+    const int nk = itn - base;
+    // the sample ring holds dft_block <= KMAX steps
+    if (base < 0 || nk <= 0 || nk > m.dft_block || nk > KMAX) return;
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int stride = gridDim.x * blockDim.x;
     const double dt = m.dt;
@@ -510,11 +528,21 @@ __global__ __launch_bounds__(256) void k_dft_flush(const DevModel *__restrict__ 
     }
 }
 
-void launch_dft_flush(const DevModel *m, int nmodels, int maxn, int nk, hipStream_t s) {
+void launch_dft_flush(const DevModel *m, int nmodels, int maxn, bool force, hipStream_t s) {
     int nb = (maxn + 255) / 256;
     if (nb < 1) nb = 1;
     if (nb > 2048) nb = 2048;
-    hipLaunchKernelGGL(k_dft_flush, dim3(nb, nmodels), dim3(256), 0, s, m, nk);
+    hipLaunchKernelGGL(k_dft_flush, dim3(nb, nmodels), dim3(256), 0, s, m, force ? 1 : 0);
+}
+
+// After a forced flush: every model's samples up to its step counter are contracted.
+__global__ void k_dft_commit(const DevModel *__restrict__ models, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) *models[k].dftdone = *models[k].it;
+}
+
+void launch_dft_commit(const DevModel *m, int nmodels, hipStream_t s) {
+    hipLaunchKernelGGL(k_dft_commit, dim3((nmodels + 255) / 256), dim3(256), 0, s, m, nmodels);
 }
 
 // ---- integral variables of source-written cells: update_integral_variables!
@@ -562,12 +590,17 @@ void launch_integrals_touched(const DevModel *m, int nmodels, int maxn, int ndim
 }
 
 // ---- step counter ----------------------------------------------------------------------
+// Advances every unfinished model's step counter; a model whose DFT block was completed
+// (and contracted by k_dft_flush) in this step moves its dftdone along.
 __global__ void k_finish(const DevModel *__restrict__ models, int n) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) {
         const DevModel &m = models[k];
         const int it = *m.it;
-        if (it < m.Nt) *m.it = it + 1;
+        if (it < m.Nt) {
+            *m.it = it + 1;
+            if (it + 1 - *m.dftdone == m.dft_block) *m.dftdone = it + 1;
+        }
     }
 }
 

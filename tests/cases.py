@@ -124,10 +124,16 @@ def beer_lambert_ok(Sa, z, ki, rtol=1e-2):
 
 
 # ---- seeded all-feature models ---------------------------------------------------------
+def _cpml(cells, d, kmax):
+    """CPML of `cells` cells (kappa up to kmax), or the plain thickness when kmax is None."""
+    return cells * d if kmax is None else CPML(thickness=cells * d, kmax=kmax)
+
+
 def spheres_3d(n=32, steps=120, seed=7, tfsf=True, soft=True, monitors=True, nspheres=5,
-               bc="periodic", pml=4, nx=None):
-    """n^3 grid (nx by n by n when nx is given), CPML `pml` cells on every face, seeded
-    dielectric spheres, a TFSF plane wave and a soft point source."""
+               bc="periodic", pml=4, nx=None, kmax=None):
+    """n^3 grid (nx by n by n when nx is given), CPML `pml` cells on every face (kappa
+    profile up to kmax when given), seeded dielectric spheres, a TFSF plane wave and a
+    soft point source."""
     rng = np.random.default_rng(seed)
     d = 40e-9
     L = (n - 1) * d
@@ -155,7 +161,7 @@ def spheres_3d(n=32, steps=120, seed=7, tfsf=True, soft=True, monitors=True, nsp
         srcs.append(SoftSource(geometry=(0.45 * Lx, 0.5 * L, 0.55 * L),
                                waveform=lambda x, y, z, t: 1e3 * w(t) + 0 * x, component="Ey"))
     dt = 0.99 / C0 / np.sqrt(3 / d ** 2)
-    model = Model(grid, srcs, tmax=(steps - 1) * dt, CN=0.99, pml=pml * d, material=mats, bc=bc)
+    model = Model(grid, srcs, tmax=(steps - 1) * dt, CN=0.99, pml=_cpml(pml, d, kmax), material=mats, bc=bc)
     mons = None
     if monitors:
         w0 = 2 * np.pi * C0 / 800e-9
@@ -216,7 +222,7 @@ def nonlinear_3d(n=24, steps=150, I0=1e17, rate="power"):
     return model
 
 
-def lorentz_2d(nx=96, nz=80, steps=200, seed=3, tfsf=True):
+def lorentz_2d(nx=96, nz=80, steps=200, seed=3, tfsf=True, kmax=None):
     """2D (x, z) Lorentz rods and a dielectric slab, TFSF plane wave + soft point source."""
     rng = np.random.default_rng(seed)
     d = 40e-9
@@ -237,7 +243,7 @@ def lorentz_2d(nx=96, nz=80, steps=200, seed=3, tfsf=True):
         srcs.append(TFSFSource(box=(8 * d, Lx - 8 * d, 8 * d, Lz - 8 * d), waveform=w,
                                component="Ex", z0=6 * d))
     dt = 0.99 / C0 / np.sqrt(2 / d ** 2)
-    model = Model(grid, srcs, tmax=(steps - 1) * dt, CN=0.99, pml=5 * d, material=mats)
+    model = Model(grid, srcs, tmax=(steps - 1) * dt, CN=0.99, pml=_cpml(5, d, kmax), material=mats)
     w0 = 2 * np.pi * C0 / 800e-9
     mons = [FieldMonitor((0.5 * Lx, 0.8 * Lz)),
             SpectralMonitor(lambda x, z: (np.abs(z - 0.8 * Lz) < d / 2) & (x > 0.2 * Lx) & (x < 0.8 * Lx),
@@ -271,7 +277,7 @@ def nonlinear_2d(nx=96, nz=80, steps=180, I0=1e17):
     return Model(grid, srcs, tmax=(steps - 1) * dt, CN=0.99, pml=5 * d, material=mats)
 
 
-def slab_1d(nz=600, steps=900, dispersive=True):
+def slab_1d(nz=600, steps=900, dispersive=True, kmax=None):
     """1D: CPML, a Lorentz + conductive slab and a soft source, plus monitors."""
     d = 40e-9
     grid = Grid(zmin=0, zmax=(nz - 1) * d, Nz=nz)
@@ -282,7 +288,7 @@ def slab_1d(nz=600, steps=900, dispersive=True):
     src = SoftSource(geometry=lambda z: np.abs(z - 0.125 * L) < d / 2, waveform=lambda z, t: w(t) + 0 * z,
                      component="Ex")
     dt = 0.99 / C0 * d
-    model = Model(grid, src, tmax=(steps - 1) * dt, CN=0.99, pml=10 * d, material=mats)
+    model = Model(grid, src, tmax=(steps - 1) * dt, CN=0.99, pml=_cpml(10, d, kmax), material=mats)
     w0 = 2 * np.pi * C0 / 800e-9
     mons = [FieldMonitor((0.875 * L,)),
             SpectralMonitor((0.875 * L,), w=2 * np.pi * C0 / np.linspace(600e-9, 1000e-9, 32))]
@@ -320,3 +326,82 @@ def tfsf_faces_3d(n=40, steps=150, pml=4, lo=9, hi=30):
     mons = [FieldMonitor((x[lo], mid, mid)), FieldMonitor((x[lo - 1], mid, mid)),
             SpectralMonitor((x[hi], mid, mid), w=np.array([w0, 1.2 * w0]))]
     return model, mons
+
+
+# ---- TFSF replay tables ----------------------------------------------------------------
+def tfsf_table_shapes(ndims, box, Nt):
+    """Shapes of the replay arrays of TFSFSource1D/2D/3D for a 1-based box
+    (src/sources.jl:269-282,387-399,565-591 and the record pass :779-877): per face, the
+    tangential components, each [face extent..., Nt] with the extents of prepare_tfsf_record."""
+    X1, X2, Y1, Y2, Z1, Z2 = box
+    Nxi, Nyi, Nzi = X2 - X1, Y2 - Y1, Z2 - Z1
+    if ndims == 1:
+        return [(Nt,)] * 4
+    if ndims == 2:
+        return [(Nzi, Nt)] * 4 + [(Nxi, Nt)] * 4
+    out = []
+    for face in range(6):
+        a, b = ((Nyi, Nzi), (Nxi, Nzi), (Nxi, Nyi))[face // 2]
+        for j in range(4):
+            big = j in (0, 3)
+            out.append((a + 1, b, Nt) if big else (a, b + 1, Nt))
+    return out
+
+
+def synthetic_tfsf_tables(ndims, box, t, seed=5, E0=1.0):
+    """Seeded replay tables of the right shapes: a Gaussian pulse in time whose amplitude
+    and delay vary smoothly across each face, H tables scaled by 1/eta0.  They need not be
+    a physical plane wave: the engine and the oracle replay any table the same way."""
+    rng = np.random.default_rng(seed)
+    eta0 = np.sqrt(4e-7 * np.pi / EPS0)
+    Nt = len(t)
+    dt = t[1] - t[0]
+    tabs = []
+    for k, shp in enumerate(tfsf_table_shapes(ndims, box, Nt)):
+        sp = shp[:-1]
+        amp = 1 + 0.3 * rng.standard_normal(sp) if sp else 1.0
+        delay = 0.3 * Nt * dt + (0.05 * Nt * dt * rng.uniform(-1, 1, sp) if sp else 0.0)
+        tt = np.asarray(t).reshape((1,) * len(sp) + (Nt,))
+        amp = np.asarray(amp)[..., None] if sp else amp
+        delay = np.asarray(delay)[..., None] if sp else delay
+        width = 0.08 * Nt * dt
+        v = E0 * amp * np.exp(-((tt - delay) / width) ** 2) * np.cos(2 * np.pi * (tt - delay) / (0.1 * Nt * dt))
+        is_h = (ndims == 3 and k % 4 in (0, 1)) or (ndims < 3 and k % 2 == 0)
+        tabs.append(v / eta0 if is_h else v)
+    return tabs
+
+
+def tfsf_tables_model(ndims, steps=120, pml=4, kmax=None):
+    """A model driven by TFSF replay tables (the path the reference's TFSFSource takes):
+    a dielectric scatterer inside the box, CPML around it."""
+    d = 40e-9
+    if ndims == 1:
+        nz = 300
+        grid = Grid(zmin=0, zmax=(nz - 1) * d, Nz=nz)
+        L = (nz - 1) * d
+        box = (0.3 * L, 0.7 * L)
+        mat = Material(geometry=lambda z: (z > 0.45 * L) & (z < 0.55 * L), eps=2.25)
+        dt = 0.99 / C0 * d
+    elif ndims == 2:
+        nx, nz = 72, 64
+        grid = Grid(xmin=0, xmax=(nx - 1) * d, Nx=nx, zmin=0, zmax=(nz - 1) * d, Nz=nz)
+        Lx, Lz = (nx - 1) * d, (nz - 1) * d
+        box = (10 * d, Lx - 10 * d, 9 * d, Lz - 9 * d)
+        mat = Material(geometry=lambda x, z: (x - 0.5 * Lx) ** 2 + (z - 0.5 * Lz) ** 2 <= (6 * d) ** 2, eps=3.0)
+        dt = 0.99 / C0 / np.sqrt(2 / d ** 2)
+    else:
+        n = 36
+        L = (n - 1) * d
+        grid = Grid(xmin=0, xmax=L, Nx=n, ymin=0, ymax=L, Ny=n, zmin=0, zmax=L, Nz=n)
+        box = (8 * d, L - 8 * d, 9 * d, L - 9 * d, 10 * d, L - 10 * d)
+        mat = Material(geometry=lambda x, y, z: (x - L / 2) ** 2 + (y - L / 2) ** 2 + (z - L / 2) ** 2
+                       <= (5 * d) ** 2, eps=2.5)
+        dt = 0.99 / C0 / np.sqrt(3 / d ** 2)
+    tmax = (steps - 1) * dt
+    # the box indices the model will use, from a plane-wave placeholder of the same box
+    probe = Model(grid, TFSFSource(box=box, waveform=lambda t: 0 * t, component="Ex"), tmax=tmax,
+                  CN=0.99, pml=_cpml(pml, d, kmax))
+    tabs = synthetic_tfsf_tables(ndims, probe.source_structs[0]["box"], probe.t)
+    model = Model(grid, TFSFSource(box=box, tables=tabs), tmax=tmax, CN=0.99,
+                  pml=_cpml(pml, d, kmax), material=mat)
+    return model

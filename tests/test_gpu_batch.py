@@ -10,7 +10,7 @@ import pytest
 
 from maxwellfdtd_amd import _lib, workloads
 from maxwellfdtd_amd.batch import Batch, Server, replay
-from maxwellfdtd_amd.models import Handle
+from maxwellfdtd_amd.models import Handle, MONITOR_COMPONENTS
 from oracle_util import field_errors, oracle, rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -64,6 +64,104 @@ def test_batch_2d_equals_alone_and_oracle():
         assert max(errs.values()) <= 1e-4, errs
         ho.close()
     b.close()
+
+
+def spectral_models(sizes_steps, seed=41):
+    """3D models with field and spectral monitors (DFT over blocks of 32 steps): step
+    counts that are not multiples of the block, so blocks of different solves end at
+    different times and every solve ends inside a block."""
+    import cases
+    return [cases.spheres_3d(n=n, steps=s, seed=seed + k)[0].describe(
+        monitors=cases.spheres_3d(n=n, steps=s, seed=seed + k)[1])
+        for k, (n, s) in enumerate(sizes_steps)]
+
+
+def monitors_of(h):
+    names = MONITOR_COMPONENTS[h.desc.ndims]
+    return [[h.monitor(i, c) for c in names] for i in range(len(h.desc.monitors))]
+
+
+def assert_same_monitors(a, b):
+    for i, (ma, mb) in enumerate(zip(a, b)):
+        for ca, cb in zip(ma, mb):
+            assert np.array_equal(ca, cb), (i, np.max(np.abs(np.asarray(ca) - np.asarray(cb))))
+
+
+def test_batch_spectral_monitors_uneven_nt():
+    """Spectral monitors in a batch of unequal Nt (45, 83, 130 steps) equal the solo
+    runs bitwise: each solve contracts its own DFT blocks, and a solve that finished is
+    not contracted again."""
+    descs = spectral_models([(20, 45), (24, 83), (20, 130)])
+    solo = []
+    for d in descs:
+        h = alone(d)
+        solo.append((monitors_of(h), h.field("Ex")))
+        h.close()
+    b = Batch(descs, options={_lib.MFD_OPT_GRAPH_STEPS: 16})
+    b.run(0)
+    for (mons, ex), hb in zip(solo, b.handles):
+        assert np.array_equal(hb.field("Ex"), ex)
+        assert_same_monitors(monitors_of(hb), mons)
+    b.close()
+
+
+def test_server_spectral_monitors_staggered():
+    """Spectral monitors of requests admitted at different times (small quanta, so the
+    solves' DFT blocks are out of phase) equal the solo runs bitwise."""
+    descs = spectral_models([(20, 70), (22, 101), (20, 59), (18, 90)], seed=43)
+    solo = []
+    for d in descs:
+        h = alone(d)
+        solo.append(monitors_of(h))
+        h.close()
+    srv = Server(options={_lib.MFD_OPT_QUANTUM: 5})
+    got = {}
+
+    def done(t, h, lat):
+        got[t] = monitors_of(h)
+
+    arrivals = [(0.004 * k, d) for k, d in enumerate(descs)]
+    tickets = replay(srv, arrivals, describe=lambda d: d, on_done=done)
+    assert srv.stats()["done"] == len(descs)
+    for t, mons in zip(tickets, solo):
+        assert_same_monitors(got[t], mons)
+    srv.close()
+
+
+def test_server_fails_what_free_memory_cannot_hold():
+    """A request within the HBM budget but larger than the device's free memory, with
+    nothing else running, fails with MFD_ERR_NOMEM instead of waiting for memory that no
+    request of the server will free; the server's statistics then show nothing active."""
+    from maxwellfdtd_amd import FieldMonitor, Grid, Model, SoftSource
+    n, d = 32, 40e-9
+    L = (n - 1) * d
+    grid = Grid(xmin=0, xmax=L, Nx=n, ymin=0, ymax=L, Ny=n, zmin=0, zmax=L, Nz=n)
+    src = SoftSource(geometry=(L / 2, L / 2, L / 2), waveform=lambda x, y, z, t: 0 * t,
+                     component="Ex")
+    dt = 0.99 / 3e8 / np.sqrt(3 / d ** 2)
+    # a field monitor on every cell for 10^6 steps: 6 x 32768 x 10^6 x 4 B = 786 GB
+    model = Model(grid, src, tmax=(10 ** 6 - 1) * dt, CN=0.99, pml=0)
+    desc = model.describe(monitors=[FieldMonitor(lambda x, y, z: 0 * x + 0 * y + 0 * z == 0)])
+    srv = Server(hbm_budget=10 ** 15)
+    t = srv.submit(desc)[0]
+    state = srv.wait(t, 60000)
+    st, code, _ = srv.poll(t)
+    assert state == _lib.MFD_REQ_FAILED and st == _lib.MFD_REQ_FAILED and code == _lib.MFD_ERR_NOMEM
+    stats = srv.stats()
+    assert stats["failed"] == 1 and stats["active"] == 0 and stats["queued"] == 0
+    srv.close()
+
+
+def test_server_stats_after_retirement():
+    """After the last request retires, the statistics report no active request."""
+    descs = [m.describe() for m in small_3d(count=2, seed=17)]
+    srv = Server(options={_lib.MFD_OPT_QUANTUM: 8})
+    for d in descs:
+        srv.submit(d)
+    assert srv.wait(0, 120000) == _lib.MFD_REQ_DONE
+    st = srv.stats()
+    assert st["done"] == 2 and st["active"] == 0 and st["queued"] == 0
+    srv.close()
 
 
 def test_batch_autotune_keeps_results():

@@ -142,6 +142,43 @@ def test_slab_1d():
     assert_parity(hg, ho, desc, model)
 
 
+@pytest.mark.parametrize("kmax", [3.0, 5.0])
+@pytest.mark.parametrize("ndims,kform", [(1, None), (2, None), (3, 2), (3, 0)])
+def test_cpml_kappa(ndims, kform, kmax):
+    """CPML with a kappa profile (kmax > 1): the slab updates divide the E/H differences by
+    K after the psi recursion (src/models.jl:812-871, src/pml.jl:68-81).  With kmax = 1 the
+    division is the identity, so only these cases exercise it on the GPU."""
+    opts = None
+    if ndims == 3:
+        model, mons = cases.spheres_3d(n=28, steps=100, kmax=kmax)
+        opts = {_lib.MFD_OPT_KERNEL_FORM: kform}
+    elif ndims == 2:
+        model, mons = cases.lorentz_2d(steps=160, kmax=kmax)
+    else:
+        # 550 steps: the left-going half of the pulse has crossed the left layer and its
+        # reflection has passed the monitors.  (Much later, once the pulse has left, the
+        # point monitor holds a small residual whose FP32-vs-FP64 difference alone, in the
+        # oracle's own FP32 build, exceeds 1e-4.)
+        model, mons = cases.slab_1d(steps=550, kmax=kmax)
+    present = [L for L in model.pml if L.present]
+    assert present and all(np.isclose(L.K.max(), kmax) for L in present)
+    hg, ho, desc = run_pair(model, mons, options=opts)
+    assert_parity(hg, ho, desc, model)
+
+
+@pytest.mark.parametrize("ndims,kform", [(1, None), (2, None), (3, 2), (3, 0)])
+def test_tfsf_replay_tables(ndims, kform):
+    """TFSF from replay tables (MFD_TFSF_TABLES), the form in which the reference's
+    TFSFSource reaches add_source! (src/sources.jl:269-346,400-497,565-776): 4 tables in
+    1D, 8 in 2D, 24 in 3D, each [face..., Nt], around a dielectric scatterer."""
+    model = cases.tfsf_tables_model(ndims)
+    assert model.source_structs[0]["mode"] == _lib.MFD_TFSF_TABLES
+    opts = {_lib.MFD_OPT_KERNEL_FORM: kform} if kform is not None else None
+    hg, ho, desc = run_pair(model, options=opts)
+    assert_parity(hg, ho, desc, model)
+    assert np.max(np.abs(ho.field("Ex"))) > 1e-2
+
+
 @pytest.mark.parametrize("kform", [2, 1, 0])
 def test_tfsf_faces_through_materials(kform):
     """TFSF as the only source (one gather launch per step; with the pipelined kernels
