@@ -188,6 +188,13 @@ int32_t mfd_model_get_monitor(mfd_model *m, int32_t imon, int32_t comp, double *
 /* MFD_OUT_SA / MFD_OUT_E2 / MFD_OUT_JE / MFD_OUT_RHO (sum of rho*rho0 over materials),
  * Nx*Ny*Nz FP64 values (src/outputs.jl:46-65).                                         */
 int32_t mfd_model_get_output(mfd_model *m, int32_t which, double *out);
+/* Total field energy of the current state: out[0] = W_E = 1/2 sum eps0*eps*|E|^2 dV,
+ * out[1] = W_H = 1/2 sum mu0*mu*|H|^2 dV, from E^n and H^(n-1/2), summed in FP64 on the
+ * device (dV = dx*dy*dz in 3D, dx*dz in 2D, dz in 1D), with eps and mu of the cell's
+ * material (eps0*eps = 1/Me, mu0*mu = dt/Mh, src/materials.jl:393-400).  No
+ * counterpart in the reference, which computes no energy: it is the energy check that
+ * field snapshots are judged by.  Allocates a small partial-sum buffer per call.       */
+int32_t mfd_model_get_energy(mfd_model *m, double *out);
 int32_t mfd_model_destroy(mfd_model *m);
 
 /* ---- engine options and instrumentation (no counterpart in the reference: its backends

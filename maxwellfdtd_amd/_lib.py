@@ -145,6 +145,7 @@ ENGINE_EXTRA = {
     "device_info": (I32, [C.POINTER(C.c_int64), C.POINTER(C.c_int64), IP, C.c_char_p, I32]),
     "model_set_options": (I32, [P, I32, I32]),
     "model_timing": (I32, [P, I32, DP, IP]),
+    "model_get_energy": (I32, [P, DP]),
     "stream_copy": (I32, [C.c_int64, I32, DP]),
     "stream_copy_fast": (I32, [C.c_int64, I32, DP]),
     "stream_multi": (I32, [C.c_int64, I32, I32, I32, I32, DP]),

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmaxwellfdtd_amd.so")
 SOURCES = ["kernels3d.hip", "kernels3d_v4.hip", "kernels3d_p.hip", "kernels2d.hip", "kernels1d.hip", "step_kernels.hip",
-           "kernels_state.hip", "engine.cpp", "batch.cpp", "stream_copy.hip"]
+           "kernels_state.hip", "energy.hip", "engine.cpp", "batch.cpp", "stream_copy.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=fast",

@@ -281,6 +281,12 @@ class Handle:
         self.lib.check(self.lib.model_get_output(self.h, which, _lib.dptr(out)), "get_output")
         return out
 
+    def energy(self):
+        """(W_E, W_H): total electric and magnetic field energy of the current state."""
+        out = np.zeros(2)
+        self.lib.check(self.lib.model_get_energy(self.h, _lib.dptr(out)), "get_energy")
+        return float(out[0]), float(out[1])
+
     def monitor(self, imon, comp):
         mon = self.desc.monitors[imon]
         n = 1 if mon["issum"] else len(mon["cells"])

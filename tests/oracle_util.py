@@ -64,3 +64,22 @@ def field_errors(ha, hb, ndims):
         b = np.concatenate([hb.field(c) for c in comps])
         out[g] = rel_l2(a, b)
     return out
+
+
+def field_energy(h):
+    """(W_E, W_H) = 1/2 sum eps0 eps |E|^2 dV, 1/2 sum mu0 mu |H|^2 dV of a handle's
+    current fields, in FP64 on the host: eps0*eps = 1/Me and mu0*mu = dt/Mh of each
+    cell's material (vacuum for id 0), dV = dx dy dz / dx dz / dz."""
+    d = h.desc
+    geo = np.asarray(d.geometry)
+    Me = np.array([d.vac[1]] + [c["Me"] for c in d.materials], dtype=float)
+    Mh = np.array([d.vac[0]] + [c["Mh"] for c in d.materials], dtype=float)
+    eps = 1.0 / Me[geo]
+    mu = d.dt / Mh[geo]
+    names = {1: (("Ex",), ("Hy",)), 2: (("Ex", "Ez"), ("Hy",)),
+             3: (("Ex", "Ey", "Ez"), ("Hx", "Hy", "Hz"))}[d.ndims]
+    e2 = sum(h.field(c) ** 2 for c in names[0])
+    h2 = sum(h.field(c) ** 2 for c in names[1])
+    dx, dy, dz = d.d
+    dV = {1: dz, 2: dx * dz, 3: dx * dy * dz}[d.ndims]
+    return 0.5 * dV * float(np.sum(eps * e2)), 0.5 * dV * float(np.sum(mu * h2))

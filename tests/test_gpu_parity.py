@@ -6,9 +6,9 @@ import numpy as np
 import pytest
 
 import cases
-from maxwellfdtd_amd import _lib, solve
+from maxwellfdtd_amd import EPS0, _lib, solve
 from maxwellfdtd_amd.models import Handle, MONITOR_COMPONENTS
-from oracle_util import field_errors, oracle, rel_l2
+from oracle_util import field_energy, field_errors, oracle, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -339,3 +339,66 @@ def test_nonfinite_is_reported():
     h = Handle(desc, _lib.engine())
     with pytest.raises(_lib.EngineError, match="I found (NaN|Inf) field values"):
         h.step(model.Nt)
+
+
+# ---- total field energy ----------------------------------------------------------------
+def test_energy_matches_fp64_fields_along_the_run():
+    """The engine's total field energy (W_E, W_H) against the same sums over the FP64
+    oracle's fields, at several steps of a run with CPML, TFSF, a soft source and
+    dielectric spheres (mu = 1 and eps from the materials)."""
+    model, mons = cases.spheres_3d(n=32, steps=160)
+    desc = model.describe()
+    hg = Handle(desc, _lib.engine())
+    ho = Handle(desc, oracle("f64"))
+    for n in (40, 40, 80):
+        hg.step(n)
+        ho.step(n)
+        we, wh = hg.energy()
+        oe, oh = field_energy(ho)
+        assert oe > 0 and oh > 0
+        assert abs(we / oe - 1) <= 1e-4 and abs(wh / oh - 1) <= 1e-4, (we, oe, wh, oh)
+
+
+def test_energy_closed_form_and_conservation():
+    """A uniform Ex in a periodic box, half of it dielectric: W_E = 1/2 eps0 eps E0^2 V
+    exactly and W_H = 0.  And in a closed lossless periodic box, once the source has
+    stopped, the discrete energy of the leapfrog scheme, W_E(E^n) + 1/2 sum mu0 mu
+    H^(n-1/2).H^(n+1/2) dV, stays constant (the plain sum of squares swings by several
+    per cent: E and H are half a step apart)."""
+    from maxwellfdtd_amd import Grid, Material, Model, SoftSource
+    n, d = 16, 40e-9
+    L = (n - 1) * d
+    grid = Grid(xmin=0, xmax=L, Nx=n, ymin=0, ymax=L, Ny=n, zmin=0, zmax=L, Nz=n)
+    src = SoftSource(geometry=(L / 2, L / 2, L / 2), waveform=lambda x, y, z, t: 0 * t,
+                     component="Ex")
+    mat = Material(geometry=lambda x, y, z: z < L / 2, eps=2.5)
+    model = Model(grid, src, tmax=10 * d / 3e8, pml=0, material=mat)
+    h = Handle(model.describe(integrals=0), _lib.engine())
+    E0 = 3.0e5
+    h.set_field("Ex", np.full(grid.shape, E0))
+    we, wh = h.energy()
+    eps_planes = np.where(np.arange(n) * d < L / 2, 2.5, 1.0)
+    expect = 0.5 * EPS0 * E0 ** 2 * d ** 3 * n * n * float(np.sum(eps_planes))
+    assert abs(we / expect - 1) < 1e-6 and wh == 0.0
+
+    pm, _ = cases.spheres_3d(n=24, steps=900, tfsf=False, monitors=False, pml=0, nspheres=3)
+    desc = pm.describe(integrals=0)
+    hg = Handle(desc, _lib.engine())
+    Mh = np.array([desc.vac[0]] + [c["Mh"] for c in desc.materials])
+    mu = desc.dt / Mh[desc.geometry]
+    dV = float(np.prod(desc.d))
+
+    def discrete_energy():
+        we_, _ = hg.energy()
+        H0 = [hg.field(c) for c in ("Hx", "Hy", "Hz")]
+        hg.step(1)
+        H1 = [hg.field(c) for c in ("Hx", "Hy", "Hz")]
+        return we_ + 0.5 * dV * float(np.sum(mu * sum(a * b for a, b in zip(H0, H1))))
+
+    hg.step(450)                 # the soft source's pulse is over by then
+    w = [discrete_energy()]
+    for _ in range(4):
+        hg.step(99)
+        w.append(discrete_energy())
+    w = np.array(w)
+    assert w[0] > 0 and np.max(np.abs(w / w[0] - 1)) < 1e-5, w / w[0] - 1
