@@ -195,6 +195,15 @@ int32_t mfd_model_get_output(mfd_model *m, int32_t which, double *out);
  * counterpart in the reference, which computes no energy: it is the energy check that
  * field snapshots are judged by.  Allocates a small partial-sum buffer per call.       */
 int32_t mfd_model_get_energy(mfd_model *m, double *out);
+/* mfd_model_get_energy without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The model must already be uploaded.
+ * The partial sums go to a device array inside the library (2048 FP64 pairs, 32 KiB per
+ * GPU, held until the process ends; any model size fits); the result pair (W_E, W_H)
+ * is written to the caller's device memory d_out[0..1], asynchronously on `stream` (a
+ * hipStream_t; NULL: the model's own copy stream).  Calls that overlap on one GPU share
+ * that array and race: serialise them.                                                 */
+int32_t mfd_model_get_energy_noalloc(mfd_model *m, double *d_out, void *stream);
 int32_t mfd_model_destroy(mfd_model *m);
 
 /* ---- engine options and instrumentation (no counterpart in the reference: its backends

@@ -146,6 +146,7 @@ ENGINE_EXTRA = {
     "model_set_options": (I32, [P, I32, I32]),
     "model_timing": (I32, [P, I32, DP, IP]),
     "model_get_energy": (I32, [P, DP]),
+    "model_get_energy_noalloc": (I32, [P, P, P]),
     "stream_copy": (I32, [C.c_int64, I32, DP]),
     "stream_copy_fast": (I32, [C.c_int64, I32, DP]),
     "stream_multi": (I32, [C.c_int64, I32, I32, I32, I32, DP]),

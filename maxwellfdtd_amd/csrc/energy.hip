@@ -11,6 +11,7 @@
 // fixed order.  The result depends only on the cell count, not on the stream or call.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "engine.hpp"
@@ -107,6 +108,42 @@ void launch_energy(const Model &m, double2 *part, double *out, hipStream_t s) {
                        0.5 * a.dV, out);
 }
 
+// ---- allocation-free variant ----------------------------------------------------------
+// Partial sums of mfd_model_get_energy_noalloc: one file-scope device array, of which
+// every GPU the library runs on holds its own copy until the process ends.  Each call's
+// first launch writes every slot its second launch reads, so nothing has to start at
+// zero; calls that overlap on one GPU (two streams, two threads) share the array and
+// race.  In host code the array's name is only a stand-in for the symbol: its device
+// address, per device, comes from hipGetSymbolAddress and is what kernels receive.
+static __device__ __align__(256) double2 scratch[ENERGY_MAX_BLOCKS];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double2 *g_scratch[SCRATCH_MAX_DEVICES];   // device address of `scratch`, per device
+
+// The device address of this device's `scratch`, looked up once per device (with that
+// device current) on the first call there.
+static int scratch_address(double2 **out) {
+    int dev = 0;
+    MFD_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) {
+        set_error("energy scratch: device index beyond the 64 the library keeps addresses for");
+        return MFD_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("energy scratch: hipGetSymbolAddress: ") + hipGetErrorString(e));
+            return MFD_ERR_HIP;
+        }
+        g_scratch[dev] = (double2 *)p;
+    }
+    *out = g_scratch[dev];
+    return MFD_OK;
+}
+
 }  // namespace mfd
 
 using namespace mfd;
@@ -128,6 +165,31 @@ int32_t mfd_model_get_energy(mfd_model *h, double *out) {
     if (e != hipSuccess) { set_error(std::string("energy: ") + hipGetErrorString(e)); return MFD_ERR_HIP; }
     out[0] = res[0];
     out[1] = res[1];
+    return MFD_OK;
+}
+
+// Same sums as mfd_model_get_energy with no allocation: the partial sums go to the
+// file-scope `scratch` array, the result (W_E, W_H) to the caller's device memory
+// d_out[0..1], asynchronously on `stream` (null: the model's copy stream).
+int32_t mfd_model_get_energy_noalloc(mfd_model *h, double *d_out, void *stream) {
+    if (!h || !d_out) { set_error("null argument"); return MFD_ERR_ARG; }
+    Model &m = h->m;
+    if (!m.uploaded) {
+        set_error("energy without allocation needs an uploaded model (mfd_model_upload or a step)");
+        return MFD_ERR_STATE;
+    }
+    const int nb = energy_blocks(m.N);
+    if (nb > ENERGY_MAX_BLOCKS) {
+        set_error("energy without allocation: " + std::to_string(nb) + " partial sums exceed the " +
+                  std::to_string(ENERGY_MAX_BLOCKS) + " of the device scratch array");
+        return MFD_ERR_ARG;
+    }
+    double2 *part = nullptr;
+    const int st = scratch_address(&part);
+    if (st != MFD_OK) return st;
+    hipStream_t s = stream ? (hipStream_t)stream : m.io;
+    launch_energy(m, part, d_out, s);
+    MFD_HIP(hipGetLastError());
     return MFD_OK;
 }
 
