@@ -22,6 +22,17 @@
  * Compiled twice: -DOFD_REAL=double (the FP64 reference run) and -DOFD_REAL=float
  * (the reference's FP32 GPU arithmetic, used to measure the intrinsic FP32 gap).
  * The exported symbols mirror include/maxwell_fdtd.h with the prefix ofd_.
+ *
+ * How each part is pinned (tests/): the reference's own propagation and loss tests;
+ * the exact one-cell-per-step translation at CN = 1; ADE coefficients against chi(w);
+ * Kerr against the cubic; plasma density against its closed form; the plasma current
+ * against the Drude response and the multiphoton loss against its energy balance;
+ * Dirichlet/Neumann faces against mirror-image periodic extensions in 1D/2D/3D; the
+ * CPML update (kappa >= 1, both layers) against a quadrature of the CPML memory
+ * kernel; the log-log rate table against the reference's SiO2_keldysh_1.03um.tf rows;
+ * TFSF replay against the auxiliary-run identity; DFT against brute-force sums.
+ * Parity unpinned: nothing in this file; mismatched-grid TFSF interpolation and the
+ * HDF5 layout live outside it.
  */
 #include <complex.h>
 #include <float.h>
