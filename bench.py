@@ -452,23 +452,83 @@ def bench_served_3d(args, lib):
     return out
 
 
+def spectra_errors(hg, ho, desc):
+    """Relative L2 and the max relative error of every spectral monitor's spectra: per
+    component and point, max over frequencies of |S_gpu - S_ref| over the largest
+    |S_ref| of that spectrum."""
+    from maxwellfdtd_amd.models import MONITOR_COMPONENTS
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    from oracle_util import rel_l2
+    out = {}
+    for i, mon in enumerate(desc.monitors):
+        if mon["kind"] != "spectral":
+            continue
+        worst, a_all, b_all = 0.0, [], []
+        for c in MONITOR_COMPONENTS[desc.ndims]:
+            a = np.atleast_2d(hg.monitor(i, c))
+            b = np.atleast_2d(ho.monitor(i, c))
+            a_all.append(a.ravel())
+            b_all.append(b.ravel())
+            den = np.max(np.abs(b), axis=-1)
+            ok = den > 0
+            if np.any(ok):
+                worst = max(worst, float(np.max(np.max(np.abs(a - b), axis=-1)[ok] / den[ok])))
+        out[f"dft{i}_rel_l2"] = float(f"{rel_l2(np.concatenate(a_all), np.concatenate(b_all)):.3e}")
+        out[f"dft{i}_max_rel"] = float(f"{worst:.3e}")
+    return out
+
+
 def bench_slab_1d(args, lib):
-    from maxwellfdtd_amd.models import Handle
+    """The 1D configuration, run as the reference defines it: FP64 state (MFD_OPT_F64,
+    --f32 for the FP32 state), the whole 8000 steps on the GPU and on the oracle."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    from oracle_util import field_energy, field_errors, oracle, rel_l2
+    from maxwellfdtd_amd import _lib
+    from maxwellfdtd_amd.models import Handle, MONITOR_COMPONENTS
     from maxwellfdtd_amd.workloads import slab_1d
     model, mons = slab_1d()
     desc = model.describe(monitors=mons)
     N = int(np.prod(desc.dims))
+    f64 = not args.f32
     h = Handle(desc, lib)
+    if f64:
+        lib.check(lib.model_set_options(h.h, _lib.MFD_OPT_F64, 1), "set_options(F64)")
+    lib.check(lib.model_upload(h.h), "upload")
     t0 = time.perf_counter()
     h.step(desc.Nt)
     wall = time.perf_counter() - t0
-    h.close()
     cpu = parity = None
     if not args.no_cpu:
-        cpu, parity = cpu_leg(desc, args.cpu_seconds, N, "1D slab", lib)
-    out = base_record(args, N * desc.Nt / wall / 1e9, wall, desc.Nt)
-    out["config"] = {"workload": "slab_1d: Nz=4000, 8000 steps, eps=2.25 slab, point + DFT monitor",
-                     "cells": N}
+        orc = oracle("f64")
+        threads = orc.set_threads(1)
+        ho = Handle(desc, orc)
+        t1 = time.perf_counter()
+        ho.step(desc.Nt)
+        cw = time.perf_counter() - t1
+        cpu = {"value": round(N * desc.Nt / cw / 1e9, 4), "unit": "Gcell-updates/s", "cores": threads,
+               "kind": "port", "sample": f"all {desc.Nt} steps of the same model, FP64, 1 thread "
+                                         "(oracle/fdtd_oracle.c; the 1D update runs on one core)",
+               "host": host_cpu_info()}
+        errs = field_errors(h, ho, 1)
+        for k in range(2):
+            errs[("Sa", "E2")[k]] = rel_l2(h.output(k), ho.output(k))
+        pm = np.concatenate([np.ravel(h.monitor(0, c)) for c in MONITOR_COMPONENTS[1]])
+        po = np.concatenate([np.ravel(ho.monitor(0, c)) for c in MONITOR_COMPONENTS[1]])
+        errs["point_monitor"] = rel_l2(pm, po)
+        ge, gh = h.energy()
+        oe, oh = field_energy(ho)
+        parity = {"steps": desc.Nt, "reference": "FP64 oracle, all steps",
+                  "rel_l2_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
+                  "energy_rel_err": float(f"{abs((ge + gh) / (oe + oh) - 1):.3e}"),
+                  **spectra_errors(h, ho, desc)}
+        log(f"parity over all {desc.Nt} steps: {parity}")
+        ho.close()
+    h.close()
+    out = base_record(args, N * desc.Nt / wall / 1e9, wall, desc.Nt, dtype="f64" if f64 else "f32")
+    out["config"] = {"workload": "slab_1d: Nz=4000, dz=40 nm, CN=0.99, 8000 steps, 10-cell CPML, eps=2.25 "
+                                 "slab on cells 2000-2600, soft 800 nm 10 fs Gaussian at 500, point + "
+                                 "32-wavelength DFT monitor at 3500", "cells": N,
+                     "state": "FP64" if f64 else "FP32"}
     out["roofline"] = None
     out["cpu_baseline"] = cpu
     out["parity"] = parity
@@ -509,6 +569,8 @@ def main():
                     help="served_3d: skip the one-request-at-a-time comparison")
     ap.add_argument("--quantum", type=int, default=64)
     ap.add_argument("--fill", type=int, default=16, help="server: active cells bound, Mi-cells")
+    ap.add_argument("--f32", action="store_true",
+                    help="slab_1d: FP32 state instead of the configuration's FP64")
     ap.add_argument("--sched", type=int, default=1,
                     help="server admission: 0 arrival order, 1 least work first (default 1)")
     args = ap.parse_args()

@@ -14,9 +14,10 @@
  *  - Arrays are column-major with x fastest (Julia layout): cell (ix,iy,iz), 0-based,
  *    has linear index ix + Nx*(iy + Ny*iz).  1D grids use Nx = Ny = 1 (axis z),
  *    2D grids use Ny = 1 (axes x,z), exactly as Field1D/Field2D (src/fields.jl:7-46).
- *  - Field state, coefficients, CPML psi and ADE states are FP32 on the device.
- *    DFT accumulators, the plasma density / rate integration and the TFSF auxiliary
- *    1D grid are FP64.  Time is t[it] = (it-1)*dt in FP64.
+ *  - Field state, coefficients, CPML psi and ADE states are FP32 on the device
+ *    (FP64 for 1D models with MFD_OPT_F64).  DFT accumulators, the plasma density /
+ *    rate integration and the TFSF auxiliary 1D grid are FP64.  Time is
+ *    t[it] = (it-1)*dt in FP64.
  *  - A handle is not thread-safe; distinct handles may be used from distinct threads.
  */
 #ifndef MAXWELL_FDTD_H
@@ -225,7 +226,11 @@ enum {
     MFD_OPT_SCHED = 13,        /* server: admit 0 in arrival order, 1 least work first (1) */
     MFD_OPT_ZCHUNK_H = 14,     /* pipelined 3D H half z planes per tile (8; 0 = ZCHUNK)    */
     MFD_OPT_XCD_H = 15,        /* 3D H tiles dealt to the XCDs like E's (1) or grid order  */
-    MFD_OPT_XCD_BLOCK = 16     /* 3D tiles per block dealt to the XCDs (64; 0: one per XCD) */
+    MFD_OPT_XCD_BLOCK = 16,    /* 3D tiles per block dealt to the XCDs (64; 0: one per XCD) */
+    MFD_OPT_F64 = 17           /* 1D models: 1 keeps fields, CPML, coefficients, ADE /
+                                  plasma state, source waveforms, TFSF tables and Sa/E2
+                                  in FP64 (the reference's CPU{Float64} run; set before
+                                  upload); monitor samples are stored in FP32           */
 };
 /* Changing an option rebuilds the launch plan; field state is kept.                   */
 int32_t mfd_model_set_options(mfd_model *m, int32_t opt, int32_t value);

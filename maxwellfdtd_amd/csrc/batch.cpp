@@ -138,7 +138,7 @@ int32_t mfd_batch_add(mfd_batch *b, mfd_model *h) {
 
 int32_t mfd_batch_set_options(mfd_batch *b, int32_t opt, int32_t value) {
     if (!b) { set_error("null batch"); return MFD_ERR_ARG; }
-    if (opt == MFD_OPT_DFT_BLOCK || opt == MFD_OPT_STAGGER) { set_error("per-model option; set it on the model"); return MFD_ERR_ARG; }
+    if (opt == MFD_OPT_DFT_BLOCK || opt == MFD_OPT_STAGGER || opt == MFD_OPT_F64) { set_error("per-model option; set it on the model"); return MFD_ERR_ARG; }
     int st = set_option(b->opt, nullptr, opt, value);
     if (st != MFD_OK) return st;
     b->ready = false;
@@ -454,7 +454,8 @@ int32_t mfd_server_set_options(mfd_server *s, int32_t opt, int32_t value) {
         s->fill_cells = (long long)value << 20;
         return MFD_OK;
     }
-    if (opt == MFD_OPT_DFT_BLOCK || opt == MFD_OPT_STAGGER || opt == MFD_OPT_KERNEL_TIMING) {
+    if (opt == MFD_OPT_DFT_BLOCK || opt == MFD_OPT_STAGGER || opt == MFD_OPT_KERNEL_TIMING ||
+        opt == MFD_OPT_F64) {
         set_error("not a server option");
         return MFD_ERR_ARG;
     }

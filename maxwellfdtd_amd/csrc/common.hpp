@@ -26,6 +26,10 @@ __device__ __forceinline__ gp<const T> G(const T *p) { return (gp<const T>)p; }
 // square root (v_sqrt_f32, within 1 ulp) rather than the correctly rounded library
 // sequence, which costs about 15 more instructions per cell and step
 __device__ __forceinline__ float sa_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+// square root in the precision of x: the same hardware form for FP32, correctly rounded
+// for FP64
+__device__ __forceinline__ float sqrt_of(float x) { return sa_sqrt(x); }
+__device__ __forceinline__ double sqrt_of(double x) { return sqrt(x); }
 
 // Two-element vectors usable through address-space-1 pointers (HIP's float2/double2
 // classes only bind to generic references).
@@ -74,12 +78,32 @@ struct DevMat {
     const int *cells;           // [cnt]: linear cell index of each state slot (stateful materials)
 };
 
+// FP64 twin of DevMat for models that keep their state in FP64 (1D models with
+// MFD_OPT_F64): the same recursions with double coefficients and double state.
+struct DevMat64 {
+    double Mh, Me, Md1, Md2;
+    int flags, nq;
+    double K2, K3;
+    const double *Aq, *Bq, *Cq;
+    double2 *P;                 // [nq][nc][cnt]: (P^n, P^{n-1})
+    int cnt;
+    int rate_kind, rate_n, pad;
+    double Rava, ksi, Cprho0, Marho0, rate_c, rate_k, rho0;
+    double Ap, Bp;
+    const double *rate_x, *rate_y;
+    double2 *rhod;              // [cnt]: (rho, drho)
+    double2 *Pp;                // [nc][cnt]: (Pp^n, Pp^{n-1})
+    double *Pa;                 // [nc][cnt]
+    double *JE;                 // [cnt]
+};
+
 // Soft/hard point source or TFSF source of one model.
 struct DevSource {
     int kind;                   // SRC_SOFT / SRC_HARD / SRC_TFSF
     int comp;
     int n;
     float coeff;
+    double coeff64;
     const int *cells;           // 0-based linear cell indices [n]
     const float *wave;          // [Nt][n]
     // TFSF: 3D face scheme (1D/2D map onto faces 4,5 / 0,1,4,5, see engine.cpp)
@@ -88,6 +112,7 @@ struct DevSource {
     int aux_n;
     const float *tab[24];       // replay tables per face: H_u, H_v, E_u, E_v
     float cm[3], ce[3], cd[3];  // Cm/d, Ce/d, EPS0*Ce/d per axis (src/sources.jl:725)
+    double cm64[3], ce64[3], cd64[3];   // the same in FP64, for FP64 models
     // auxiliary 1D run (FP64): fields [6][Nz] = Hx, Hy, Dx, Dy, Ex, Ey
     double *aux;
     double *auxpsi;             // [2 faces][4][nbmax]: dEx/dz, dEy/dz, dHx/dz, dHy/dz
@@ -98,6 +123,10 @@ struct DevSource {
     const double *aux_coeff, *aux_wave;   // [aux_n][Nt]
     double Mh0, Me0, Md10, Md20, idz;
     const float *auxtab;        // tabulated aux run [Nt][4][Nz]: Hx, Hy, Ex, Ey (or null)
+    // FP64 models: the waveform table and the replay tables in FP64 (the aux run is
+    // then stepped live, in FP64, instead of tabulated)
+    const double *wave64;
+    const double *tab64[24];
 };
 
 // Field / spectral monitor of one model.
@@ -140,7 +169,47 @@ struct DevModel {
     int ntouched;
     const DevSource *xtfsf;     // TFSF source whose x faces the E kernel finishes, or null
     int dless;                  // plain cells keep no D: D = E/Me is rebuilt on demand
+    // FP64 state (1D models with MFD_OPT_F64): the fields are F64[] (F[] is null), the
+    // material table is mats64 (mats keeps the FP32 table for the id lookups of the
+    // shared kernels), and every CPML layer array (K, A, B, psi) holds doubles
+    int f64;
+    double *F64[9];
+    const DevMat64 *mats64;
+    double idz64;               // 1/dz in FP64
+    double *Sa64, *E2_64;       // the integral variables of an FP64 model (Sa, E2 null)
 };
+
+// The field arrays and material table of a model in precision T (float: F, mats;
+// double: F64, mats64).
+template <class T> struct Prec;
+template <> struct Prec<float> {
+    typedef DevMat Mat;
+    static __device__ __forceinline__ float *Sa(const DevModel &m) { return m.Sa; }
+    static __device__ __forceinline__ float *E2(const DevModel &m) { return m.E2; }
+    static __device__ __forceinline__ float *field(const DevModel &m, int c) { return m.F[c]; }
+    static __device__ __forceinline__ const DevMat *mats(const DevModel &m) { return m.mats; }
+    static __device__ __forceinline__ float idz(const DevModel &m) { return m.idz; }
+    static __device__ __forceinline__ float dt(const DevModel &m) { return m.dtf; }
+};
+template <> struct Prec<double> {
+    typedef DevMat64 Mat;
+    static __device__ __forceinline__ double *Sa(const DevModel &m) { return m.Sa64; }
+    static __device__ __forceinline__ double *E2(const DevModel &m) { return m.E2_64; }
+    static __device__ __forceinline__ double *field(const DevModel &m, int c) { return m.F64[c]; }
+    static __device__ __forceinline__ const DevMat64 *mats(const DevModel &m) { return m.mats64; }
+    static __device__ __forceinline__ double idz(const DevModel &m) { return m.idz64; }
+    static __device__ __forceinline__ double dt(const DevModel &m) { return m.dt; }
+};
+
+// Field access of the kernels shared by FP32 and FP64 models (sources, TFSF faces,
+// monitors, integrals): values cross as FP32 inputs or outputs; additions to the state
+// happen in the model's own precision.
+__device__ __forceinline__ float fld_get(const DevModel &m, int c, long long i) {
+    return m.f64 ? (float)m.F64[c][i] : m.F[c][i];
+}
+__device__ __forceinline__ double fld_get64(const DevModel &m, int c, long long i) {
+    return m.f64 ? m.F64[c][i] : (double)m.F[c][i];
+}
 
 constexpr unsigned char ID_MASK = 0x7f, ID_TOUCHED = 0x80;
 // E-kernel material modes: vacuum only, coefficient table only, full (ADE/Kerr/plasma)
@@ -171,8 +240,10 @@ void launch_h3d_v4(const DevModel *m, const Tile *tiles, int ntiles, int grid, i
 void launch_e3d_v4(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int ty, hipStream_t s);
 void launch_h2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int tw, hipStream_t s);
 void launch_e2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int tw, hipStream_t s);
-void launch_h1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, hipStream_t s);
-void launch_e1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, hipStream_t s);
+void launch_h1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool f64,
+                hipStream_t s);
+void launch_e1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ,
+                bool f64, hipStream_t s);
 void launch_integrals_touched(const DevModel *m, int nmodels, int maxn, int ndims, hipStream_t s);
 void launch_point_sources(const DevModel *m, int nmodels, int slot, int maxn, hipStream_t s);
 void launch_tfsf(const DevModel *m, int nmodels, int slot, int ndims, int maxface, hipStream_t s,

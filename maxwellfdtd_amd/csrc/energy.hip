@@ -24,8 +24,11 @@ constexpr int ENERGY_MAX_BLOCKS = 2048;   // partial sums per call: 2048 x 16 B 
 struct EnergyArgs {
     const float *E[3];
     const float *H[3];
+    const double *E64[3];     // an FP64 model's fields (E, H null)
+    const double *H64[3];
     const unsigned char *id;
     const DevMat *mats;
+    const DevMat64 *mats64;   // an FP64 model's coefficients (else null)
     long long N;
     int nE, nH;           // components present (3D 3/3, 2D E 2 / H 1, 1D 1/1)
     double dt, dV;
@@ -47,12 +50,17 @@ __global__ __launch_bounds__(ENERGY_THREADS) void k_energy_partial(EnergyArgs a,
     double we = 0.0, wh = 0.0;
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < a.N; i += stride) {
-        const DevMat &mt = a.mats[a.id[i] & ID_MASK];
+        const int g = a.id[i] & ID_MASK;
         double e2 = 0.0, h2 = 0.0;
-        for (int c = 0; c < a.nE; c++) { const double v = a.E[c][i]; e2 += v * v; }
-        for (int c = 0; c < a.nH; c++) { const double v = a.H[c][i]; h2 += v * v; }
-        we += (double)mt.iMe * e2;
-        wh += a.dt / (double)mt.Mh * h2;
+        for (int c = 0; c < a.nE; c++) { const double v = a.E[c] ? a.E[c][i] : a.E64[c][i]; e2 += v * v; }
+        for (int c = 0; c < a.nH; c++) { const double v = a.H[c] ? a.H[c][i] : a.H64[c][i]; h2 += v * v; }
+        if (a.mats64) {
+            we += e2 / a.mats64[g].Me;
+            wh += a.dt / a.mats64[g].Mh * h2;
+        } else {
+            we += (double)a.mats[g].iMe * e2;
+            wh += a.dt / (double)a.mats[g].Mh * h2;
+        }
     }
     const double2 s = block_sum2(make_double2(we, wh), sh);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -89,10 +97,12 @@ static EnergyArgs energy_args(const Model &m) {
     } else {
         a.nE = a.nH = 1;
         a.E[0] = d.F[EX]; a.H[0] = d.F[HY];
+        a.E64[0] = d.F64[EX]; a.H64[0] = d.F64[HY];
         a.dV = m.d.dz;
     }
     a.id = d.id;
     a.mats = d.mats;
+    a.mats64 = d.f64 ? d.mats64 : nullptr;
     a.N = m.N;
     a.dt = m.d.dt;
     return a;

@@ -47,7 +47,7 @@ int64_t Model::footprint() const {
     int64_t b = 0;
     int nf = 0;
     for (int c = 0; c < 9; c++) nf += comp_present(c);
-    b += (int64_t)nf * N * 4;
+    b += (int64_t)nf * N * (f64 ? 8 : 4);
     b += N;
     bool stateful = false;
     std::vector<long long> cnt(mats.size() + 1, 0);
@@ -187,22 +187,34 @@ int Model::upload() {
     // bytes past a 256-byte boundary so that equal cell indices of different arrays do
     // not all land on the same HBM channel and bank
     {
-        float **slots[11];
+        // an FP64 model keeps its fields and integral variables in FP64 (8 B per cell)
+        void **slots[11];
+        size_t esz[11];
         int nslot = 0;
         for (int c = 0; c < 9; c++) {
             m.F[c] = nullptr;
-            if (comp_present(c)) slots[nslot++] = &m.F[c];
+            m.F64[c] = nullptr;
+            if (!comp_present(c)) continue;
+            slots[nslot] = f64 ? (void **)&m.F64[c] : (void **)&m.F[c];
+            esz[nslot++] = f64 ? sizeof(double) : sizeof(float);
         }
-        if (int_flags & MFD_INT_SA) slots[nslot++] = &m.Sa;
-        if (int_flags & MFD_INT_E2) slots[nslot++] = &m.E2;
-        const size_t arr = ((size_t)N * sizeof(float) + 255) / 256 * 256;
+        const size_t isz = f64 ? sizeof(double) : sizeof(float);
+        if (int_flags & MFD_INT_SA) { slots[nslot] = f64 ? (void **)&m.Sa64 : (void **)&m.Sa; esz[nslot++] = isz; }
+        if (int_flags & MFD_INT_E2) { slots[nslot] = f64 ? (void **)&m.E2_64 : (void **)&m.E2; esz[nslot++] = isz; }
         const size_t st = ((size_t)std::max(0, stagger) + 255) / 256 * 256;
-        const size_t total = nslot * arr + (size_t)nslot * st;
+        size_t total = 0;
+        for (int k = 0; k < nslot; k++) total += ((size_t)N * esz[k] + 255) / 256 * 256 + st;
         char *pool;
         UP(arena.alloc(&pool, total));
         UP(hipMemsetAsync(pool, 0, total, io));
-        for (int k = 0; k < nslot; k++) *slots[k] = (float *)(pool + k * (arr + st));
+        size_t off = 0;
+        for (int k = 0; k < nslot; k++) {
+            *slots[k] = pool + off;
+            off += ((size_t)N * esz[k] + 255) / 256 * 256 + st;
+        }
     }
+    m.f64 = f64 ? 1 : 0;
+    m.idz64 = 1.0 / d.dz;
     // materials: id per cell (uint8) and compact state slots
     const int nmat = (int)mats.size();
     std::vector<long long> cnt(nmat + 1, 0);
@@ -300,6 +312,49 @@ int Model::upload() {
             UP(hipMemsetAsync(v.JE, 0, c * sizeof(float), io));
         }
     }
+    if (f64) {
+        // the FP64 table: coefficients from the host's FP64 values, state in FP64 (the
+        // FP32 table above keeps the flags and counts; its state arrays are not used)
+        hmats64.assign(nmat + 1, DevMat64{});
+        DevMat64 &v0 = hmats64[0];
+        v0.Mh = d.Mh; v0.Me = d.Me; v0.Md1 = d.Md1; v0.Md2 = d.Md2;
+        for (int k = 0; k < nmat; k++) {
+            const HostMat &hm = mats[k];
+            DevMat64 &v = hmats64[k + 1];
+            const long long c = cnt[k + 1];
+            v.Mh = hm.d.Mh; v.Me = hm.d.Me; v.Md1 = hm.d.Md1; v.Md2 = hm.d.Md2;
+            v.flags = hmats[k + 1].flags;
+            v.nq = hm.d.nq;
+            v.cnt = (int)c;
+            v.K2 = hm.d.Mk2 * hm.d.Me * hm.d.Me;
+            v.K3 = hm.d.Mk3 * hm.d.Me * hm.d.Me * hm.d.Me;
+            if (hm.d.nq > 0) {
+                double *p;
+                UP(upload_vec(arena, &p, hm.Aq, io)); v.Aq = p;
+                UP(upload_vec(arena, &p, hm.Bq, io)); v.Bq = p;
+                UP(upload_vec(arena, &p, hm.Cq, io)); v.Cq = p;
+                UP(arena.alloc(&v.P, (size_t)hm.d.nq * nc * c));
+                UP(hipMemsetAsync(v.P, 0, (size_t)hm.d.nq * nc * c * sizeof(double2), io));
+            }
+            if (hm.d.isplasma) {
+                const DevMat &f = hmats[k + 1];
+                v.Rava = f.Rava; v.ksi = f.ksi; v.rho0 = f.rho0; v.Cprho0 = f.Cprho0;
+                v.Marho0 = f.Marho0; v.Ap = hm.d.Ap; v.Bp = hm.d.Bp;
+                v.rate_kind = f.rate_kind; v.rate_c = f.rate_c; v.rate_k = f.rate_k;
+                v.rate_n = f.rate_n; v.rate_x = f.rate_x; v.rate_y = f.rate_y;
+                v.rhod = f.rhod;
+                UP(arena.alloc(&v.Pp, (size_t)nc * c));
+                UP(hipMemsetAsync(v.Pp, 0, (size_t)nc * c * sizeof(double2), io));
+                UP(arena.alloc(&v.Pa, (size_t)nc * c));
+                UP(hipMemsetAsync(v.Pa, 0, (size_t)nc * c * sizeof(double), io));
+                UP(arena.alloc(&v.JE, c));
+                UP(hipMemsetAsync(v.JE, 0, c * sizeof(double), io));
+            }
+        }
+        DevMat64 *dm64;
+        UP(upload_vec(arena, &dm64, hmats64, io));
+        m.mats64 = dm64;
+    }
     DevMat *dmats;
     UP(upload_vec(arena, &dmats, hmats, io));
     m.mats = dmats;
@@ -312,6 +367,22 @@ int Model::upload() {
         l.present = 1;
         l.ib = L[f].ib - 1;
         l.nb = L[f].nb;
+        const size_t n = psi_len(*this, f, l.nb);
+        float **dst[4] = {&l.psiE0, &l.psiE1, &l.psiH0, &l.psiH1};
+        if (f64) {
+            // an FP64 model's layer arrays hold doubles behind the same pointers
+            double *p;
+            UP(upload_vec(arena, &p, L[f].K, io)); l.K = (const float *)p;
+            UP(upload_vec(arena, &p, L[f].A, io)); l.A = (const float *)p;
+            UP(upload_vec(arena, &p, L[f].B, io)); l.B = (const float *)p;
+            for (int k = 0; k < 4; k++) {
+                double *q;
+                UP(arena.alloc(&q, n));
+                UP(hipMemsetAsync(q, 0, n * sizeof(double), io));
+                *dst[k] = (float *)q;
+            }
+            continue;
+        }
         float *p;
         UP(upload_vec(arena, &p, to_f32(L[f].K), io)); l.K = p;
         std::vector<float> ki(L[f].K.size());
@@ -319,8 +390,6 @@ int Model::upload() {
         UP(upload_vec(arena, &p, ki, io)); l.Ki = p;
         UP(upload_vec(arena, &p, to_f32(L[f].A), io)); l.A = p;
         UP(upload_vec(arena, &p, to_f32(L[f].B), io)); l.B = p;
-        const size_t n = psi_len(*this, f, l.nb);
-        float **dst[4] = {&l.psiE0, &l.psiE1, &l.psiH0, &l.psiH1};
         for (int k = 0; k < 4; k++) {
             UP(arena.alloc(dst[k], n));
             UP(hipMemsetAsync(*dst[k], 0, n * sizeof(float), io));
@@ -343,6 +412,11 @@ int Model::upload() {
             UP(upload_vec(arena, &ip, hs.cells, io)); v.cells = ip;
             float *fp;
             UP(upload_vec(arena, &fp, to_f32(hs.wave), io)); v.wave = fp;
+            v.coeff64 = hs.coeff;
+            if (f64) {
+                double *dp;
+                UP(upload_vec(arena, &dp, hs.wave, io)); v.wave64 = dp;
+            }
             maxsrc_n = std::max(maxsrc_n, v.n);
             continue;
         }
@@ -364,6 +438,9 @@ int Model::upload() {
             v.cm[a] = (float)(Cm / ds[a]);
             v.ce[a] = (float)(Ce / ds[a]);
             v.cd[a] = (float)(MFD_EPS0 * Ce / ds[a]);
+            v.cm64[a] = Cm / ds[a];
+            v.ce64[a] = Ce / ds[a];
+            v.cd64[a] = MFD_EPS0 * Ce / ds[a];
         }
         if (v.mode == 0) {
             int slotmap[24];
@@ -384,6 +461,11 @@ int Model::upload() {
                 float *fp;
                 UP(upload_vec(arena, &fp, to_f32(hs.tables[slotmap[k]]), io));
                 v.tab[k] = fp;
+                if (f64) {
+                    double *dp;
+                    UP(upload_vec(arena, &dp, hs.tables[slotmap[k]], io));
+                    v.tab64[k] = dp;
+                }
             }
         } else {
             const int Nz = d.Nz;
@@ -416,7 +498,8 @@ int Model::upload() {
             v.Mh0 = d.Mh; v.Me0 = d.Me; v.Md10 = d.Md1; v.Md20 = d.Md2;
             v.idz = 1.0 / d.dz;
             const size_t tab_n = (size_t)d.Nt * 4 * Nz;
-            if (tab_n * sizeof(float) <= ((size_t)1 << 30)) {
+            // an FP64 model steps the aux run live instead (its table would be FP32)
+            if (!f64 && tab_n * sizeof(float) <= ((size_t)1 << 30)) {
                 float *tab;
                 UP(arena.alloc(&tab, tab_n));
                 v.auxtab = tab;
@@ -524,10 +607,14 @@ int Model::read_field(int comp, double *out) {
         std::fill(out, out + N, 0.0);
         return MFD_OK;
     }
-    std::vector<float> tmp(N);
     int st = sync();
     if (st == MFD_OK && (comp == DX || comp == DY || comp == DZ)) st = materialize_d();
     if (st != MFD_OK) return st;
+    if (f64) {
+        MFD_HIP(copy_sync(out, dm.F64[comp], N * sizeof(double), hipMemcpyDeviceToHost, io));
+        return MFD_OK;
+    }
+    std::vector<float> tmp(N);
     MFD_HIP(copy_sync(tmp.data(), dm.F[comp], N * sizeof(float), hipMemcpyDeviceToHost, io));
     for (long long i = 0; i < N; i++) out[i] = tmp[i];
     return MFD_OK;
@@ -540,12 +627,16 @@ int Model::write_field(int comp, const double *in) {
     }
     int st = upload();
     if (st != MFD_OK) return st;
-    std::vector<float> tmp(N);
-    for (long long i = 0; i < N; i++) tmp[i] = (float)in[i];
     st = sync();
     if (st == MFD_OK) st = materialize_d();
     if (st != MFD_OK) return st;
     user_set = true;        // a state the caller wrote runs with stored D from now on
+    if (f64) {
+        MFD_HIP(copy_sync(dm.F64[comp], in, N * sizeof(double), hipMemcpyHostToDevice, io));
+        return MFD_OK;
+    }
+    std::vector<float> tmp(N);
+    for (long long i = 0; i < N; i++) tmp[i] = (float)in[i];
     MFD_HIP(copy_sync(dm.F[comp], tmp.data(), N * sizeof(float), hipMemcpyHostToDevice, io));
     return MFD_OK;
 }
@@ -618,9 +709,11 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
         if (kform == 1 && m->d.Nx % 4 != 0) kform = 2;
         if (kform == 2 && m->N >= (1LL << 30)) kform = 0;   // 32-bit cell offsets
     }
+    f64 = ms[0]->f64;
     for (size_t k = 0; k < ms.size(); k++) {
         Model *m = ms[k];
         if (m->d.ndims != ndims) { set_error("a run mixes 1D/2D/3D models"); return MFD_ERR_ARG; }
+        if (m->f64 != f64) { set_error("a run mixes FP32 and FP64 models"); return MFD_ERR_ARG; }
         int st = m->upload();
         if (st != MFD_OK) return st;
         hd.push_back(m->dm);
@@ -795,7 +888,7 @@ int Runner::launch_step(bool timing, bool dft_flush) {
     }
     else if (ndims == 3) launch_h3d(d_models, d_tiles, ntiles_h, grid_h, hmode, stream);
     else if (ndims == 2) launch_h2d(d_models, d_tiles, ntiles_h, grid_h, hmode, opt.ty, stream);
-    else launch_h1d(d_models, d_tiles, ntiles_h, grid_h, hmode, stream);
+    else launch_h1d(d_models, d_tiles, ntiles_h, grid_h, hmode, f64, stream);
     if (timing) { MFD_HIP(hipEventRecord(ev[1], stream)); MFD_HIP(hipEventRecord(ev[2], stream)); }
     if (ndims == 3 && kform == 1) launch_e3d_v4(d_models, d_tiles_e, ntiles, grid, mode, any_int, opt.ty, stream);
     else if (ndims == 3 && kform == 2) {
@@ -804,7 +897,7 @@ int Runner::launch_step(bool timing, bool dft_flush) {
     }
     else if (ndims == 3) launch_e3d(d_models, d_tiles_e, ntiles, grid, mode, any_int, stream);
     else if (ndims == 2) launch_e2d(d_models, d_tiles_e, ntiles, grid, mode, any_int, opt.ty, stream);
-    else launch_e1d(d_models, d_tiles_e, ntiles, grid, mode, any_int, stream);
+    else launch_e1d(d_models, d_tiles_e, ntiles, grid, mode, any_int, f64, stream);
     if (split_state && nchunks) launch_e_state(d_models, d_chunks, nchunks, ndims, any_int, stream);
     if (timing) MFD_HIP(hipEventRecord(ev[3], stream));
     if (tfsf_gather) {
@@ -997,6 +1090,13 @@ int set_option(RunOptions &o, Model *m, int opt, int value) {
         case MFD_OPT_XCD_BLOCK:
             if (value < 0) { set_error("XCD block must be >= 0"); return MFD_ERR_ARG; }
             o.xcd_block = value;
+            break;
+        case MFD_OPT_F64:
+            if (!m || m->uploaded || m->d.ndims != 1 || (value != 0 && value != 1)) {
+                set_error("FP64 state: 1D models only, 0 or 1, set before upload");
+                return MFD_ERR_ARG;
+            }
+            m->f64 = value != 0;
             break;
         case MFD_OPT_STAGGER:
             if (!m || m->uploaded || value < 0) { set_error("stagger must be >= 0, set on a model before upload"); return MFD_ERR_ARG; }
@@ -1390,6 +1490,12 @@ int32_t mfd_model_get_output(mfd_model *h, int32_t which, double *out) {
     if (!m.uploaded) { std::fill(out, out + N, 0.0); return MFD_OK; }
     int st = m.sync();
     if (st != MFD_OK) return st;
+    if ((which == MFD_OUT_SA || which == MFD_OUT_E2) && m.f64) {
+        const double *src = which == MFD_OUT_SA ? m.dm.Sa64 : m.dm.E2_64;
+        if (!src) { set_error("integral not enabled"); return MFD_ERR_STATE; }
+        MFD_HIP(copy_sync(out, src, N * sizeof(double), hipMemcpyDeviceToHost, m.io));
+        return MFD_OK;
+    }
     if (which == MFD_OUT_SA || which == MFD_OUT_E2) {
         const float *src = which == MFD_OUT_SA ? m.dm.Sa : m.dm.E2;
         if (!src) { set_error("integral not enabled"); return MFD_ERR_STATE; }
@@ -1406,7 +1512,10 @@ int32_t mfd_model_get_output(mfd_model *h, int32_t which, double *out) {
         const int cnt = v.cnt;
         if (cnt == 0) continue;
         std::vector<double> vals(cnt);
-        if (which == MFD_OUT_JE) {
+        if (which == MFD_OUT_JE && m.f64) {
+            MFD_HIP(copy_sync(vals.data(), m.hmats64[k + 1].JE, cnt * sizeof(double),
+                              hipMemcpyDeviceToHost, m.io));
+        } else if (which == MFD_OUT_JE) {
             std::vector<float> tmp(cnt);
             MFD_HIP(copy_sync(tmp.data(), v.JE, cnt * sizeof(float), hipMemcpyDeviceToHost, m.io));
             for (int i = 0; i < cnt; i++) vals[i] = tmp[i];

@@ -98,6 +98,8 @@ struct Model {
     int dft_done = 0;           // steps already contracted into the DFT accumulators
     int dft_block = 32;
     int stagger = 0;            // byte offset added per per-cell array in the field pool
+    bool f64 = false;           // 1D: state kept in FP64 (MFD_OPT_F64)
+    std::vector<DevMat64> hmats64;
     int ncomp_mon = 0;
     int maxsrc_n = 0, maxmon_n = 0, maxface = 0;
     std::vector<int> touched;   // cells written by sources (sorted)
@@ -157,6 +159,7 @@ struct Runner {
     std::vector<Model *> dless_models;   // models this runner steps without stored D
     int maxtouched = 1;
     int kform = 0;              // 3D kernel form in use
+    bool f64 = false;           // the models keep their state in FP64 (1D)
     bool split_state = false;   // stateful cells' E computed by the state pass
     bool tfsf_gather = false;   // TFSF corrections + touched integrals in one launch
     bool aux_needed = false;    // some TFSF aux run is stepped on the device each step

@@ -35,27 +35,33 @@ __device__ inline double ionrate(MR &mt, double I) {
     return 0.0;
 }
 
+template <class T> struct vec2;
+template <> struct vec2<float> { typedef v2f type; };
+template <> struct vec2<double> { typedef v2d type; };
+
 // d[c]: updated D; e[c]: on entry E^n (read only when the material has ADE/plasma
-// state), on exit E^{n+1}.  s is the cell's state slot within its material.  mt may be a
-// generic or a global-address-space reference.
-template <int NC, class MR>
-__device__ __forceinline__ void material_e(MR &mt, int s, const float (&d)[NC],
-                                           float (&e)[NC], double dt, float dtf) {
+// state), on exit E^{n+1}.  s is the cell's state slot within its material.  mt is a
+// DevMat (T = float) or a DevMat64 (T = double), generic or global-address-space; dtT
+// is dt in T.  The plasma density and rate are FP64 either way.
+template <int NC, class T, class MR>
+__device__ __forceinline__ void material_e(MR &mt, int s, const T (&d)[NC], T (&e)[NC],
+                                           double dt, T dtT) {
+    typedef typename vec2<T>::type V2;
     const int flags = mt.flags;
-    float sumP[NC];
+    T sumP[NC];
 #pragma unroll
-    for (int c = 0; c < NC; c++) sumP[c] = 0.0f;
+    for (int c = 0; c < NC; c++) sumP[c] = (T)0;
     if (flags & MAT_DISP) {
         const int nq = mt.nq, cnt = mt.cnt;
         for (int q = 0; q < nq; q++) {
-            const float A = G(mt.Aq)[q], B = G(mt.Bq)[q], Cc = G(mt.Cq)[q];
+            const T A = (T)G(mt.Aq)[q], B = (T)G(mt.Bq)[q], Cc = (T)G(mt.Cq)[q];
 #pragma unroll
             for (int c = 0; c < NC; c++) {
                 // pole state is read and written once per step: streamed past the caches
-                gp<v2f> pp = (gp<v2f>)(mt.P + (size_t)(q * NC + c) * cnt + s);
-                const v2f p = __builtin_nontemporal_load(pp);
-                const float pn = A * p.x + B * p.y + Cc * e[c];
-                v2f np;
+                gp<V2> pp = (gp<V2>)(mt.P + (size_t)(q * NC + c) * cnt + s);
+                const V2 p = __builtin_nontemporal_load(pp);
+                const T pn = A * p.x + B * p.y + Cc * e[c];
+                V2 np;
                 np.x = pn;
                 np.y = p.x;
                 __builtin_nontemporal_store(np, pp);
@@ -71,26 +77,26 @@ __device__ __forceinline__ void material_e(MR &mt, int s, const float (&d)[NC],
         const double II = mt.ksi * E2;
         gp<v2d> prd = (gp<v2d>)(mt.rhod + s);
         v2d rd = __builtin_nontemporal_load(prd);
-        const float cp = (float)(mt.Cprho0 * rd.x);
-        float J[NC];
+        const T cp = (T)(mt.Cprho0 * rd.x);
+        T J[NC];
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            gp<v2f> pp = (gp<v2f>)(mt.Pp + (size_t)c * cnt + s);
-            const v2f p = __builtin_nontemporal_load(pp);
-            const float pn = mt.Ap * p.x + mt.Bp * p.y + cp * e[c];
-            v2f np;
+            gp<V2> pp = (gp<V2>)(mt.Pp + (size_t)c * cnt + s);
+            const V2 p = __builtin_nontemporal_load(pp);
+            const T pn = (T)mt.Ap * p.x + (T)mt.Bp * p.y + cp * e[c];
+            V2 np;
             np.x = pn;
             np.y = p.x;
             __builtin_nontemporal_store(np, pp);
             sumP[c] += pn;
-            J[c] = (pn - p.y) / (2.0f * dtf);
+            J[c] = (pn - p.y) / ((T)2 * dtT);
         }
         const double invE2 = (E2 >= DBL_EPS_) ? 1.0 / E2 : 0.0;
-        const float ca = (float)(mt.Marho0 * rd.y * invE2);
+        const T ca = (T)(mt.Marho0 * rd.y * invE2);
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            gp<float> pa = G(mt.Pa) + (size_t)c * cnt + s;
-            const float v = __builtin_nontemporal_load(pa) + ca * e[c];
+            gp<T> pa = G(mt.Pa) + (size_t)c * cnt + s;
+            const T v = __builtin_nontemporal_load(pa) + ca * e[c];
             __builtin_nontemporal_store(v, pa);
             sumP[c] += v;
         }
@@ -106,19 +112,19 @@ __device__ __forceinline__ void material_e(MR &mt, int s, const float (&d)[NC],
         rd.x = rho;
         rd.y = R1 * (1.0 - rho);
         __builtin_nontemporal_store(rd, prd);
-        float je = 0.0f;
+        T je = (T)0;
 #pragma unroll
         for (int c = 0; c < NC; c++) je += J[c] * e[c];
-        __builtin_nontemporal_store(__builtin_nontemporal_load(&G(mt.JE)[s]) + je * dtf, &G(mt.JE)[s]);
+        __builtin_nontemporal_store(__builtin_nontemporal_load(&G(mt.JE)[s]) + je * dtT, &G(mt.JE)[s]);
     }
-    const float Me = mt.Me;
+    const T Me = (T)mt.Me;
     if (flags & MAT_KERR) {
-        const float K2 = mt.K2, K3 = mt.K3;
+        const T K2 = (T)mt.K2, K3 = (T)mt.K3;
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            const float u = d[c] - sumP[c];
-            e[c] = (1.0f + K2 * u + 2.0f * K3 * (u * u)) /
-                   (1.0f + 2.0f * K2 * u + 3.0f * K3 * (u * u)) * u * Me;
+            const T u = d[c] - sumP[c];
+            e[c] = ((T)1 + K2 * u + (T)2 * K3 * (u * u)) /
+                   ((T)1 + (T)2 * K2 * u + (T)3 * K3 * (u * u)) * u * Me;
         }
     } else {
 #pragma unroll
@@ -127,7 +133,8 @@ __device__ __forceinline__ void material_e(MR &mt, int s, const float (&d)[NC],
 }
 
 // Non-finite guard on Ex (src/models.jl:202-209): one atomic per wave that saw one.
-__device__ __forceinline__ void check_finite(float ex, int *flag, bool active) {
+template <class T>
+__device__ __forceinline__ void check_finite(T ex, int *flag, bool active) {
     const bool bad = active && !isfinite(ex);
     if (__any(bad)) {
         if (bad) atomicOr(flag, isnan(ex) ? 1 : 2);

@@ -24,11 +24,19 @@ __global__ __launch_bounds__(256) void k_point_sources(const DevModel *__restric
     bool wroteEx = false;
     float v = 0.0f;
     if (k < S.n) {
-        float *F = m.F[S.comp];
         const int c = S.cells[k];
         const float w = S.wave[(size_t)it0 * S.n + k];
-        v = (S.kind == SRC_SOFT) ? F[c] + S.coeff * w : w;
-        F[c] = v;
+        if (m.f64) {            // the state is FP64: the waveform and the addition too
+            double *F = m.F64[S.comp];
+            const double w64 = S.wave64[(size_t)it0 * S.n + k];
+            const double v64 = (S.kind == SRC_SOFT) ? F[c] + S.coeff64 * w64 : w64;
+            F[c] = v64;
+            v = (float)v64;
+        } else {
+            float *F = m.F[S.comp];
+            v = (S.kind == SRC_SOFT) ? F[c] + S.coeff * w : w;
+            F[c] = v;
+        }
         wroteEx = S.comp == EX;
     }
     check_finite(v, m.flag, wroteEx);
@@ -224,6 +232,41 @@ __global__ __launch_bounds__(256) void k_tfsf_axis(const DevModel *__restrict__ 
     // In the auxiliary (normal-incidence) mode the Hz and Ez incident fields are
     // identically zero; their corrections add exact zeros and are skipped.
     auto live = [&](int k) { return S.mode == 0 || (tabc[k] != HZ && tabc[k] != EZ); };
+    // an FP64 model: incident values and coefficients in FP64, added to the FP64 state
+    auto inc64 = [&](int k) -> double {
+        if (S.mode == 1) {
+            int z;
+            if (axis == 2) z = side ? N2 : (k < 2 ? N1 - 1 : N1);
+            else z = v;
+            const int comp = tabc[k];
+            const int slot = comp == HX ? 0 : comp == HY ? 1 : comp == EX ? 4 : comp == EY ? 5 : -1;
+            return slot < 0 ? 0.0 : S.aux[(size_t)slot * m.Nz + z];
+        }
+        const double *T = S.tab64[4 * f + k];
+        if (!T) return 0.0;
+        const int na = (k == 0 || k == 3) ? Nu + 1 : Nu;
+        const int nb = (k == 0 || k == 3) ? Nv : Nv + 1;
+        return T[(u - U1) + (size_t)na * ((v - V1) + (size_t)nb * it0)];
+    };
+    if (m.f64) {
+        const double cm64 = S.cm64[axis], ce64 = S.ce64[axis], cd64 = S.cd64[axis];
+        const double s064 = s0;
+        if (u <= U2 && v <= V2 - 1 && live(3)) m.F64[cHu][cell(nH)] += (s064 * cm64) * inc64(3);
+        if (u <= U2 - 1 && v <= V2 && live(2)) m.F64[cHv][cell(nH)] += (-s064 * cm64) * inc64(2);
+        if (u <= U2 - 1 && v <= V2 && live(1)) {
+            const double iv = inc64(1);
+            const long long c = cell(nE);
+            m.F64[cEu][c] += (-s064 * ce64) * iv;
+            m.F64[cDu][c] += (-s064 * cd64) * iv;
+        }
+        if (u <= U2 && v <= V2 - 1 && live(0)) {
+            const double iv = inc64(0);
+            const long long c = cell(nE);
+            m.F64[cEv][c] += (s064 * ce64) * iv;
+            m.F64[cDv][c] += (s064 * cd64) * iv;
+        }
+        return;
+    }
     if (u <= U2 && v <= V2 - 1 && live(3)) {     // H_u += (s0) Cm/d * inc E_v
         const float iv = inc(3);
         float *F = m.F[cHu];
@@ -405,21 +448,20 @@ __global__ __launch_bounds__(256) void k_monitors(const DevModel *__restrict__ m
         if (!M.spectral && M.issum) {
             if (blockIdx.x != 0) continue;
             for (int c = 0; c < M.ncomp; c++) {
-                const float *F = m.F[M.comps[c]];
                 double acc = 0.0;
-                for (int k = threadIdx.x; k < M.n; k += blockDim.x) acc += (double)F[M.cells[k]];
+                for (int k = threadIdx.x; k < M.n; k += blockDim.x) acc += fld_get64(m, M.comps[c], M.cells[k]);
                 const double tot = block_sum(acc, sh);
                 if (threadIdx.x == 0) M.sum[(size_t)it0 * M.ncomp + c] = tot;
             }
         } else if (!M.spectral) {
             for (int k = gid; k < M.n; k += stride)
                 for (int c = 0; c < M.ncomp; c++)
-                    M.rec[((size_t)it0 * M.ncomp + c) * M.n + k] = m.F[M.comps[c]][M.cells[k]];
+                    M.rec[((size_t)it0 * M.ncomp + c) * M.n + k] = fld_get(m, M.comps[c], M.cells[k]);
         } else {
             const int kk = it0 % m.dft_block;
             for (int k = gid; k < M.n; k += stride)
                 for (int c = 0; c < M.ncomp; c++)
-                    M.samp[((size_t)kk * M.ncomp + c) * M.n + k] = m.F[M.comps[c]][M.cells[k]];
+                    M.samp[((size_t)kk * M.ncomp + c) * M.n + k] = fld_get(m, M.comps[c], M.cells[k]);
         }
     }
 }
@@ -571,6 +613,13 @@ __global__ __launch_bounds__(256) void k_integrals_touched(const DevModel *__res
                 Sa[i] += sa_sqrt(a * a + b * b) * dt;
             }
             if (E2) E2[i] += (ex * ex + ez * ez) * dt;
+        } else if (m.f64) {
+            const double hy = m.F64[HY][i], ex = m.F64[EX][i];
+            if (m.Sa64) {
+                const double a = ex * hy;
+                m.Sa64[i] += sqrt(a * a) * m.dt;
+            }
+            if (m.E2_64) m.E2_64[i] += ex * ex * m.dt;
         } else {
             const float hy = m.F[HY][i], ex = m.F[EX][i];
             if (Sa) {

@@ -316,10 +316,13 @@ class Handle:
 
 class MI355X:
     """The MI355X backend: FP32 fields on one gfx950 GPU (the stand-in for GPU{T},
-    src/backends.jl:13-23)."""
+    src/backends.jl:13-23); MI355X(precision="f64") keeps a 1D model's state in FP64."""
+
+    def __init__(self, precision="f32"):
+        self.precision = precision
 
     def __repr__(self):
-        return "MI355X()"
+        return f"MI355X(precision={self.precision!r})"
 
 
 @dataclass
@@ -371,17 +374,25 @@ def _shape(grid, flat):
 
 
 def solve(model, *, backend=None, fname=None, nframes=None, nstride=None, dtout=None,
-          components=None, monitors=None, integrals=None, chunk=256):
+          components=None, monitors=None, integrals=None, chunk=256, precision=None):
     """solve!(model; backend, nframes, nstride, dtout, components, monitors)
     (src/models.jl:166-242) on the MI355X engine.  Field frames are kept in memory
-    instead of being written to an HDF5 file."""
+    instead of being written to an HDF5 file.  precision="f64" (1D models) keeps the
+    state in FP64, as the reference's CPU{Float64} run; the default is the FP32 state of
+    its GPU{Float32} run (MI355X(precision=...) sets the same)."""
     if backend is not None and not isinstance(backend, MI355X):
         raise ValueError("this package runs on the MI355X backend only")
+    if precision is None:
+        precision = getattr(backend, "precision", "f32")
+    if precision not in ("f32", "f64"):
+        raise ValueError("precision must be 'f32' or 'f64'")
     lib = _lib.engine()
     if isinstance(monitors, (FieldMonitor, SpectralMonitor)):
         monitors = [monitors]
     desc = model.describe(monitors=monitors, integrals=integrals)
     h = Handle(desc, lib)
+    if precision == "f64":
+        lib.check(lib.model_set_options(h.h, _lib.MFD_OPT_F64, 1), "set_options(F64)")
     grid = model.grid
     names = MONITOR_COMPONENTS[grid.ndims]
     comps = tuple(components) if components is not None else names

@@ -277,6 +277,32 @@ def nonlinear_2d(nx=96, nz=80, steps=180, I0=1e17):
     return Model(grid, srcs, tmax=(steps - 1) * dt, CN=0.99, pml=5 * d, material=mats)
 
 
+def nonlinear_1d(nz=500, steps=700, I0=1e17):
+    """1D: a Kerr + plasma slab and a Debye slab under an intense soft-source pulse,
+    CPML at both ends, point and spectral monitors."""
+    d = 20e-9
+    grid = Grid(zmin=0, zmax=(nz - 1) * d, Nz=nz)
+    L = (nz - 1) * d
+    n0, n2 = 1.45, 2.5e-20
+    chi3 = 4 / 3 * n0 ** 2 * EPS0 * C0 * n2
+    w0 = 2 * np.pi * C0 / 800e-9
+    plasma = Plasma(ionrate=PowerLawRate(sigma=1e-90, K=6), rho0=2.1e28, nuc=1e15, frequency=w0,
+                    Uiev=9.0, mr=0.64)
+    mats = [Material(geometry=lambda z: (z > 0.4 * L) & (z < 0.6 * L), eps=n0 ** 2, chi3=chi3,
+                     plasma=plasma),
+            Material(geometry=lambda z: (z > 0.7 * L) & (z < 0.8 * L), eps=1.5,
+                     chi=DebyeSusceptibility(deps=1.0, tau=2e-15))]
+    E0 = np.sqrt(I0 / (n0 * EPS0 * C0 / 2))
+    w = lambda t: E0 * np.exp(-(t - 12e-15) ** 2 / (4e-15) ** 2) * np.cos(w0 * (t - 12e-15))
+    src = SoftSource(geometry=lambda z: np.abs(z - 0.2 * L) < d / 2, waveform=lambda z, t: w(t) + 0 * z,
+                     component="Ex")
+    dt = 0.99 / C0 * d
+    model = Model(grid, src, tmax=(steps - 1) * dt, CN=0.99, pml=10 * d, material=mats)
+    mons = [FieldMonitor((0.9 * L,)),
+            SpectralMonitor((0.9 * L,), w=w0 * np.linspace(0.7, 1.3, 16))]
+    return model, mons
+
+
 def slab_1d(nz=600, steps=900, dispersive=True, kmax=None):
     """1D: CPML, a Lorentz + conductive slab and a soft source, plus monitors."""
     d = 40e-9
