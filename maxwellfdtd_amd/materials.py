@@ -133,11 +133,28 @@ class IonizationRate:
         return out
 
 
+RATE_TABLE_LOG10_RANGE = (10.0, 20.0)    # intensities 1e10 .. 1e20 W/m^2
+RATE_TABLE_POINTS = 2001
+
+
+def tabulate_rate(fn, lo=RATE_TABLE_LOG10_RANGE[0], hi=RATE_TABLE_LOG10_RANGE[1],
+                  n=RATE_TABLE_POINTS):
+    """An arbitrary ionization-rate function R(I) as the device evaluates it: an
+    IonizationRate table of n points evenly spaced in log10(I) over [lo, hi], linear in
+    log-log between them (IonizationRate.jl:30-52).  Rates that are zero or negative at
+    a node are floored at 1e-300 (log10 = -300), which the device turns into 0."""
+    x = np.linspace(lo, hi, n)
+    vals = np.array([float(fn(10.0 ** v)) for v in x])
+    y = np.log10(np.maximum(vals, 1e-300))
+    return IonizationRate(x=x, y=y)
+
+
 @dataclass
 class Plasma:
     """Plasma(; ionrate, rho0, nuc, frequency, Uiev, mr=1) (src/materials.jl:48-61).
-    `ionrate` must be a PowerLawRate or an IonizationRate: an arbitrary Python callable
-    cannot be evaluated inside the device kernels."""
+    `ionrate` is a PowerLawRate (evaluated exactly on the device), an IonizationRate
+    table, or any function of the intensity, which is tabulated with tabulate_rate
+    (2001 log-log points over 1e10-1e20 W/m^2) as the Julia backend shim does."""
     ionrate: Any
     rho0: float
     nuc: float
@@ -211,9 +228,12 @@ def material_coefficients(material, dt):
         elif isinstance(r, IonizationRate):
             rate = dict(rate_kind=2, rate_c=0.0, rate_k=0.0,
                         rate_x=np.asarray(r.x, float), rate_y=np.asarray(r.y, float))
+        elif callable(r):
+            t = tabulate_rate(r)
+            rate = dict(rate_kind=2, rate_c=0.0, rate_k=0.0, rate_x=t.x, rate_y=t.y)
         else:
-            raise TypeError("Plasma.ionrate must be a PowerLawRate or an IonizationRate; "
-                            "a Python callable cannot run inside the device kernels")
+            raise TypeError("Plasma.ionrate must be a PowerLawRate, an IonizationRate or a "
+                            "function of the intensity")
         c.update(isplasma=1, Rava=Rava, ksi=ksi, rho0=float(pl.rho0), Ap=Ap, Bp=Bp, Cp=Cp,
                  Ma=Ma, **rate)
     return c

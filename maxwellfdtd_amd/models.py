@@ -374,33 +374,60 @@ def _shape(grid, flat):
 
 
 def solve(model, *, backend=None, fname=None, nframes=None, nstride=None, dtout=None,
-          components=None, monitors=None, integrals=None, chunk=256, precision=None):
-    """solve!(model; backend, nframes, nstride, dtout, components, monitors)
-    (src/models.jl:166-242) on the MI355X engine.  Field frames are kept in memory
-    instead of being written to an HDF5 file.  precision="f64" (1D models) keeps the
-    state in FP64, as the reference's CPU{Float64} run; the default is the FP32 state of
-    its GPU{Float32} run (MI355X(precision=...) sets the same)."""
+          components=None, monitors=None, integrals=None, chunk=256, precision=None,
+          tfsf_record=False, tfsf_box=None, tfsf_fname=None, lib=None):
+    """solve!(model; backend, fname, nframes, nstride, dtout, components, monitors,
+    tfsf_record, tfsf_box, tfsf_fname) (src/models.jl:166-242) on the MI355X engine.
+
+    Field frames at the output times are returned in memory, and written to `fname` in
+    the reference's HDF5 layout when it is given (then they are not kept in memory).
+    tfsf_record=True records the incident field on the faces of tfsf_box (face
+    coordinates) and writes the record file tfsf_fname (default: fname with "_tfsf"
+    before the extension) that TFSFSource(fname=...) replays.  precision="f64" (1D
+    models) keeps the state in FP64, as the reference's CPU{Float64} run; the default
+    is the FP32 state of its GPU{Float32} run (MI355X(precision=...) sets the same).
+    `lib` is the ABI implementation to run on (default: the HIP engine)."""
+    import os
+    from . import outputs as _out
+    from . import tfsf_files as _tf
     if backend is not None and not isinstance(backend, MI355X):
         raise ValueError("this package runs on the MI355X backend only")
     if precision is None:
         precision = getattr(backend, "precision", "f32")
     if precision not in ("f32", "f64"):
         raise ValueError("precision must be 'f32' or 'f64'")
-    lib = _lib.engine()
+    lib = _lib.engine() if lib is None else lib
     if isinstance(monitors, (FieldMonitor, SpectralMonitor)):
         monitors = [monitors]
     desc = model.describe(monitors=monitors, integrals=integrals)
+    nuser = len(desc.monitors)
+    grid = model.grid
+    if tfsf_record:
+        if tfsf_box is None:
+            raise ValueError("tfsf_record needs tfsf_box")
+        if tfsf_fname is None:
+            base = fname if fname is not None else "out.hdf"
+            root, ext = os.path.splitext(base)
+            tfsf_fname = root + "_tfsf" + ext
+        desc.monitors = desc.monitors + _tf.record_monitors(grid, tfsf_box)
     h = Handle(desc, lib)
     if precision == "f64":
         lib.check(lib.model_set_options(h.h, _lib.MFD_OPT_F64, 1), "set_options(F64)")
-    grid = model.grid
     names = MONITOR_COMPONENTS[grid.ndims]
-    comps = tuple(components) if components is not None else names
     isfields = not (nstride is None and nframes is None and dtout is None)
+    comps = _out.check_components(grid.ndims, components) if isfields else ()
     tout = output_times(model.t, nstride, nframes, dtout) if isfields else np.zeros(0)
-    frames = {c: [] for c in comps} if isfields else {}
-    if model.isplasma and isfields:
+    keep = fname is None
+    frames = {c: [] for c in comps} if isfields and keep else {}
+    if model.isplasma and isfields and keep:
         frames["rho"] = []
+    ofile = None
+    if fname is not None:
+        d = os.path.dirname(fname)
+        if d:
+            os.makedirs(d, exist_ok=True)
+        ofile = _out.OutputFile(fname, model, tout if isfields else None, comps,
+                                np.float64 if precision == "f64" else np.float32)
     dt = model.dt
     try:
         itout = 0
@@ -415,25 +442,43 @@ def solve(model, *, backend=None, fname=None, nframes=None, nstride=None, dtout=
             h.step(n)
             it += n
             if isfields and itout < len(tout) and abs(model.t[it - 1] - tout[itout]) <= dt / 2:
-                for c in comps:
-                    frames[c].append(_shape(grid, h.field(c)))
+                fr = {c: _shape(grid, h.field(c)) for c in comps}
                 if model.isplasma:
-                    frames["rho"].append(_shape(grid, h.output(_lib_rho())))
+                    fr["rho"] = _shape(grid, h.output(_lib.MFD_OUT_RHO))
+                if ofile is not None:
+                    ofile.write_fields(itout, fr)
+                else:
+                    for c, v in fr.items():
+                        frames[c].append(v)
                 itout += 1
         data = {c: _shape(grid, h.field(c)) for c in FIELD_COMPONENTS[grid.ndims]}
         mons = []
-        for i, mon in enumerate(desc.monitors):
-            entry = dict(kind=mon["kind"], issum=mon["issum"], inds=mon["cells"])
+        for i, mon in enumerate(desc.monitors[:nuser]):
+            entry = dict(kind=mon["kind"], issum=mon["issum"], inds=mon["cells"],
+                         inds_1based=_out.cartesian_inds(grid, mon["cells"]))
             if mon["kind"] == "spectral":
                 entry["w"] = mon["w"]
             for c in names:
                 entry[c] = h.monitor(i, c)
             mons.append(entry)
+        if tfsf_record:
+            tables = {}
+            for i, mon in enumerate(desc.monitors[nuser:], start=nuser):
+                name, comp, shape = mon["tfsf"]
+                rec = h.monitor(i, comp)                       # (n, Nt), x fastest
+                tables[name] = np.reshape(rec, tuple(shape) + (model.Nt,), order="F")
+            _tf.write_record_file(tfsf_fname, grid, model.t, tfsf_box, tables)
         Sa = _shape(grid, h.output(0)) if desc.integrals & 1 else None
         E2 = _shape(grid, h.output(1)) if desc.integrals & 2 else None
         JE = _shape(grid, h.output(2)) if model.isplasma else None
         rho = _shape(grid, h.output(3)) if model.isplasma else None
+        if ofile is not None:
+            ofile.finish(Sa if Sa is not None else np.zeros(grid.shape),
+                         E2 if E2 is not None else np.zeros(grid.shape), JE, rho, mons)
+            ofile = None
     finally:
+        if ofile is not None:
+            ofile.close()
         h.close()
     return Solution(model, Fields(grid, data), mons, Sa, E2, JE, rho, frames, tout)
 

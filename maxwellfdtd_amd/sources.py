@@ -66,8 +66,11 @@ class TFSFSource(Source):
         produced by the x/y-uniform auxiliary run of the record pass, i.e. a soft
         `component` source on the plane nearest z0 with waveform(t[, p]), computed on the
         device in FP64 every step.
-    Reading a reference HDF5 record file (`TFSFSource(fname)`) needs an HDF5 reader,
-    which this package does not include.
+      - `fname`: a record file as solve(...; tfsf_record=True) or the reference's
+        prepare_tfsf_record/write_tfsf_record write it (HDF5: box, t, x, y, z, inc*),
+        resampled onto the model's grid and time axis with linear interpolation and
+        flat extrapolation (src/sources.jl:264-317,387-466,592-710); `box` then comes
+        from the file.
     """
     box: Sequence[float] = ()
     tables: Optional[Sequence[np.ndarray]] = None
@@ -78,11 +81,6 @@ class TFSFSource(Source):
     explicit: bool = False
     fname: Optional[str] = None
 
-    def __post_init__(self):
-        if self.fname is not None:
-            raise NotImplementedError(
-                "TFSF record files are HDF5; pass `tables` (same-grid replay arrays) or a "
-                "plane-wave `waveform` instead")
 
 
 def eval_waveform(fn, p, grid, cells, t):
@@ -139,7 +137,13 @@ def tfsf_struct(src, grid, t):
     """SourceStruct(::TFSFSource, field, t) (src/sources.jl:284-317,408-466,592-710):
     1-based box indices and the incident-field description."""
     x, y, z = coords(grid)
-    b = list(src.box)
+    file_tables = None
+    if src.fname is not None:
+        from .tfsf_files import replay_tables
+        fbox, file_tables = replay_tables(src.fname, grid, t)
+        b = list(fbox)
+    else:
+        b = list(src.box)
     box = [1, 1, 1, 1, 1, 1]
     if grid.ndims == 1:
         box[4], box[5] = _nearest(z, b[0]), _nearest(z, b[1])
@@ -159,6 +163,10 @@ def tfsf_struct(src, grid, t):
     for a, bb, n in need:
         if not (2 <= a < bb <= n):
             raise ValueError("TFSF box must lie strictly inside the grid: 2 <= i1 < i2 <= N")
+    if file_tables is not None:
+        return dict(box=box, mode=0,
+                    tables=[np.ascontiguousarray(np.asarray(tb, dtype=float).ravel(order="F"))
+                            for tb in file_tables])
     if src.tables is not None:
         tabs = [np.asarray(tb, dtype=float) for tb in src.tables]
         need_n = {1: 4, 2: 8, 3: 24}[grid.ndims]

@@ -169,9 +169,10 @@ def test_source_without_cells():
         Model(g, src, tmax=1e-15)
 
 
-def test_tfsf_file_not_supported():
-    with pytest.raises(NotImplementedError):
-        TFSFSource(box=(0, 1), fname="record.h5")
+def test_tfsf_file_missing(tmp_path):
+    g = Grid(zmin=0, zmax=1e-6, Nz=11)
+    with pytest.raises(FileNotFoundError):
+        Model(g, TFSFSource(fname=str(tmp_path / "record.h5")), tmax=1e-15)
 
 
 # ---- geometry (src/util.jl:1-100) ------------------------------------------------------
