@@ -412,7 +412,12 @@ def solve(model, *, backend=None, fname=None, nframes=None, nstride=None, dtout=
         desc.monitors = desc.monitors + _tf.record_monitors(grid, tfsf_box)
     h = Handle(desc, lib)
     if precision == "f64":
-        lib.check(lib.model_set_options(h.h, _lib.MFD_OPT_F64, 1), "set_options(F64)")
+        if hasattr(lib, "model_set_options"):
+            lib.check(lib.model_set_options(h.h, _lib.MFD_OPT_F64, 1), "set_options(F64)")
+        elif not (hasattr(lib, "real_size") and lib.real_size() == 8):
+            # an implementation without options computes in one fixed precision
+            h.close()
+            raise ValueError(f"{lib.path} has no FP64 state")
     names = MONITOR_COMPONENTS[grid.ndims]
     isfields = not (nstride is None and nframes is None and dtout is None)
     comps = _out.check_components(grid.ndims, components) if isfields else ()

@@ -6,6 +6,8 @@ The second group are small seeded models that switch on every feature of the per
 update (CPML on all faces, materials, ADE poles, Kerr, plasma, TFSF, monitors,
 integrals, every boundary condition) for parity checks of the engine against the oracle.
 """
+import math
+
 import numpy as np
 
 from maxwellfdtd_amd import (C0, EPS0, CPML, DebyeSusceptibility, DrudeSusceptibility,
@@ -431,3 +433,37 @@ def tfsf_tables_model(ndims, steps=120, pml=4, kmax=None):
     model = Model(grid, TFSFSource(box=box, tables=tabs), tmax=tmax, CN=0.99,
                   pml=_cpml(pml, d, kmax), material=mat)
     return model
+
+
+def plane_wave_pair(nd, n=16, steps=40, grid=None, CN=0.99, scatterer=True):
+    """(record model, aux-mode model with a scatterer, box coords, grid, tmax) for a
+    normal-incidence plane wave."""
+    d = 40e-9
+    L = (n - 1) * d
+    w = gauss_waveform(lam0=800e-9, tau0=3e-15, dt0=8e-15)
+    if grid is None:
+        if nd == 1:
+            grid = Grid(zmin=0, zmax=(4 * n - 1) * d, Nz=4 * n)
+        elif nd == 2:
+            grid = Grid(xmin=0, xmax=L, Nx=n, zmin=0, zmax=L, Nz=n)
+        else:
+            grid = Grid(xmin=0, xmax=L, Nx=n, ymin=0, ymax=L, Ny=n, zmin=0, zmax=L, Nz=n)
+    zL = grid.z[-1]
+    zb = (4 * d, zL - 4 * d)
+    box = {1: zb, 2: (4 * d, L - 4 * d) + zb, 3: (4 * d, L - 4 * d, 4 * d, L - 4 * d) + zb}[nd]
+    tmax = (steps - 0.5) * 0.99 * d / (C0 * math.sqrt(nd))
+    plane = {1: lambda z: np.abs(z - 2 * d) < d / 2,
+             2: lambda x, z: (np.abs(z - 2 * d) < d / 2) + 0 * x > 0,
+             3: lambda x, y, z: (np.abs(z - 2 * d) < d / 2) + 0 * x * y > 0}[nd]
+    wave = {1: lambda z, t: w(t) + 0 * z, 2: lambda x, z, t: w(t) + 0 * x,
+            3: lambda x, y, z, t: w(t) + 0 * x}[nd]
+    rec = Model(grid, SoftSource(geometry=plane, waveform=wave, component="Ex"), tmax=tmax, CN=CN,
+                pml=2 * d)
+    c = zL / 2
+    geo = {1: lambda z: np.abs(z - c) < 3 * d,
+           2: lambda x, z: (x - L / 2) ** 2 + (z - c) ** 2 <= (3 * d) ** 2,
+           3: lambda x, y, z: (x - L / 2) ** 2 + (y - L / 2) ** 2 + (z - c) ** 2 <= (3 * d) ** 2}[nd]
+    mat = Material(geometry=geo, eps=3.0) if scatterer else None
+    aux = Model(grid, TFSFSource(box=box, waveform=w, component="Ex", z0=2 * d), tmax=tmax, CN=CN,
+                pml=2 * d, material=mat)
+    return rec, aux, box, grid, tmax, mat

@@ -203,40 +203,6 @@ def test_output_component_check():
 
 
 # ---- TFSF record files ------------------------------------------------------------------
-def _plane_wave_models(nd, n=16, steps=40, grid=None, CN=0.99, scatterer=True):
-    """(record model, aux-mode model with a scatterer, box coords, grid, tmax) for a
-    normal-incidence plane wave."""
-    d = 40e-9
-    L = (n - 1) * d
-    w = cases.gauss_waveform(lam0=800e-9, tau0=3e-15, dt0=8e-15)
-    if grid is None:
-        if nd == 1:
-            grid = Grid(zmin=0, zmax=(4 * n - 1) * d, Nz=4 * n)
-        elif nd == 2:
-            grid = Grid(xmin=0, xmax=L, Nx=n, zmin=0, zmax=L, Nz=n)
-        else:
-            grid = Grid(xmin=0, xmax=L, Nx=n, ymin=0, ymax=L, Ny=n, zmin=0, zmax=L, Nz=n)
-    zL = grid.z[-1]
-    zb = (4 * d, zL - 4 * d)
-    box = {1: zb, 2: (4 * d, L - 4 * d) + zb, 3: (4 * d, L - 4 * d, 4 * d, L - 4 * d) + zb}[nd]
-    tmax = (steps - 0.5) * 0.99 * d / (C0 * math.sqrt(nd))
-    plane = {1: lambda z: np.abs(z - 2 * d) < d / 2,
-             2: lambda x, z: (np.abs(z - 2 * d) < d / 2) + 0 * x > 0,
-             3: lambda x, y, z: (np.abs(z - 2 * d) < d / 2) + 0 * x * y > 0}[nd]
-    wave = {1: lambda z, t: w(t) + 0 * z, 2: lambda x, z, t: w(t) + 0 * x,
-            3: lambda x, y, z, t: w(t) + 0 * x}[nd]
-    rec = Model(grid, SoftSource(geometry=plane, waveform=wave, component="Ex"), tmax=tmax, CN=CN,
-                pml=2 * d)
-    c = zL / 2
-    geo = {1: lambda z: np.abs(z - c) < 3 * d,
-           2: lambda x, z: (x - L / 2) ** 2 + (z - c) ** 2 <= (3 * d) ** 2,
-           3: lambda x, y, z: (x - L / 2) ** 2 + (y - L / 2) ** 2 + (z - c) ** 2 <= (3 * d) ** 2}[nd]
-    mat = Material(geometry=geo, eps=3.0) if scatterer else None
-    aux = Model(grid, TFSFSource(box=box, waveform=w, component="Ex", z0=2 * d), tmax=tmax, CN=CN,
-                pml=2 * d, material=mat)
-    return rec, aux, box, grid, tmax, mat
-
-
 @pytest.mark.parametrize("nd", [1, 2, 3])
 def test_tfsf_record_and_same_grid_replay(tmp_path, nd):
     """solve(...; tfsf_record=true, tfsf_box) writes prepare_tfsf_record's datasets with
@@ -244,7 +210,7 @@ def test_tfsf_record_and_same_grid_replay(tmp_path, nd):
     recorded tables exactly, so around a scatterer it equals the plane-wave (auxiliary
     run) TFSF of the same wave bit for bit."""
     orc = oracle("f64")
-    rec, aux, box, grid, tmax, mat = _plane_wave_models(nd)
+    rec, aux, box, grid, tmax, mat = cases.plane_wave_pair(nd)
     path = str(tmp_path / "rec_tfsf.h5")
     solve(rec, tfsf_record=True, tfsf_box=box, tfsf_fname=path, lib=orc)
     f = H5File(path)
@@ -272,7 +238,7 @@ def test_tfsf_replay_on_another_transverse_grid(tmp_path):
     step): the incident plane wave is uniform across x and y, so the interpolated tables
     are exact and the replay equals that grid's own plane-wave TFSF."""
     orc = oracle("f64")
-    rec, _, box, grid, tmax, _ = _plane_wave_models(3)
+    rec, _, box, grid, tmax, _ = cases.plane_wave_pair(3)
     path = str(tmp_path / "rec.h5")
     solve(rec, tfsf_record=True, tfsf_box=box, tfsf_fname=path, lib=orc)
     d = 40e-9
