@@ -216,7 +216,7 @@ enum {
     MFD_OPT_KERNEL_TIMING = 3, /* hipEvent pairs around every H and E launch (eager)       */
     MFD_OPT_DFT_BLOCK = 4,     /* time steps per DFT flush (1..64, before upload)          */
     MFD_OPT_ROWS = 5,          /* y rows per 3D workgroup (1, 2, 4, 8)                     */
-    MFD_OPT_KERNEL_FORM = 6,   /* 3D kernel form: 0 scalar, 1 4-cell vector, 2 pipelined   */
+    MFD_OPT_KERNEL_FORM = 6,   /* 3D kernel form: 0 one cell per thread, 2 z march (default)*/
     MFD_OPT_STAGGER = 7,       /* byte offset between field arrays (before upload)         */
     MFD_OPT_QUANTUM = 8,       /* server: steps per scheduling round (default 64)          */
     MFD_OPT_FILL_MCELLS = 9,   /* server: admit while active cells < value * 2^20 (16)     */
@@ -237,16 +237,6 @@ int32_t mfd_model_set_options(mfd_model *m, int32_t opt, int32_t value);
 /* Accumulated hipEvent time of the H (which = 0) or E (which = 1) half-step launches
  * since the last reset (which = 2 resets), measured on the model's stream.             */
 int32_t mfd_model_timing(mfd_model *m, int32_t which, double *total_ms, int32_t *count);
-/* HBM bandwidth probes: a float4 device copy of `bytes` bytes (plain loads and stores:
- * the calibration run for the byte counters); the fastest copy form measured here (4
- * float4 per lane in flight, nontemporal loads and stores, 16 workgroups per CU: the
- * practical HBM ceiling); and a grid-stride kernel with nin reads and nout writes
- * (nin == nout, 1..6) of `bytes_per_array` each, launched with blocks_per_cu (>= 1)
- * workgroups per CU.  Average milliseconds per iteration.                              */
-int32_t mfd_stream_copy(int64_t bytes, int32_t iters, double *ms_per_iter);
-int32_t mfd_stream_copy_fast(int64_t bytes, int32_t iters, double *ms_per_iter);
-int32_t mfd_stream_multi(int64_t bytes_per_array, int32_t nin, int32_t nout,
-                         int32_t blocks_per_cu, int32_t iters, double *ms_per_iter);
 
 /* ---- batches (no counterpart in the reference, which runs one model per solve!,
  * src/models.jl:166-242) ------------------------------------------------------------

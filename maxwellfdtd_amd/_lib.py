@@ -147,9 +147,6 @@ ENGINE_EXTRA = {
     "model_timing": (I32, [P, I32, DP, IP]),
     "model_get_energy": (I32, [P, DP]),
     "model_get_energy_noalloc": (I32, [P, P, P]),
-    "stream_copy": (I32, [C.c_int64, I32, DP]),
-    "stream_copy_fast": (I32, [C.c_int64, I32, DP]),
-    "stream_multi": (I32, [C.c_int64, I32, I32, I32, I32, DP]),
     "batch_create": (I32, [C.POINTER(P)]),
     "batch_add": (I32, [P, P]),
     "batch_set_options": (I32, [P, I32, I32]),
@@ -179,3 +176,43 @@ def engine():
                 "(python -c 'import __graft_entry__ as g; g.build()')")
         _engine = Binding(ENGINE_LIB, "mfd_", ENGINE_EXTRA)
     return _engine
+
+
+PROBE_LIB = os.path.join(os.path.dirname(ENGINE_LIB), "libmfd_hbm_probe.so")
+_probe = None
+
+
+class _Probe:
+    """ctypes binding of the HBM probe library (bandwidth measurements)."""
+
+    SIGS = {
+        "probe_last_error": (C.c_char_p, []),
+        "probe_copy": (I32, [C.c_int64, I32, DP]),
+        "probe_copy_fast": (I32, [C.c_int64, I32, DP]),
+        "probe_multi": (I32, [C.c_int64, I32, I32, I32, I32, DP]),
+    }
+
+    def __init__(self, path):
+        self.path = path
+        self.lib = C.CDLL(path)
+        for name, (res, args) in self.SIGS.items():
+            fn = getattr(self.lib, "mfd_" + name)
+            fn.restype = res
+            fn.argtypes = args
+            setattr(self, name, fn)
+
+    def check(self, status, what=""):
+        if status != MFD_OK:
+            msg = self.probe_last_error()
+            raise EngineError(f"mfd_{what} failed ({status}): {msg.decode() if msg else ''}")
+
+
+def probe():
+    """The HBM probe library; raises if it has not been built."""
+    global _probe
+    if _probe is None:
+        if not os.path.exists(PROBE_LIB):
+            raise EngineError(f"{PROBE_LIB} is missing: build first "
+                              "(python -c 'import __graft_entry__ as g; g.build()')")
+        _probe = _Probe(PROBE_LIB)
+    return _probe

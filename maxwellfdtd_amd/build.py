@@ -1,8 +1,9 @@
-"""Build the HIP engine library libmaxwellfdtd_amd.so in-tree for gfx950.
+"""Build the HIP engine library libmaxwellfdtd_amd.so in-tree for gfx950, and beside it
+the HBM probe library libmfd_hbm_probe.so (bandwidth measurements, not the solver).
 
 Usage: python -m maxwellfdtd_amd.build   (or maxwellfdtd_amd.build.build()).
 Every source is compiled by hipcc for gfx950 only, with packed-FP32 instruction
-selection turned off, and linked into one shared library with a C ABI.
+selection turned off, and linked into a shared library with a C ABI.
 """
 import os
 import shutil
@@ -14,8 +15,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmaxwellfdtd_amd.so")
-SOURCES = ["kernels3d.hip", "kernels3d_v4.hip", "kernels3d_p.hip", "kernels2d.hip", "kernels1d.hip", "step_kernels.hip",
-           "kernels_state.hip", "energy.hip", "engine.cpp", "batch.cpp", "stream_copy.hip"]
+PROBE_LIB = os.path.join(HERE, "libmfd_hbm_probe.so")
+SOURCES = ["kernels3d.hip", "kernels3d_p.hip", "kernels2d.hip", "kernels1d.hip", "step_kernels.hip",
+           "kernels_state.hip", "energy.hip", "engine.cpp", "batch.cpp"]
+PROBE_SOURCES = ["hbm_probe.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=fast",
@@ -32,19 +35,24 @@ def _compile(src):
     return obj
 
 
-def build(verbose=True):
-    os.makedirs(BUILD, exist_ok=True)
-    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
-    tmp = LIB + ".tmp"
+def _link(objs, lib):
+    tmp = lib + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"link failed:\n{r.stderr}")
-    os.replace(tmp, LIB)
+        raise RuntimeError(f"link of {os.path.basename(lib)} failed:\n{r.stderr}")
+    os.replace(tmp, lib)
+
+
+def build(verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = SOURCES + PROBE_SOURCES
+    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        objs = dict(zip(srcs, ex.map(_compile, srcs)))
+    _link([objs[s] for s in SOURCES], LIB)
+    _link([objs[s] for s in PROBE_SOURCES], PROBE_LIB)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {os.path.basename(PROBE_LIB)}")
     return LIB
 
 

@@ -223,7 +223,6 @@ __host__ __device__ inline int tile_z1(const Tile &t) { return (t.w >> 16) & 0xf
 
 // tile geometry per dimensionality
 constexpr int TX3 = 64, TY3 = 4;    // 3D scalar kernels: 64 x 4 columns, marching z
-constexpr int TX3V = 256;           // 3D vector kernels: 256 x (4 per lane) x TY rows
 constexpr int TX2 = 64;             // 2D: 64 x-columns per wave, TW waves per tile, marching z
 constexpr int TX1 = 256;            // 1D: 256 cells
 constexpr int XFACE_ZC = 64;        // longest tile z range with the E kernel's x-face epilogue
@@ -236,8 +235,6 @@ void launch_h3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, in
                   hipStream_t s);
 void launch_e3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int ty,
                   bool pml, hipStream_t s);
-void launch_h3d_v4(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int ty, hipStream_t s);
-void launch_e3d_v4(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int ty, hipStream_t s);
 void launch_h2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int tw, hipStream_t s);
 void launch_e2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int tw, hipStream_t s);
 void launch_h1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool f64,

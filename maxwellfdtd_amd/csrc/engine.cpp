@@ -666,7 +666,7 @@ void Runner::release() {
 static void make_tiles(const Model &m, int idx, int zchunk, int ty, int kform, std::vector<Tile> &out) {
     const int Nx = m.d.Nx, Ny = m.d.Ny, Nz = m.d.Nz;
     if (m.d.ndims == 3 && kform > 0) {
-        const int tx = kform == 1 ? TX3V : 64;
+        const int tx = 64;
         for (int z0 = 0; z0 < Nz; z0 += zchunk)
             for (int y0 = 0; y0 < Ny; y0 += ty)
                 for (int x0 = 0; x0 < Nx; x0 += tx)
@@ -706,7 +706,6 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     long long maxN = 1;
     kform = ndims == 3 ? opt.vec : 0;
     for (Model *m : ms) {
-        if (kform == 1 && m->d.Nx % 4 != 0) kform = 2;
         if (kform == 2 && m->N >= (1LL << 30)) kform = 0;   // 32-bit cell offsets
     }
     f64 = ms[0]->f64;
@@ -881,8 +880,7 @@ int Runner::launch_step(bool timing, bool dft_flush) {
     if (timing) MFD_HIP(hipEventRecord(ev[0], stream));
     const int mode = any_full ? MODE_FULL : any_mats ? MODE_SIMPLE : MODE_VAC;
     const int hmode = h_mats ? MODE_SIMPLE : MODE_VAC;
-    if (ndims == 3 && kform == 1) launch_h3d_v4(d_models, d_tiles, ntiles_h, grid_h, hmode, opt.ty, stream);
-    else if (ndims == 3 && kform == 2) {
+    if (ndims == 3 && kform == 2) {
         launch_h3d_p(d_models, d_tiles, nph, grid_of(nph), hmode, opt.ty, true, stream);
         launch_h3d_p(d_models, d_tiles + nph, nih, grid_of(nih), hmode, opt.ty, false, stream);
     }
@@ -890,8 +888,7 @@ int Runner::launch_step(bool timing, bool dft_flush) {
     else if (ndims == 2) launch_h2d(d_models, d_tiles, ntiles_h, grid_h, hmode, opt.ty, stream);
     else launch_h1d(d_models, d_tiles, ntiles_h, grid_h, hmode, f64, stream);
     if (timing) { MFD_HIP(hipEventRecord(ev[1], stream)); MFD_HIP(hipEventRecord(ev[2], stream)); }
-    if (ndims == 3 && kform == 1) launch_e3d_v4(d_models, d_tiles_e, ntiles, grid, mode, any_int, opt.ty, stream);
-    else if (ndims == 3 && kform == 2) {
+    if (ndims == 3 && kform == 2) {
         launch_e3d_p(d_models, d_tiles_e, nie, grid_of(nie), mode, any_int, opt.ty, false, stream);
         launch_e3d_p(d_models, d_tiles_e + nie, npe, grid_of(npe), mode, any_int, opt.ty, true, stream);
     }
@@ -1080,7 +1077,7 @@ int set_option(RunOptions &o, Model *m, int opt, int value) {
             o.ty = value;
             break;
         case MFD_OPT_KERNEL_FORM:
-            if (value < 0 || value > 2) { set_error("3D kernel form must be 0, 1 or 2"); return MFD_ERR_ARG; }
+            if (value != 0 && value != 2) { set_error("3D kernel form must be 0 (one cell per thread) or 2 (pipelined z march)"); return MFD_ERR_ARG; }
             o.vec = value;
             break;
         case MFD_OPT_SNAKE: o.snake = value != 0; break;

@@ -1,13 +1,30 @@
-// Plain device copy used as the HBM reference: achievable copy bandwidth on this GPU
-// and the scale of the FETCH_SIZE / WRITE_SIZE counters for a known byte count.
+// HBM bandwidth probes, built as their own library (libmfd_hbm_probe.so) beside the
+// engine: a plain device copy (the achievable copy bandwidth on this GPU, and the scale
+// of the FETCH_SIZE / WRITE_SIZE counters for a known byte count), the fastest copy form
+// measured here, and a many-stream variant.  Nothing in the engine calls them.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
 #include <vector>
 
-#include "engine.hpp"
+#include "hbm_probe.h"
 
-namespace mfd {
+namespace mfd_probe {
+
+enum { MFD_OK = 0, MFD_ERR_ARG = -1, MFD_ERR_NOMEM = -2, MFD_ERR_HIP = -3 };
+
+static thread_local std::string g_err;
+static void set_error(const std::string &s) { g_err = s; }
+
+#define MFD_HIP(x)                                                           \
+    do {                                                                     \
+        const hipError_t e_ = (x);                                           \
+        if (e_ != hipSuccess) {                                              \
+            set_error(std::string(#x) + ": " + hipGetErrorString(e_));       \
+            return MFD_ERR_HIP;                                              \
+        }                                                                    \
+    } while (0)
 
 __global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ src,
                                               float4 *__restrict__ dst, long long n4) {
@@ -37,19 +54,6 @@ __global__ __launch_bounds__(256) void k_copy_fast(const float4 *__restrict__ sr
     }
 }
 
-static int32_t time_copy(int64_t bytes, int32_t iters, double *ms_per_iter, bool fast);
-
-}  // namespace mfd
-
-extern "C" int32_t mfd_stream_copy(int64_t bytes, int32_t iters, double *ms_per_iter) {
-    return mfd::time_copy(bytes, iters, ms_per_iter, false);
-}
-
-extern "C" int32_t mfd_stream_copy_fast(int64_t bytes, int32_t iters, double *ms_per_iter) {
-    return mfd::time_copy(bytes, iters, ms_per_iter, true);
-}
-
-namespace mfd {
 // device buffers and events of one probe, released on every return path
 struct ProbeRes {
     std::vector<void *> mem;
@@ -96,10 +100,8 @@ static int32_t cu_count_of_device(int *cus) {
     MFD_HIP(hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev));
     return MFD_OK;
 }
-}  // namespace mfd
 
-static int32_t mfd::time_copy(int64_t bytes, int32_t iters, double *ms_per_iter, bool fast) {
-    using namespace mfd;
+static int32_t time_copy(int64_t bytes, int32_t iters, double *ms_per_iter, bool fast) {
     if (bytes < 1024 || iters < 1 || !ms_per_iter) { set_error("bad arguments"); return MFD_ERR_ARG; }
     const long long n4 = bytes / 16;
     ProbeRes r;
@@ -120,7 +122,6 @@ static int32_t mfd::time_copy(int64_t bytes, int32_t iters, double *ms_per_iter,
 // Multi-stream variant: NIN arrays read, NOUT arrays written per element, as the
 // half-step kernels do; used to measure how HBM throughput depends on the number of
 // concurrent streams.
-namespace mfd {
 template <int NIN, int NOUT>
 __global__ __launch_bounds__(256) void k_multi(float4 *const *__restrict__ in,
                                                float4 *const *__restrict__ out, long long n4) {
@@ -140,11 +141,8 @@ __global__ __launch_bounds__(256) void k_multi(float4 *const *__restrict__ in,
         for (int k = 0; k < NOUT; k++) ((__attribute__((address_space(1))) v4 *)op[k])[i] = s;
     }
 }
-}  // namespace mfd
-
-extern "C" int32_t mfd_stream_multi(int64_t bytes_per_array, int32_t nin, int32_t nout,
-                                    int32_t blocks_per_cu, int32_t iters, double *ms_per_iter) {
-    using namespace mfd;
+static int32_t time_multi(int64_t bytes_per_array, int32_t nin, int32_t nout, int32_t blocks_per_cu,
+                          int32_t iters, double *ms_per_iter) {
     if (nin < 1 || nin > 6 || nout < 1 || nout > 6 || nin != nout || iters < 1 ||
         blocks_per_cu < 1 || bytes_per_array < 1024) {
         set_error("need nin == nout in 1..6, blocks_per_cu >= 1, bytes >= 1024");
@@ -177,3 +175,24 @@ extern "C" int32_t mfd_stream_multi(int64_t bytes_per_array, int32_t nin, int32_
         }
     }, ms_per_iter);
 }
+
+}  // namespace mfd_probe
+
+extern "C" {
+
+const char *mfd_probe_last_error(void) { return mfd_probe::g_err.c_str(); }
+
+int32_t mfd_probe_copy(int64_t bytes, int32_t iters, double *ms_per_iter) {
+    return mfd_probe::time_copy(bytes, iters, ms_per_iter, false);
+}
+
+int32_t mfd_probe_copy_fast(int64_t bytes, int32_t iters, double *ms_per_iter) {
+    return mfd_probe::time_copy(bytes, iters, ms_per_iter, true);
+}
+
+int32_t mfd_probe_multi(int64_t bytes_per_array, int32_t nin, int32_t nout, int32_t blocks_per_cu,
+                        int32_t iters, double *ms_per_iter) {
+    return mfd_probe::time_multi(bytes_per_array, nin, nout, blocks_per_cu, iters, ms_per_iter);
+}
+
+}  // extern "C"

@@ -1,4 +1,4 @@
-"""HBM bandwidth probe: times the engine library's device copy kernel.
+"""HBM bandwidth probe: times the device copy kernels of libmfd_hbm_probe.so.
 
     python -m maxwellfdtd_amd.hbm_probe [--mib 1024] [--iters 5]
 
@@ -19,14 +19,14 @@ def main():
     ap.add_argument("--plain-only", action="store_true",
                     help="only the plain copy (the byte-counter calibration run)")
     a = ap.parse_args()
-    lib = _lib.engine()
+    lib = _lib.probe()
     nbytes = a.mib << 20
     ms = C.c_double()
-    lib.check(lib.stream_copy(nbytes, a.iters, C.byref(ms)), "stream_copy")
+    lib.check(lib.probe_copy(nbytes, a.iters, C.byref(ms)), "probe_copy")
     print(f"copy {nbytes} bytes: read {nbytes} + write {nbytes} per launch, "
           f"{ms.value * 1e3:.1f} us, {2 * nbytes / ms.value / 1e6:.0f} GB/s", flush=True)
     if not a.plain_only:
-        lib.check(lib.stream_copy_fast(nbytes, a.iters, C.byref(ms)), "stream_copy_fast")
+        lib.check(lib.probe_copy_fast(nbytes, a.iters, C.byref(ms)), "probe_copy_fast")
         print(f"fast copy (4 x float4 per lane, nontemporal) {nbytes} bytes: "
               f"{ms.value * 1e3:.1f} us, {2 * nbytes / ms.value / 1e6:.0f} GB/s", flush=True)
 

@@ -122,14 +122,17 @@ def test_engine_rejects_bad_descriptor(engine_so, bad):
 def test_hbm_probes_check_arguments(engine_so):
     """The copy probes refuse bad sizes, counts and a missing output before any device
     call, so these run without a GPU."""
-    lib = _lib.engine()
+    lib = _lib.probe()
     ms = C.c_double(0.0)
-    assert lib.stream_copy(512, 1, C.byref(ms)) == _lib.MFD_ERR_ARG
-    assert lib.stream_copy_fast(1 << 20, 0, C.byref(ms)) == _lib.MFD_ERR_ARG
-    assert lib.stream_copy(1 << 20, 1, None) == _lib.MFD_ERR_ARG
-    assert lib.stream_multi(1 << 20, 2, 3, 16, 1, C.byref(ms)) == _lib.MFD_ERR_ARG
-    assert lib.stream_multi(1 << 20, 7, 7, 16, 1, C.byref(ms)) == _lib.MFD_ERR_ARG
-    assert lib.stream_multi(1 << 20, 2, 2, 16, 1, None) == _lib.MFD_ERR_ARG
+    assert lib.probe_copy(512, 1, C.byref(ms)) == _lib.MFD_ERR_ARG
+    assert lib.probe_copy_fast(1 << 20, 0, C.byref(ms)) == _lib.MFD_ERR_ARG
+    assert lib.probe_copy(1 << 20, 1, None) == _lib.MFD_ERR_ARG
+    assert lib.probe_multi(1 << 20, 2, 3, 16, 1, C.byref(ms)) == _lib.MFD_ERR_ARG
+    assert lib.probe_multi(1 << 20, 7, 7, 16, 1, C.byref(ms)) == _lib.MFD_ERR_ARG
+    assert b"nin == nout" in lib.probe_last_error()
+    assert lib.probe_multi(1 << 20, 2, 2, 16, 1, None) == _lib.MFD_ERR_ARG
+    engine = _lib.engine()
+    assert not hasattr(engine.lib, "mfd_stream_copy")
 
 
 def test_engine_rejects_bad_bc(engine_so):
