@@ -208,6 +208,8 @@ def nonlinear_3d(n=24, steps=150, I0=1e17, rate="power"):
     w0 = 2 * np.pi * C0 / 800e-9
     if rate == "power":
         ionrate = PowerLawRate(sigma=1e-90, K=6)
+    elif rate == "function":     # a plain function of I, tabulated log-log by the package
+        ionrate = lambda I: 1e-90 * I ** 6
     else:
         x = np.linspace(13, 19, 61)
         ionrate = IonizationRate(x=x, y=6 * x - 90.0 + 0.01 * np.sin(x))
