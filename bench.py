@@ -111,9 +111,9 @@ def oracle_rate(orc, desc, seconds, N, label=""):
 def cpu_leg(desc, seconds, N, label, gpu_lib, seconds_1t=5.0):
     """The oracle (FP64) on all usable host cores for a bounded number of steps, then on
     one thread for a shorter sample, then the GPU engine for the first sample's steps:
-    CPU rates and GPU-vs-FP64 error."""
+    CPU rates and GPU-vs-FP64 error (fields, integrals, total field energy, DFT spectra)."""
     sys.path.insert(0, os.path.join(HERE, "tests"))
-    from oracle_util import field_errors, oracle, rel_l2
+    from oracle_util import field_energy, field_errors, oracle, rel_l2, spectra_errors
     from maxwellfdtd_amd import _lib
     from maxwellfdtd_amd.models import Handle
     orc = oracle("f64")
@@ -141,8 +141,12 @@ def cpu_leg(desc, seconds, N, label, gpu_lib, seconds_1t=5.0):
         errs["Sa"] = rel_l2(hg.output(_lib.MFD_OUT_SA), ho.output(_lib.MFD_OUT_SA))
     if desc.integrals & 2:
         errs["E2"] = rel_l2(hg.output(_lib.MFD_OUT_E2), ho.output(_lib.MFD_OUT_E2))
-    parity = {"steps": ns, "rel_l2_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()}}
-    log(f"parity after {ns} steps: {errs}")
+    ge, gh = hg.energy()
+    oe, oh = field_energy(ho)
+    parity = {"steps": ns, "rel_l2_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
+              "energy_rel_err": float(f"{abs((ge + gh) / (oe + oh) - 1):.3e}") if oe + oh > 0 else None,
+              **spectra_errors(hg, ho, desc)}
+    log(f"parity after {ns} steps: {parity}")
     hg.close()
     ho.close()
     return cpu, parity
@@ -452,37 +456,11 @@ def bench_served_3d(args, lib):
     return out
 
 
-def spectra_errors(hg, ho, desc):
-    """Relative L2 and the max relative error of every spectral monitor's spectra: per
-    component and point, max over frequencies of |S_gpu - S_ref| over the largest
-    |S_ref| of that spectrum."""
-    from maxwellfdtd_amd.models import MONITOR_COMPONENTS
-    sys.path.insert(0, os.path.join(HERE, "tests"))
-    from oracle_util import rel_l2
-    out = {}
-    for i, mon in enumerate(desc.monitors):
-        if mon["kind"] != "spectral":
-            continue
-        worst, a_all, b_all = 0.0, [], []
-        for c in MONITOR_COMPONENTS[desc.ndims]:
-            a = np.atleast_2d(hg.monitor(i, c))
-            b = np.atleast_2d(ho.monitor(i, c))
-            a_all.append(a.ravel())
-            b_all.append(b.ravel())
-            den = np.max(np.abs(b), axis=-1)
-            ok = den > 0
-            if np.any(ok):
-                worst = max(worst, float(np.max(np.max(np.abs(a - b), axis=-1)[ok] / den[ok])))
-        out[f"dft{i}_rel_l2"] = float(f"{rel_l2(np.concatenate(a_all), np.concatenate(b_all)):.3e}")
-        out[f"dft{i}_max_rel"] = float(f"{worst:.3e}")
-    return out
-
-
 def bench_slab_1d(args, lib):
     """The 1D configuration, run as the reference defines it: FP64 state (MFD_OPT_F64,
     --f32 for the FP32 state), the whole 8000 steps on the GPU and on the oracle."""
     sys.path.insert(0, os.path.join(HERE, "tests"))
-    from oracle_util import field_energy, field_errors, oracle, rel_l2
+    from oracle_util import field_energy, field_errors, oracle, rel_l2, spectra_errors
     from maxwellfdtd_amd import _lib
     from maxwellfdtd_amd.models import Handle, MONITOR_COMPONENTS
     from maxwellfdtd_amd.workloads import slab_1d

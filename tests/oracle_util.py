@@ -83,3 +83,34 @@ def field_energy(h):
     dx, dy, dz = d.d
     dV = {1: dz, 2: dx * dz, 3: dx * dy * dz}[d.ndims]
     return 0.5 * dV * float(np.sum(eps * e2)), 0.5 * dV * float(np.sum(mu * h2))
+
+
+def spectra_errors(hg, ho, desc):
+    """Relative L2 and the max relative error of every spectral monitor's spectra: per
+    component and point, max over frequencies of |S_gpu - S_ref| over the largest
+    |S_ref| of that spectrum.  Spectra whose largest value is below a thousandth of the
+    largest spectrum of their field (E or H, all components and points) are left out of
+    the maximum: points the field has not reached and components that vanish by
+    symmetry hold only rounding noise there.  They are in the relative L2."""
+    from maxwellfdtd_amd.models import MONITOR_COMPONENTS
+    out = {}
+    for i, mon in enumerate(desc.monitors):
+        if mon["kind"] != "spectral":
+            continue
+        a_all, b_all, spec = [], [], {}
+        for c in MONITOR_COMPONENTS[desc.ndims]:
+            a = np.atleast_2d(hg.monitor(i, c))
+            b = np.atleast_2d(ho.monitor(i, c))
+            a_all.append(a.ravel())
+            b_all.append(b.ravel())
+            spec[c] = (np.max(np.abs(a - b), axis=-1), np.max(np.abs(b), axis=-1))
+        peak = {f: max((float(np.max(d)) for c, (_, d) in spec.items() if c[0] == f and d.size),
+                       default=0.0) for f in "EH"}
+        worst = 0.0
+        for c, (err, den) in spec.items():
+            ok = den > 1e-3 * peak[c[0]]
+            if np.any(ok):
+                worst = max(worst, float(np.max(err[ok] / den[ok])))
+        out[f"dft{i}_rel_l2"] = float(f"{rel_l2(np.concatenate(a_all), np.concatenate(b_all)):.3e}")
+        out[f"dft{i}_max_rel"] = float(f"{worst:.3e}")
+    return out
