@@ -222,7 +222,8 @@ enum {
     MFD_OPT_FILL_MCELLS = 9,   /* server: admit while active cells < value * 2^20 (16)     */
     MFD_OPT_SNAKE = 10,        /* E half walks the tile list backwards (default 1)         */
     MFD_OPT_DLESS = 11,        /* plain cells keep no D (rebuilt as E/Me on read; default 1) */
-    MFD_OPT_PML_SPLIT = 12,    /* tiles no CPML layer reaches run a slab-free kernel (0)   */
+    MFD_OPT_PML_SPLIT = 12,    /* CPML-free tiles in slab-free kernels: 0 off, 1 after the */
+                               /* CPML tiles, 2 beside them on a second stream             */
     MFD_OPT_SCHED = 13,        /* server: admit 0 in arrival order, 1 least work first (1) */
     MFD_OPT_ZCHUNK_H = 14,     /* pipelined 3D H half z planes per tile (8; 0 = ZCHUNK)    */
     MFD_OPT_XCD_H = 15,        /* 3D H tiles dealt to the XCDs like E's (1) or grid order  */

@@ -661,6 +661,9 @@ void Runner::release() {
     nchunks = 0;
     for (auto e : ev) (void)hipEventDestroy(e);
     ev.clear();
+    if (side) { (void)hipStreamSynchronize(side); (void)hipStreamDestroy(side); side = nullptr; }
+    if (fork_ev) { (void)hipEventDestroy(fork_ev); fork_ev = nullptr; }
+    if (join_ev) { (void)hipEventDestroy(join_ev); join_ev = nullptr; }
 }
 
 static void make_tiles(const Model &m, int idx, int zchunk, int ty, int kform, std::vector<Tile> &out) {
@@ -789,7 +792,12 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     MFD_HIP(hipHostMalloc(&h_flags, hd.size() * sizeof(int)));
     MFD_HIP(copy_sync(d_models, hd.data(), hd.size() * sizeof(DevModel), hipMemcpyHostToDevice, stream));
     // pipelined 3D: tiles that no CPML layer reaches go to the slab-free kernels
-    const bool split = ndims == 3 && kform == 2 && opt.pml_split;
+    const bool split = ndims == 3 && kform == 2 && opt.pml_split > 0;
+    if (split && opt.pml_split == 2) {
+        MFD_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        MFD_HIP(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+        MFD_HIP(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
+    }
     auto classify = [&](const std::vector<Tile> &all, std::vector<Tile> &tl_pml, std::vector<Tile> &tl_int) {
         for (const Tile &t : all) {
             bool in = !split;
@@ -881,16 +889,24 @@ int Runner::launch_step(bool timing, bool dft_flush) {
     const int mode = any_full ? MODE_FULL : any_mats ? MODE_SIMPLE : MODE_VAC;
     const int hmode = h_mats ? MODE_SIMPLE : MODE_VAC;
     if (ndims == 3 && kform == 2) {
-        launch_h3d_p(d_models, d_tiles, nph, grid_of(nph), hmode, opt.ty, true, stream);
+        // side by side (MFD_OPT_PML_SPLIT = 2): the CPML tiles run on the side stream,
+        // forked after the previous work on the runner's stream and joined before the next
+        hipStream_t sp = side ? side : stream;
+        if (side) { MFD_HIP(hipEventRecord(fork_ev, stream)); MFD_HIP(hipStreamWaitEvent(side, fork_ev, 0)); }
+        launch_h3d_p(d_models, d_tiles, nph, grid_of(nph), hmode, opt.ty, true, sp);
         launch_h3d_p(d_models, d_tiles + nph, nih, grid_of(nih), hmode, opt.ty, false, stream);
+        if (side) { MFD_HIP(hipEventRecord(join_ev, side)); MFD_HIP(hipStreamWaitEvent(stream, join_ev, 0)); }
     }
     else if (ndims == 3) launch_h3d(d_models, d_tiles, ntiles_h, grid_h, hmode, stream);
     else if (ndims == 2) launch_h2d(d_models, d_tiles, ntiles_h, grid_h, hmode, opt.ty, stream);
     else launch_h1d(d_models, d_tiles, ntiles_h, grid_h, hmode, f64, stream);
     if (timing) { MFD_HIP(hipEventRecord(ev[1], stream)); MFD_HIP(hipEventRecord(ev[2], stream)); }
     if (ndims == 3 && kform == 2) {
+        hipStream_t sp = side ? side : stream;
+        if (side) { MFD_HIP(hipEventRecord(fork_ev, stream)); MFD_HIP(hipStreamWaitEvent(side, fork_ev, 0)); }
         launch_e3d_p(d_models, d_tiles_e, nie, grid_of(nie), mode, any_int, opt.ty, false, stream);
-        launch_e3d_p(d_models, d_tiles_e + nie, npe, grid_of(npe), mode, any_int, opt.ty, true, stream);
+        launch_e3d_p(d_models, d_tiles_e + nie, npe, grid_of(npe), mode, any_int, opt.ty, true, sp);
+        if (side) { MFD_HIP(hipEventRecord(join_ev, side)); MFD_HIP(hipStreamWaitEvent(stream, join_ev, 0)); }
     }
     else if (ndims == 3) launch_e3d(d_models, d_tiles_e, ntiles, grid, mode, any_int, stream);
     else if (ndims == 2) launch_e2d(d_models, d_tiles_e, ntiles, grid, mode, any_int, opt.ty, stream);
@@ -1082,7 +1098,10 @@ int set_option(RunOptions &o, Model *m, int opt, int value) {
             break;
         case MFD_OPT_SNAKE: o.snake = value != 0; break;
         case MFD_OPT_DLESS: o.dless = value != 0; break;
-        case MFD_OPT_PML_SPLIT: o.pml_split = value != 0; break;
+        case MFD_OPT_PML_SPLIT:
+            if (value < 0 || value > 2) { set_error("CPML split must be 0, 1 or 2"); return MFD_ERR_ARG; }
+            o.pml_split = value;
+            break;
         case MFD_OPT_XCD_H: o.xcd_h = value != 0; break;
         case MFD_OPT_XCD_BLOCK:
             if (value < 0) { set_error("XCD block must be >= 0"); return MFD_ERR_ARG; }

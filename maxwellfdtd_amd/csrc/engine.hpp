@@ -134,10 +134,12 @@ struct RunOptions {
     bool kernel_timing = false; // hipEvent pairs around the H/E launches (eager mode)
     bool snake = true;          // E half walks each XCD's tile chunk backwards
     bool dless = true;          // plain cells keep no D where the run allows it
-    // pipelined 3D: tiles outside every CPML layer get launches of their own (off: with
-    // one launch, waves of CPML tiles, heavy in ALU work, share CUs with the memory-bound
-    // interior waves and the step is faster)
-    bool pml_split = false;
+    // pipelined 3D: tiles outside every CPML layer get launches of their own, built
+    // without the slab code (fewer registers, more waves).  0: one launch per half-step
+    // (waves of CPML tiles, heavy in ALU work, share CUs with the memory-bound interior
+    // waves); 1: the two launches one after the other; 2: the two launches side by side,
+    // the CPML tiles' on a second stream forked from and joined to the runner's stream
+    int pml_split = 0;
     bool xcd_h = true;          // 3D H tile list dealt to the XCDs like E's (else grid order)
     int xcd_block = 64;         // 3D: tiles per block dealt to the XCDs (0: one chunk per XCD)
 };
@@ -178,6 +180,8 @@ struct Runner {
     uint64_t gmask = 0;         // steps of the graph that launch the DFT flush
     int cu_count = 256;
     std::vector<hipEvent_t> ev;
+    hipStream_t side = nullptr;           // MFD_OPT_PML_SPLIT = 2: the CPML tiles' stream
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     double t_h_ms = 0, t_e_ms = 0;
     long long n_h = 0, n_e = 0;
 

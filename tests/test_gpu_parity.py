@@ -102,6 +102,17 @@ def test_cpml_free_tiles():
     for c in ("Hx", "Hy", "Hz", "Ex", "Ey", "Ez", "Dx"):
         assert rel_l2(hg.field(c), whole.field(c)) <= 1e-5, c
     assert rel_l2(hg.output(0), whole.output(0)) <= 1e-5
+    # the two launches side by side on two streams (graph replays and eager steps) run
+    # the same kernels on the same tiles: bit for bit the serial split
+    for gsteps in (16, 0):
+        side = Handle(desc, _lib.engine())
+        side.lib.check(side.lib.model_set_options(side.h, _lib.MFD_OPT_PML_SPLIT, 2), "set_options")
+        side.lib.check(side.lib.model_set_options(side.h, _lib.MFD_OPT_GRAPH_STEPS, gsteps), "set_options")
+        side.step(desc.Nt)
+        for c in ("Hx", "Hy", "Hz", "Ex", "Ey", "Ez", "Dx"):
+            assert np.array_equal(hg.field(c), side.field(c)), (gsteps, c)
+        assert np.array_equal(hg.output(0), side.output(0))
+        side.close()
 
 
 def test_spheres_3d_boundary_conditions():
