@@ -308,24 +308,31 @@ def bench_batched_2d(args, lib):
     from maxwellfdtd_amd import _lib
     from maxwellfdtd_amd.batch import Batch
     from maxwellfdtd_amd.models import Handle
-    from maxwellfdtd_amd.workloads import batched_2d
+    from maxwellfdtd_amd.workloads import algorithmic_bytes, batched_2d
 
     t0 = time.perf_counter()
-    descs = [m.describe() for m in batched_2d(count=args.count,
-                                               smin=max(2, int(2000 * args.step_scale)),
-                                               smax=max(2, int(10000 * args.step_scale)))]
+    models = list(batched_2d(count=args.count, smin=max(2, int(2000 * args.step_scale)),
+                             smax=max(2, int(10000 * args.step_scale))))
+    descs = [m.describe() for m in models]
+    step_bytes = np.array([algorithmic_bytes(m) for m in models])
+    del models
     cells = np.array([int(np.prod(d.dims)) for d in descs], dtype=float)
     work = float(np.sum(cells * np.array([d.Nt for d in descs])))
     log(f"{len(descs)} instances built in {time.perf_counter()-t0:.1f}s: "
         f"{cells.sum()/1e6:.1f} M cells, {work/1e12:.3f} T cell-updates")
 
-    # batched: one batch, autotuned launch shape, solves retire as they finish
+    # batched: one batch, autotuned launch shape, solves retire as they finish.  The
+    # autotuner's trial steps are real steps; throughput counts the steps after tuning
+    # over the time after tuning (as for the plain runs below), latency runs from the
+    # batch's start
     t0 = time.perf_counter()
     b = Batch(descs)
     t_build = time.perf_counter() - t0
     t0 = time.perf_counter()
     tune = b.autotune(trial_steps=8)
-    log(f"autotune: {tune}")
+    t_tune = time.perf_counter() - t0
+    it_tuned = np.array([h.it for h in b.handles], dtype=float)
+    log(f"autotune: {tune} in {t_tune:.2f} s")
     done_t = np.zeros(len(descs))
     while b.active > 0:
         try:
@@ -343,72 +350,102 @@ def bench_batched_2d(args, lib):
     wall_b = time.perf_counter() - t0
     done_steps = np.array([h.it for h in b.handles], dtype=float)
     work = float(np.sum(cells * done_steps))
-    g_batch = work / wall_b / 1e9
+    run_b = wall_b - t_tune
+    g_batch = float(np.sum(cells * (done_steps - it_tuned))) / run_b / 1e9
+    bytes_b = float(np.sum(step_bytes * (done_steps - it_tuned)))
     p50_b, p99_b = percentiles(done_t * 1e3)
     log(f"batched: {wall_b:.2f} s ({t_build:.1f} s host build+copy outside), {g_batch:.2f} Gcell/s, "
         f"p50 {p50_b:.0f} ms p99 {p99_b:.0f} ms")
     b.close()
 
-    # plain: the same solves one at a time (each its own graph-replayed run)
+    # plain: the same solves one at a time, each a batch of one tuned by the same
+    # autotuner over the same candidate launch shapes; its throughput counts the steps
+    # after tuning over the time after tuning (the tuning is left out, in the plain
+    # runs' favour), its latency is the queue's wall clock
     t_start = time.perf_counter()
     busy = 0.0
     lat_p = []
     nplain = len(descs) if args.plain_count <= 0 else min(args.plain_count, len(descs))
     work_p = 0.0
+    shapes = {}
     for j, d in enumerate(descs[:nplain]):
         if j % 32 == 0:
             log(f"plain: solve {j} of {nplain}, {busy:.1f} s so far")
-        h = Handle(d, lib)
-        h.lib.check(h.lib.model_upload(h.h), "upload")
+        one = Batch([d])
+        t_one = one.autotune(trial_steps=8)
+        key = (t_one["zchunk"], t_one["rows"], t_one["persistent"])
+        shapes[key] = shapes.get(key, 0) + 1
+        h = one.handles[0]
+        it0 = h.it
         t1 = time.perf_counter()
         try:
-            h.step(d.Nt)
+            while one.active > 0:
+                one.run(0)
         except _lib.EngineError as e:
             log(f"plain: {e}")
         busy += time.perf_counter() - t1
         lat_p.append((time.perf_counter() - t_start) * 1e3)
-        work_p += float(np.prod(d.dims)) * h.it
-        h.close()
+        work_p += float(np.prod(d.dims)) * (h.it - it0)
+        one.close()
     g_plain = work_p / busy / 1e9
     p50_p, p99_p = percentiles(lat_p)
     log(f"plain ({nplain} solves): {busy:.2f} s GPU-busy, {g_plain:.2f} Gcell/s, p50 {p50_p:.0f} ms "
         f"p99 {p99_p:.0f} ms")
-    out = base_record(args, g_batch, wall_b, int(max(d.Nt for d in descs)))
+    out = base_record(args, g_batch, run_b, int(max(d.Nt for d in descs)))
     out["config"] = {"workload": f"batched_2d: {len(descs)} uneven 2D instances, Nx,Nz in "
                                  f"256..2048, Lorentz rod + dielectrics, steps 2000..10000"
                                  + (f" x {args.step_scale}" if args.step_scale != 1.0 else ""),
                      "instances": len(descs), "cells": int(cells.sum()), "cell_updates": work,
-                     "autotune": tune}
+                     "autotune": tune, "autotune_s": round(t_tune, 3)}
+    achieved = bytes_b / run_b / 1e9
+    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                       "kernel": "whole batch: algorithmic bytes of every solve's steps (base, "
+                                 "CPML psi, Lorentz state) over the batch's wall time after tuning"}
     out["batched"] = {"gcells_per_s": round(g_batch, 3), "wall_s": round(wall_b, 3),
                       "p50_ms": round(p50_b, 1), "p99_ms": round(p99_b, 1),
                       "latency": "batch start -> solve finished, all submitted at t=0, "
                                  f"observed every {args.chunk} steps"}
     out["plain"] = {"gcells_per_s": round(g_plain, 3), "solves": nplain, "busy_s": round(busy, 3),
                     "p50_ms": round(p50_p, 1), "p99_ms": round(p99_p, 1),
-                    "latency": "queue of all solves at t=0 run one after another"}
+                    "latency": "queue of all solves at t=0 run one after another",
+                    "launch_shapes": {f"zchunk={k[0]},rows={k[1]},persistent={k[2]}": v
+                                      for k, v in sorted(shapes.items())},
+                    "tuning": "each solve autotuned over the batch's candidate shapes; "
+                              "throughput over the steps after tuning"}
     out["speedup_vs_plain"] = round(g_batch / g_plain, 3)
     out["nonfinite_solves"] = failed
-    out["roofline"] = None
     return out
 
 
 def bench_served_3d(args, lib):
     from maxwellfdtd_amd.batch import Server, replay
     from maxwellfdtd_amd.models import Handle
-    from maxwellfdtd_amd.workloads import served_3d
+    from maxwellfdtd_amd.workloads import algorithmic_bytes, served_3d
 
     t0 = time.perf_counter()
-    reqs = [(t_arr, m.describe()) for t_arr, m in
-            served_3d(count=args.count, smin=max(2, int(1000 * args.step_scale)),
-                      smax=max(2, int(4000 * args.step_scale)))]
+    reqs, job_bytes = [], 0.0
+    for k, (t_arr, m) in enumerate(served_3d(count=args.count, smin=max(2, int(1000 * args.step_scale)),
+                                             smax=max(2, int(4000 * args.step_scale)))):
+        job_bytes += algorithmic_bytes(m) * m.Nt
+        reqs.append((t_arr, m.describe()))
+        if k % 32 == 31:
+            log(f"built {k + 1} of {args.count} requests")
     cells = np.array([int(np.prod(d.dims)) for _, d in reqs], dtype=float)
     work = float(np.sum(cells * np.array([d.Nt for _, d in reqs])))
     log(f"{len(reqs)} requests built in {time.perf_counter()-t0:.1f}s: {work/1e12:.3f} T cell-updates, "
         f"arrivals over {reqs[-1][0]:.2f} s")
 
     srv = Server(options={8: args.quantum, 9: args.fill, 13: args.sched})
+    ndone = [0]
+
+    def on_done(ticket, handle, lat_ms):
+        ndone[0] += 1
+        if ndone[0] % 16 == 0:
+            log(f"served: {ndone[0]} of {len(reqs)} requests done after {time.perf_counter() - t0:.1f} s")
+
     t0 = time.perf_counter()
-    replay(srv, reqs, describe=lambda d: d)
+    replay(srv, reqs, describe=lambda d: d, on_done=on_done)
     wall_s = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
@@ -417,7 +454,9 @@ def bench_served_3d(args, lib):
     # plain: one request at a time in arrival order (a FIFO queue in front of a single
     # solver); service times measured, waiting derived from the arrival times
     clock, busy, lat_p = 0.0, 0.0, []
-    for t_arr, d in ([] if args.no_plain else reqs):
+    for j, (t_arr, d) in enumerate([] if args.no_plain else reqs):
+        if j % 16 == 0:
+            log(f"plain: request {j} of {len(reqs)}, {busy:.1f} s busy so far")
         h = Handle(d, lib)
         h.lib.check(h.lib.model_upload(h.h), "upload")
         t1 = time.perf_counter()
@@ -439,7 +478,12 @@ def bench_served_3d(args, lib):
     out["served"] = {"gcells_per_s": round(st["gcells_per_s"], 3), "p50_ms": round(st["p50_ms"], 1),
                      "p99_ms": round(st["p99_ms"], 1), "mean_ms": round(st["mean_ms"], 1),
                      "done": st["done"], "failed": st["failed"], "busy_s": round(st["busy_s"], 3)}
-    out["roofline"] = None
+    achieved = job_bytes / st["busy_s"] / 1e9 if st["busy_s"] > 0 else 0.0
+    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                       "kernel": "whole trace: algorithmic bytes of every request's steps (base, "
+                                 "CPML psi, plasma state) over the server's busy time (first "
+                                 "admission to last completion)"}
     if args.no_plain:
         return out
     span = clock - reqs[0][0]
@@ -560,7 +604,7 @@ def main():
     if args.steps is None:
         args.steps = 2000
     if args.count is None:
-        args.count = 512 if args.workload == "batched_2d" else 32
+        args.count = 512 if args.workload == "batched_2d" else 256
 
     from maxwellfdtd_amd import _lib
     lib = _lib.engine()

@@ -259,3 +259,35 @@ def bytes_per_cell_update(model):
     if model.grid.ndims == 2:
         return 17 + 8 * slab, 29 + 8 * slab
     return 13 + 8 * slab, 17 + 8 * slab
+
+
+def _listed(chi):
+    if chi is None:
+        return []
+    return list(chi) if isinstance(chi, (list, tuple)) else [chi]
+
+
+def algorithmic_bytes(model):
+    """Algorithmic HBM bytes of one step of `model` (FP32 state, uint8 id, perfect
+    neighbour reuse): the base of the H and E halves and the CPML psi pairs
+    (bytes_per_cell_update), plus per material cell the E^n read of a stateful cell and
+    its pole state (Debye 8 B per component, Drude/Lorentz 16 B: P and P^(n-1) read and
+    written) and the plasma state (rho and drho FP64 read and written 32 B, Pp 16 B and
+    Pa 8 B per component, the JE accumulator 8 B).  Kerr adds no state.  Returns bytes
+    per step for the whole grid."""
+    N = int(np.prod(model.grid.dims3))
+    bh, be = bytes_per_cell_update(model)
+    total = (bh + be) * N
+    nc = {1: 1, 2: 2, 3: 3}[model.grid.ndims]
+    counts = np.bincount(np.asarray(model.geometry).ravel(), minlength=len(model.materials) + 1)
+    for k, mat in enumerate(model.materials, start=1):
+        chis = _listed(mat.chi)
+        extra = 0.0
+        if chis or mat.plasma is not None:
+            extra += 4 * nc                                   # E^n, read once
+        for c in chis:
+            extra += (8 if type(c).__name__ == "DebyeSusceptibility" else 16) * nc
+        if mat.plasma is not None:
+            extra += 32 + (16 + 8) * nc + 8
+        total += extra * counts[k]
+    return float(total)
