@@ -165,7 +165,8 @@ struct DevModel {
     const DevSource *src;
     const DevMon *mon;
     int dft_block;              // DFT time-block length
-    const int *touched;         // cells written by a source (integral fixup list)
+    const int *touched;         // cells written by a source (integral fixup list); bit 31
+                                // (TOUCHED_STATEFUL): the cell's material has ADE/plasma state
     int ntouched;
     const DevSource *xtfsf;     // TFSF source whose x faces the E kernel finishes, or null
     int dless;                  // plain cells keep no D: D = E/Me is rebuilt on demand
@@ -212,6 +213,7 @@ __device__ __forceinline__ double fld_get64(const DevModel &m, int c, long long 
 }
 
 constexpr unsigned char ID_MASK = 0x7f, ID_TOUCHED = 0x80;
+constexpr int TOUCHED_STATEFUL = (int)0x80000000u, TOUCHED_CELL = 0x7fffffff;
 // E-kernel material modes: vacuum only, coefficient table only, full (ADE/Kerr/plasma)
 enum { MODE_VAC = 0, MODE_SIMPLE = 1, MODE_FULL = 2 };
 

@@ -247,8 +247,15 @@ int Model::upload() {
         unsigned char *did;
         UP(upload_vec(arena, &did, ids, io));
         m.id = did;
+        // the device list carries, in bit 31, whether the cell's material has ADE or
+        // plasma state, so that the face kernel needs no id and material lookups
+        std::vector<int> tdev(touched);
+        for (int &c : tdev) {
+            const int g = geom[c];
+            if (g > 0 && (mats[g - 1].d.nq > 0 || mats[g - 1].d.isplasma)) c |= TOUCHED_STATEFUL;
+        }
         int *dt_;
-        UP(upload_vec(arena, &dt_, touched, io));
+        UP(upload_vec(arena, &dt_, tdev, io));
         m.touched = dt_;
         m.ntouched = (int)touched.size();
         if (stateful) {

@@ -315,15 +315,13 @@ __global__ __launch_bounds__(256) void k_tfsf_gather(const DevModel *__restrict_
     float exf = 0.0f;
     if (active) {
         const DevSource &S = m.src[slot];
-        const unsigned i = (unsigned)G(m.touched)[k];
+        const int tk = G(m.touched)[k];
+        const unsigned i = (unsigned)(tk & TOUCHED_CELL);
         const int Nx = m.Nx, Ny = m.Ny;
         const int c3[3] = {(int)(i % (unsigned)Nx), (int)((i / (unsigned)Nx) % (unsigned)Ny),
                            (int)(i / ((unsigned)Nx * (unsigned)Ny))};
         bool xh = false;
-        if (m.xtfsf) {
-            const int g = G(m.id)[i] & ID_MASK;
-            xh = tfsf_xhandled(S, c3[0], c3[1], c3[2], (m.mats[g].flags & (MAT_DISP | MAT_PLASMA)) != 0);
-        }
+        if (m.xtfsf) xh = tfsf_xhandled(S, c3[0], c3[1], c3[2], (tk & TOUCHED_STATEFUL) != 0);
         float f[9];
         if (xh) {
             f[HY] = G((const float *)m.F[HY])[i];
@@ -597,7 +595,7 @@ __global__ __launch_bounds__(256) void k_integrals_touched(const DevModel *__res
     const float dt = m.dtf;
     float *Sa = m.Sa, *E2 = m.E2;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m.ntouched; k += gridDim.x * blockDim.x) {
-        const long long i = m.touched[k];
+        const long long i = m.touched[k] & TOUCHED_CELL;
         if (ndims == 3) {
             const float hx = m.F[HX][i], hy = m.F[HY][i], hz = m.F[HZ][i];
             const float ex = m.F[EX][i], ey = m.F[EY][i], ez = m.F[EZ][i];
