@@ -207,16 +207,22 @@ def bench_3d(args, lib):
     # stream, eager launches (no graph)
     h_ms, e_ms, cnt = kernel_timing(lib, h, args.timing_steps)
     bh, be = bytes_per_cell_update(model)
-    log(f"H kernel {h_ms*1e3:.1f} us ({N*bh/h_ms/1e6:.0f} GB/s), E kernel {e_ms*1e3:.1f} us "
-        f"({N*be/e_ms/1e6:.0f} GB/s) over {cnt} launches")
+    fused = args.kform == 3
+    if fused:
+        # one launch does both half-steps (the second timing window holds only the state
+        # pass of ADE/plasma cells, empty here)
+        log(f"fused step kernel {h_ms*1e3:.1f} us ({N*(bh+be)/h_ms/1e6:.0f} GB/s at {bh+be:.3f} B/cell) "
+            f"over {cnt} launches")
+    else:
+        log(f"H kernel {h_ms*1e3:.1f} us ({N*bh/h_ms/1e6:.0f} GB/s), E kernel {e_ms*1e3:.1f} us "
+            f"({N*be/e_ms/1e6:.0f} GB/s) over {cnt} launches")
     h.close()
 
-    achieved = N * be / (e_ms * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(HERE, "profiles", "traffic_plane_wave_spheres_3d_256.json")
     if os.path.exists(tpath) and args.n == 256:
         with open(tpath) as f:
-            traffic = json.load(f).get("e_kernel_bytes_per_launch")
+            traffic = json.load(f).get("fused_kernel_bytes_per_launch" if fused else "e_kernel_bytes_per_launch")
 
     cpu = parity = None
     if not args.no_cpu:
@@ -230,13 +236,22 @@ def bench_3d(args, lib):
                      "zchunk": args.zchunk, "rows_per_workgroup": args.ty, "kernel_form": args.kform,
                      "snake_order": args.snake, "plain_cells_without_d": args.dless,
                      "cpml_split": args.pml_split, "zchunk_h": args.zchunk_h, "xcd_chunked_h": args.xcd_h, "xcd_block": args.xcd_block}
-    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                       "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                       "traffic": traffic,
-                       "kernel": "k_e3d_p (E half-step)", "bytes_per_cell": round(be, 3),
-                       "avg_launch_us": round(e_ms * 1e3, 2),
-                       "h_kernel": {"avg_launch_us": round(h_ms * 1e3, 2), "bytes_per_cell": round(bh, 3),
-                                    "achieved": round(N * bh / h_ms / 1e6, 1)}}
+    if fused:
+        achieved = N * (bh + be) / (h_ms * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                           "traffic": traffic,
+                           "kernel": "k_step3d_f (fused H+E step)", "bytes_per_cell": round(bh + be, 3),
+                           "avg_launch_us": round(h_ms * 1e3, 2)}
+    else:
+        achieved = N * be / (e_ms * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                           "traffic": traffic,
+                           "kernel": "k_e3d_p (E half-step)", "bytes_per_cell": round(be, 3),
+                           "avg_launch_us": round(e_ms * 1e3, 2),
+                           "h_kernel": {"avg_launch_us": round(h_ms * 1e3, 2), "bytes_per_cell": round(bh, 3),
+                                        "achieved": round(N * bh / h_ms / 1e6, 1)}}
     out["step_bandwidth_GBs"] = round(N * (bh + be) * args.steps / wall / 1e9, 1)
     out["cpu_baseline"] = cpu
     out["parity"] = parity
@@ -273,7 +288,15 @@ def bench_dispersive_3d(args, lib):
     bh = 37 + 16 * s
     be = b_cell - bh
     h.close()
-    achieved = N * be / (e_ms * 1e-3) / 1e9
+    fused = args.kform == 3
+    if fused:
+        # the fused launch does H and E of every cell; the state pass finishes E of the
+        # ADE cells (its poles are part of b_cell): the two windows together
+        achieved = N * b_cell / ((h_ms + e_ms) * 1e-3) / 1e9
+        log(f"fused step kernel {h_ms*1e3:.1f} us + state pass {e_ms*1e3:.1f} us "
+            f"({achieved:.0f} GB/s at {b_cell:.3f} B/cell)")
+    else:
+        achieved = N * be / (e_ms * 1e-3) / 1e9
     cpu = parity = None
     if not args.no_cpu:
         cpu, parity = cpu_leg(desc, args.cpu_seconds, N, "384x384x192 dispersive", lib)
@@ -283,12 +306,19 @@ def bench_dispersive_3d(args, lib):
                                  "substrate, 384x384 DFT plane at 16 frequencies",
                      "cells": N, "model_steps": desc.Nt, "f_metal": round(f_metal, 5),
                      "f_sub": round(f_sub, 5), "bytes_per_cell_update": round(b_cell, 3)}
-    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                       "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                       "kernel": "k_e3d_p (E half-step, MODE_FULL)", "bytes_per_cell": round(be, 3),
-                       "avg_launch_us": round(e_ms * 1e3, 2),
-                       "h_kernel": {"avg_launch_us": round(h_ms * 1e3, 2),
-                                    "bytes_per_cell": round(bh, 3)}}
+    if fused:
+        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                           "kernel": "k_step3d_f (fused H+E step, MODE_FULL) + k_e_state",
+                           "bytes_per_cell": round(b_cell, 3), "avg_launch_us": round((h_ms + e_ms) * 1e3, 2),
+                           "fused_us": round(h_ms * 1e3, 2), "state_pass_us": round(e_ms * 1e3, 2)}
+    else:
+        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                           "kernel": "k_e3d_p (E half-step, MODE_FULL)", "bytes_per_cell": round(be, 3),
+                           "avg_launch_us": round(e_ms * 1e3, 2),
+                           "h_kernel": {"avg_launch_us": round(h_ms * 1e3, 2),
+                                        "bytes_per_cell": round(bh, 3)}}
     out["step_bandwidth_GBs"] = round(N * b_cell * steps / wall / 1e9, 1)
     out["cpu_baseline"] = cpu
     out["parity"] = parity
@@ -578,7 +608,8 @@ def main():
     ap.add_argument("--xcd-h", type=int, default=1,
                     help="H tiles in per-XCD contiguous chunks (1) or in grid order (0)")
     ap.add_argument("--ty", type=int, default=4)
-    ap.add_argument("--kform", type=int, default=2)
+    ap.add_argument("--kform", type=int, default=3,
+                    help="3D kernel form: 3 fused step (default), 2 two-pass, 0 one cell per thread")
     ap.add_argument("--snake", type=int, default=1)
     ap.add_argument("--dless", type=int, default=1, help="plain cells keep no D (default 1)")
     ap.add_argument("--pml-split", type=int, default=0,

@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmaxwellfdtd_amd.so")
 PROBE_LIB = os.path.join(HERE, "libmfd_hbm_probe.so")
-SOURCES = ["kernels3d.hip", "kernels3d_p.hip", "kernels2d.hip", "kernels1d.hip", "step_kernels.hip",
+SOURCES = ["kernels3d.hip", "kernels3d_p.hip", "kernels3d_f.hip", "kernels2d.hip", "kernels1d.hip", "step_kernels.hip",
            "kernels_state.hip", "energy.hip", "engine.cpp", "batch.cpp"]
 PROBE_SOURCES = ["hbm_probe.hip"]
 
@@ -26,9 +26,17 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=f
          "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable"]
 
 
+# sources that must not have multiplies and adds fused behind their back: the fused step
+# writes its fused multiply-adds out, so that a cell's H recomputed by a neighbouring
+# tile rounds exactly as the owner's (-ffp-contract=fast would let the backend fuse other
+# pairs, differently in each kernel instantiation)
+NO_CONTRACT = {"kernels3d_f.hip"}
+
+
 def _compile(src):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+    flags = [("-ffp-contract=off" if f == "-ffp-contract=fast" and src in NO_CONTRACT else f) for f in FLAGS]
+    cmd = [HIPCC, *flags, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")

@@ -42,7 +42,9 @@ enum { MAT_DISP = 1, MAT_KERR = 2, MAT_PLASMA = 4 };
 enum { SRC_SOFT = 0, SRC_HARD = 1, SRC_TFSF = 2 };
 
 // One CPML face layer: profiles from LeftPMLLayer/RightPMLLayer (src/pml.jl:57-127)
-// and the psi arrays of that face (src/pml.jl:221-304), FP32.
+// and the psi arrays of that face (src/pml.jl:221-304), FP32.  In an FP32 model the two
+// faces of an axis share each psi allocation: the upper face's array starts right after
+// the lower face's.
 struct DevLayer {
     int present;
     int ib;        // 0-based boundary index: left layer covers i <= ib, right i >= ib
@@ -52,6 +54,7 @@ struct DevLayer {
     const float *Ki;        // 1/K (rounded once from FP64), for the pipelined 3D kernels
     float *psiE0, *psiE1;   // H half-step psi (x: Eyx,Ezx  y: Exy,Ezy  z: Exz,Eyz)
     float *psiH0, *psiH1;   // E half-step psi (x: Hyx,Hzx  y: Hxy,Hzy  z: Hxz,Hyz)
+    float *psiE0b, *psiE1b; // fused 3D step: the other buffer of the H half-step psi
 };
 
 // Material coefficient table entry (index 0 = vacuum) with the material's compact
@@ -178,6 +181,10 @@ struct DevModel {
     const DevMat64 *mats64;
     double idz64;               // 1/dz in FP64
     double *Sa64, *E2_64;       // the integral variables of an FP64 model (Sa, E2 null)
+    // fused 3D step: the other buffer of H and E (Falt[HX..HZ], Falt[EX..EZ]); each step
+    // reads F and writes Falt, and a launch after it swaps the two (and the H-half psi
+    // pairs of the layers)
+    float *Falt[9];
 };
 
 // The field arrays and material table of a model in precision T (float: F, mats;
@@ -228,6 +235,7 @@ constexpr int TX3 = 64, TY3 = 4;    // 3D scalar kernels: 64 x 4 columns, marchi
 constexpr int TX2 = 64;             // 2D: 64 x-columns per wave, TW waves per tile, marching z
 constexpr int TX1 = 256;            // 1D: 256 cells
 constexpr int XFACE_ZC = 64;        // longest tile z range with the E kernel's x-face epilogue
+constexpr int FUSED_XFACE_ZC = 32;  // the same for the fused step (its LDS holds R rows)
 
 // launchers (defined in the .hip files)
 void launch_h3d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, hipStream_t s);
@@ -237,6 +245,11 @@ void launch_h3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, in
                   hipStream_t s);
 void launch_e3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int ty,
                   bool pml, hipStream_t s);
+// fused 3D step (one z march per step over ping-pong H/E buffers), rows = 2 or 4;
+// pmask: CPML axes the tiles reach (0 none, 1 x only, 7 any)
+void launch_step3d_f(DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int rows,
+                     int pmask, hipStream_t s);
+void launch_swap_pp(DevModel *m, int nmodels, hipStream_t s);
 void launch_h2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int tw, hipStream_t s);
 void launch_e2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int tw, hipStream_t s);
 void launch_h1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool f64,

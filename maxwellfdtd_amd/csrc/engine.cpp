@@ -79,6 +79,12 @@ int64_t Model::footprint() const {
     }
     if (int_flags & MFD_INT_SA) b += N * 4;
     if (int_flags & MFD_INT_E2) b += N * 4;
+    // the fused 3D step's second H/E buffers and H-half psi arrays
+    if (d.ndims == 3 && !f64) {
+        b += 6 * N * 4;
+        for (int f = 0; f < 6; f++)
+            if (L[f].present) b += (int64_t)2 * psi_len(*this, f, L[f].nb) * 4;
+    }
     return b;
 }
 
@@ -397,11 +403,25 @@ int Model::upload() {
         UP(upload_vec(arena, &p, ki, io)); l.Ki = p;
         UP(upload_vec(arena, &p, to_f32(L[f].A), io)); l.A = p;
         UP(upload_vec(arena, &p, to_f32(L[f].B), io)); l.B = p;
-        for (int k = 0; k < 4; k++) {
-            UP(arena.alloc(dst[k], n));
-            UP(hipMemsetAsync(*dst[k], 0, n * sizeof(float), io));
-        }
     }
+    // FP32 psi arrays: the two faces of an axis share one allocation, the upper face's
+    // array right after the lower face's, so a kernel can address either from one base
+    if (!f64)
+        for (int a = 0; a < 3; a++) {
+            DevLayer &l0 = m.L[2 * a], &l1 = m.L[2 * a + 1];
+            const size_t n0 = l0.present ? psi_len(*this, 2 * a, l0.nb) : 0;
+            const size_t n1 = l1.present ? psi_len(*this, 2 * a + 1, l1.nb) : 0;
+            if (n0 + n1 == 0) continue;
+            float **d0[4] = {&l0.psiE0, &l0.psiE1, &l0.psiH0, &l0.psiH1};
+            float **d1[4] = {&l1.psiE0, &l1.psiE1, &l1.psiH0, &l1.psiH1};
+            for (int k = 0; k < 4; k++) {
+                float *q;
+                UP(arena.alloc(&q, n0 + n1));
+                UP(hipMemsetAsync(q, 0, (n0 + n1) * sizeof(float), io));
+                if (l0.present) *d0[k] = q;
+                if (l1.present) *d1[k] = q + n0;
+            }
+        }
     // sources
     hsrc.assign(srcs.size(), DevSource{});
     maxsrc_n = 1;
@@ -577,9 +597,53 @@ int Model::upload() {
     dm = m;
     it = 0;
     dft_done = 0;
+    pp_alloc = false;
+    pp_parity = 0;
     status = MFD_OK;
     uploaded = true;
     return MFD_OK;
+}
+
+// The second H/E buffers and H-half psi arrays of the fused 3D step, allocated the first
+// time a runner steps the model in that form.  Each step writes every cell of them, so
+// their initial contents never reach a result; they start zeroed all the same.
+int Model::ensure_pp() {
+    if (pp_alloc) return MFD_OK;
+    const int cs[6] = {HX, HY, HZ, EX, EY, EZ};
+    for (int c : cs) {
+        MFD_HIP(arena.alloc(&dm.Falt[c], (size_t)N));
+        MFD_HIP(hipMemsetAsync(dm.Falt[c], 0, (size_t)N * sizeof(float), io));
+    }
+    // paired per axis like the psi arrays of upload()
+    for (int a = 0; a < 3; a++) {
+        DevLayer &l0 = dm.L[2 * a], &l1 = dm.L[2 * a + 1];
+        const size_t n0 = l0.present ? psi_len(*this, 2 * a, l0.nb) : 0;
+        const size_t n1 = l1.present ? psi_len(*this, 2 * a + 1, l1.nb) : 0;
+        if (n0 + n1 == 0) continue;
+        float **d0[2] = {&l0.psiE0b, &l0.psiE1b};
+        float **d1[2] = {&l1.psiE0b, &l1.psiE1b};
+        for (int k = 0; k < 2; k++) {
+            float *q;
+            MFD_HIP(arena.alloc(&q, n0 + n1));
+            MFD_HIP(hipMemsetAsync(q, 0, (n0 + n1) * sizeof(float), io));
+            if (l0.present) *d0[k] = q;
+            if (l1.present) *d1[k] = q + n0;
+        }
+    }
+    MFD_HIP(hipStreamSynchronize(io));
+    pp_alloc = true;
+    return MFD_OK;
+}
+
+// Host mirror of the buffer swap the fused step's last tile makes on the device.
+void Model::swap_pp() {
+    const int cs[6] = {HX, HY, HZ, EX, EY, EZ};
+    for (int c : cs) std::swap(dm.F[c], dm.Falt[c]);
+    for (int f = 0; f < 6; f++) {
+        std::swap(dm.L[f].psiE0, dm.L[f].psiE0b);
+        std::swap(dm.L[f].psiE1, dm.L[f].psiE1b);
+    }
+    pp_parity ^= 1;
 }
 
 // D of the plain cells after runs without stored D: rebuilt as E/Me in place.
@@ -706,9 +770,9 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     int dev = 0;
     MFD_HIP(hipGetDevice(&dev));
     MFD_HIP(hipDeviceGetAttribute(&cu_count, hipDeviceAttributeMultiprocessorCount, dev));
-    std::vector<DevModel> hd;
+    hd.clear();
     std::vector<Tile> tiles, tiles_h;
-    any_mats = any_full = any_int = any_mon = any_spectral = h_mats = false;
+    any_mats = any_full = any_int = any_mon = any_spectral = h_mats = any_pml = false;
     maxtouched = 1;
     maxsrc = 0;
     maxsrc_n = maxmon_n = maxface = 1;
@@ -716,7 +780,7 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     long long maxN = 1;
     kform = ndims == 3 ? opt.vec : 0;
     for (Model *m : ms) {
-        if (kform == 2 && m->N >= (1LL << 30)) kform = 0;   // 32-bit cell offsets
+        if (kform >= 2 && m->N >= (1LL << 30)) kform = 0;   // 32-bit cell offsets
     }
     f64 = ms[0]->f64;
     for (size_t k = 0; k < ms.size(); k++) {
@@ -725,10 +789,8 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
         if (m->f64 != f64) { set_error("a run mixes FP32 and FP64 models"); return MFD_ERR_ARG; }
         int st = m->upload();
         if (st != MFD_OK) return st;
-        hd.push_back(m->dm);
-        make_tiles(*m, (int)k, opt.zchunk, opt.ty, kform, tiles);
-        if (ndims == 3 && kform == 2 && opt.zchunk_h > 0) make_tiles(*m, (int)k, opt.zchunk_h, opt.ty, kform, tiles_h);
         any_mats |= m->has_mats;
+        for (int f = 0; f < 6; f++) any_pml |= m->L[f].present;
         for (const auto &hm : m->mats) {
             any_full |= hm.d.nq > 0 || hm.d.iskerr || hm.d.isplasma;
             // the H half needs the material id only where mu differs from vacuum
@@ -745,9 +807,35 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
         maxN = std::max(maxN, m->N);
         if (m->dft_block != dft_block) { set_error("DFT block mismatch"); return MFD_ERR_ARG; }
     }
+    // the fused step (kform 3) covers FP32 3D models with mu = 1 everywhere, at least two
+    // cells per axis and no cell inside both CPML layers of one axis; otherwise the
+    // pipelined two-pass form runs
+    if (kform == 3) {
+        bool ok = ndims == 3 && !f64 && !h_mats;
+        for (Model *m : ms) {
+            ok &= m->d.Nx >= 2 && m->d.Ny >= 2 && m->d.Nz >= 2;
+            for (int a = 0; a < 3; a++)
+                if (m->L[2 * a].present && m->L[2 * a + 1].present) ok &= m->L[2 * a + 1].ib > m->L[2 * a].ib;
+        }
+        if (!ok) kform = 2;
+    }
+    fused = kform == 3;
+    frows = (opt.ty == 2) ? 2 : 4;
+    for (size_t k = 0; k < ms.size(); k++) {
+        Model *m = ms[k];
+        if (fused) {
+            int st = m->ensure_pp();
+            if (st != MFD_OK) return st;
+        }
+        hd.push_back(m->dm);
+        make_tiles(*m, (int)k, opt.zchunk, fused ? frows : opt.ty, kform, tiles);
+        if (ndims == 3 && kform == 2 && opt.zchunk_h > 0) make_tiles(*m, (int)k, opt.zchunk_h, opt.ty, kform, tiles_h);
+    }
+    pp_seen.assign(ms.size(), 0);
+    for (size_t k = 0; k < ms.size(); k++) pp_seen[k] = ms[k]->pp_parity;
     // cells of stateful materials get their E from the state pass on the pipelined 3D
     // kernels and the 2D kernels; the other forms update them inline
-    split_state = (ndims == 3 && kform == 2) || ndims == 2;
+    split_state = (ndims == 3 && kform >= 2) || ndims == 2;
     std::vector<int4> chunks;
     if (split_state)
         for (size_t k = 0; k < ms.size(); k++)
@@ -770,7 +858,7 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
         for (const DevSource &v : m->hsrc) aux_needed |= v.kind == SRC_TFSF && v.mode == 1 && !v.auxtab;
     }
     // the pipelined 3D E kernel finishes TFSF x-face cells itself
-    const bool xface = tfsf_gather && kform == 2 && opt.zchunk <= XFACE_ZC;
+    const bool xface = tfsf_gather && kform >= 2 && opt.zchunk <= (fused ? FUSED_XFACE_ZC : XFACE_ZC);
     if (xface)
         for (size_t k = 0; k < ms.size(); k++) hd[k].xtfsf = ms[k]->dm.src;
     dless_models.clear();
@@ -779,7 +867,7 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     // ADE, plasma or source cell (and, in 3D, on the TFSF faces the x-face path finishes)
     for (size_t k = 0; k < ms.size(); k++) {
         Model *m = ms[k];
-        const bool ok = opt.dless && !m->user_set && ((ndims == 3 && kform == 2) || ndims == 2);
+        const bool ok = opt.dless && !m->user_set && ((ndims == 3 && kform >= 2) || ndims == 2);
         if (!ok) {
             int st = m->materialize_d();
             if (st != MFD_OK) return st;
@@ -799,7 +887,9 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     MFD_HIP(hipHostMalloc(&h_flags, hd.size() * sizeof(int)));
     MFD_HIP(copy_sync(d_models, hd.data(), hd.size() * sizeof(DevModel), hipMemcpyHostToDevice, stream));
     // pipelined 3D: tiles that no CPML layer reaches go to the slab-free kernels
-    const bool split = ndims == 3 && kform == 2 && opt.pml_split > 0;
+    // the fused step always splits them (the CPML instantiation needs far more registers)
+    // and runs the two launches side by side
+    const bool split = ndims == 3 && ((kform == 2 && opt.pml_split > 0) || (fused && any_pml && opt.pml_split > 0));
     if (split && opt.pml_split == 2) {
         MFD_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
         MFD_HIP(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
@@ -810,12 +900,22 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
             bool in = !split;
             if (split) {
                 const DevModel &dm = hd[t.x];
-                const int x1 = std::min(dm.Nx, t.y + 64) - 1, y1 = std::min(dm.Ny, t.z + opt.ty) - 1;
+                const int x1 = std::min(dm.Nx, t.y + 64) - 1, y1 = std::min(dm.Ny, t.z + (fused ? frows : opt.ty)) - 1;
                 const int z0 = tile_z0(t), z1 = tile_z1(t) - 1;
                 const DevLayer *L = dm.L;
                 in = (L[0].present && t.y <= L[0].ib) || (L[1].present && x1 >= L[1].ib) ||
                      (L[2].present && t.z <= L[2].ib) || (L[3].present && y1 >= L[3].ib) ||
                      (L[4].present && z0 <= L[4].ib) || (L[5].present && z1 >= L[5].ib);
+                if (fused) {
+                    // the fused step also recomputes H one cell below the tile in x, y and z
+                    // (wrapping round under periodic faces): those cells count too
+                    auto halo_in = [&](int a, int c0, int n) {
+                        const int h = c0 > 0 ? c0 - 1 : (dm.bcs[2 * a] == BC_PERIODIC ? n - 1 : -1);
+                        if (h < 0) return false;
+                        return (L[2 * a].present && h <= L[2 * a].ib) || (L[2 * a + 1].present && h >= L[2 * a + 1].ib);
+                    };
+                    in = in || halo_in(0, t.y, dm.Nx) || halo_in(1, t.z, dm.Ny) || halo_in(2, z0, dm.Nz);
+                }
             }
             (in ? tl_pml : tl_int).push_back(t);
         }
@@ -860,9 +960,40 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     // plain grid order, which beats one chunk per XCD but not 64-tile blocks)
     std::vector<Tile> perm, rev;
     const bool hchunk = opt.xcd_h || ndims != 3;
-    order(th_pml, false, perm, hchunk);
-    ntiles_pml_h = (int)perm.size();
-    order(th_int, false, perm, hchunk);
+    if (fused) {
+        // fused step: three lists by the CPML axes a tile reaches, counting the cells one
+        // below it in x, y and z whose H it recomputes (wrapping round under periodic
+        // faces): x only (the x-slab columns, half the tiles of a 64-column tiling), any
+        // other mix, none.  Each list gets a kernel built for those axes.
+        std::vector<Tile> cl[3];
+        for (const Tile &t : tiles) {
+            const DevModel &dm = hd[t.x];
+            if (!split) { cl[1].push_back(t); continue; }   // one launch built for any axis
+            const int c0[3] = {t.y, t.z, tile_z0(t)};
+            const int c1[3] = {std::min(dm.Nx, t.y + 64) - 1, std::min(dm.Ny, t.z + frows) - 1, tile_z1(t) - 1};
+            const int n[3] = {dm.Nx, dm.Ny, dm.Nz};
+            int mask = 0;
+            for (int a = 0; a < 3; a++) {
+                const DevLayer &l0 = dm.L[2 * a], &l1 = dm.L[2 * a + 1];
+                const int h = c0[a] > 0 ? c0[a] - 1 : (dm.bcs[2 * a] == BC_PERIODIC ? n[a] - 1 : c0[a]);
+                auto hits = [&](int lo, int hi) {
+                    return (l0.present && lo <= l0.ib) || (l1.present && hi >= l1.ib);
+                };
+                if (hits(c0[a], c1[a]) || hits(h, h)) mask |= 1 << a;
+            }
+            cl[mask == 0 ? 2 : mask == 1 ? 0 : 1].push_back(t);
+        }
+        for (int c = 0; c < 3; c++) {
+            const size_t b = perm.size();
+            order(cl[c], false, perm, true);
+            nfc[c] = (int)(perm.size() - b);
+        }
+        ntiles_pml_h = nfc[0] + nfc[1];
+    } else {
+        order(th_pml, false, perm, hchunk);
+        ntiles_pml_h = (int)perm.size();
+        order(th_int, false, perm, hchunk);
+    }
     order(tl_int, opt.snake, rev);
     ntiles_int_e = (int)rev.size();
     order(tl_pml, opt.snake, rev);
@@ -895,7 +1026,19 @@ int Runner::launch_step(bool timing, bool dft_flush) {
     if (timing) MFD_HIP(hipEventRecord(ev[0], stream));
     const int mode = any_full ? MODE_FULL : any_mats ? MODE_SIMPLE : MODE_VAC;
     const int hmode = h_mats ? MODE_SIMPLE : MODE_VAC;
-    if (ndims == 3 && kform == 2) {
+    if (fused) {
+        // x-slab tiles on the side stream; CPML-free tiles, then the other CPML tiles, on
+        // the runner's stream, side by side; then the buffer swap
+        hipStream_t sp = side ? side : stream;
+        const int n1 = nfc[0], n7 = nfc[1], n0 = nfc[2];
+        if (side) { MFD_HIP(hipEventRecord(fork_ev, stream)); MFD_HIP(hipStreamWaitEvent(side, fork_ev, 0)); }
+        launch_step3d_f(d_models, d_tiles, n1, grid_of(n1), mode, any_int, frows, 1, sp);
+        launch_step3d_f(d_models, d_tiles + n1 + n7, n0, grid_of(n0), mode, any_int, frows, 0, stream);
+        launch_step3d_f(d_models, d_tiles + n1, n7, grid_of(n7), mode, any_int, frows, 7, stream);
+        if (side) { MFD_HIP(hipEventRecord(join_ev, side)); MFD_HIP(hipStreamWaitEvent(stream, join_ev, 0)); }
+        launch_swap_pp(d_models, nm, stream);
+    }
+    else if (ndims == 3 && kform == 2) {
         // side by side (MFD_OPT_PML_SPLIT = 2): the CPML tiles run on the side stream,
         // forked after the previous work on the runner's stream and joined before the next
         hipStream_t sp = side ? side : stream;
@@ -908,7 +1051,9 @@ int Runner::launch_step(bool timing, bool dft_flush) {
     else if (ndims == 2) launch_h2d(d_models, d_tiles, ntiles_h, grid_h, hmode, opt.ty, stream);
     else launch_h1d(d_models, d_tiles, ntiles_h, grid_h, hmode, f64, stream);
     if (timing) { MFD_HIP(hipEventRecord(ev[1], stream)); MFD_HIP(hipEventRecord(ev[2], stream)); }
-    if (ndims == 3 && kform == 2) {
+    if (fused) {
+        // the fused launch did the E half too
+    } else if (ndims == 3 && kform == 2) {
         hipStream_t sp = side ? side : stream;
         if (side) { MFD_HIP(hipEventRecord(fork_ev, stream)); MFD_HIP(hipStreamWaitEvent(side, fork_ev, 0)); }
         launch_e3d_p(d_models, d_tiles_e, nie, grid_of(nie), mode, any_int, opt.ty, false, stream);
@@ -944,12 +1089,36 @@ int Runner::launch_step(bool timing, bool dft_flush) {
     return MFD_OK;
 }
 
-// Host mirror of the device step bookkeeping (k_finish): the step counter, and the DFT
-// block position of a model with spectral monitors.
-static void host_advance(Model *m, int dft_block) {
+// Host mirror of the device step bookkeeping (k_finish): the step counter, the DFT block
+// position of a model with spectral monitors and, for the fused step, the buffer swap.
+void Runner::advance(size_t k) {
+    Model *m = models[k];
     if (m->it >= m->d.Nt) return;
     m->it++;
     if (m->it - m->dft_done == dft_block) m->dft_done = m->it;
+    if (fused) {
+        m->swap_pp();
+        pp_seen[k] = m->pp_parity;
+    }
+}
+
+// A model's buffers may have been swapped by another runner since this one uploaded its
+// descriptor (a model stepped alone, then in a batch, then alone again): take the
+// current pointers before launching.
+int Runner::refresh() {
+    for (size_t k = 0; k < models.size(); k++) {
+        Model *m = models[k];
+        if (pp_seen[k] == m->pp_parity) continue;
+        DevModel &h = hd[k];
+        for (int c = 0; c < 9; c++) { h.F[c] = m->dm.F[c]; h.Falt[c] = m->dm.Falt[c]; }
+        for (int f = 0; f < 6; f++) {
+            h.L[f].psiE0 = m->dm.L[f].psiE0; h.L[f].psiE1 = m->dm.L[f].psiE1;
+            h.L[f].psiE0b = m->dm.L[f].psiE0b; h.L[f].psiE1b = m->dm.L[f].psiE1b;
+        }
+        MFD_HIP(copy_sync(d_models + k, &h, sizeof(DevModel), hipMemcpyHostToDevice, stream));
+        pp_seen[k] = m->pp_parity;
+    }
+    return MFD_OK;
 }
 
 // Steps k (0-based) of the next `steps` steps at which some model's DFT block completes:
@@ -1016,6 +1185,10 @@ int Runner::prepare() {
 
 // nsteps steps of every model; models that already finished are skipped on the device.
 int Runner::run(int nsteps) {
+    {
+        int st = refresh();
+        if (st != MFD_OK) return st;
+    }
     for (Model *m : dless_models) m->d_stale = true;
     int done = 0;
     const bool timing = opt.kernel_timing;
@@ -1026,12 +1199,12 @@ int Runner::run(int nsteps) {
             if (st != MFD_OK) return st;
             MFD_HIP(hipGraphLaunch(gexec, stream));
             for (int k = 0; k < G; k++)
-                for (Model *m : models) host_advance(m, dft_block);
+                for (size_t q = 0; q < models.size(); q++) advance(q);
             done += G;
         } else {
             int st = launch_step(timing, (dft_mask(1) & 1) != 0);
             if (st != MFD_OK) return st;
-            for (Model *m : models) host_advance(m, dft_block);
+            for (size_t q = 0; q < models.size(); q++) advance(q);
             done += 1;
         }
     }
@@ -1100,7 +1273,10 @@ int set_option(RunOptions &o, Model *m, int opt, int value) {
             o.ty = value;
             break;
         case MFD_OPT_KERNEL_FORM:
-            if (value != 0 && value != 2) { set_error("3D kernel form must be 0 (one cell per thread) or 2 (pipelined z march)"); return MFD_ERR_ARG; }
+            if (value != 0 && value != 2 && value != 3) {
+                set_error("3D kernel form must be 0 (one cell per thread), 2 (pipelined two-pass z march) or 3 (fused step)");
+                return MFD_ERR_ARG;
+            }
             o.vec = value;
             break;
         case MFD_OPT_SNAKE: o.snake = value != 0; break;

@@ -111,12 +111,18 @@ struct Model {
     bool d_stale = false;       // D of plain cells not stored by the last runs
     bool d_tfsf = false;        // ... whose stored-D cells are the TFSF face cells
     hipStream_t io = nullptr;   // non-blocking stream for this model's copies
+    // fused 3D step: the other H/E buffers and H-half psi arrays exist (pp_alloc), and
+    // pp_parity counts the swaps mod 2 (dm always holds the current pointers)
+    bool pp_alloc = false;
+    int pp_parity = 0;
 
     int ncomp() const { return d.ndims == 3 ? 3 : d.ndims == 2 ? 2 : 1; }
     bool comp_present(int c) const;
     int64_t footprint() const;
     int ensure_io();
     int upload();
+    int ensure_pp();
+    void swap_pp();
     int sync();
     int materialize_d();
     void release();
@@ -129,7 +135,8 @@ struct RunOptions {
     int zchunk = 16;            // z planes per tile (3D/2D)
     int zchunk_h = 8;           // pipelined 3D H half: its own z planes per tile (0 = zchunk)
     int ty = 4;                 // rows per workgroup of the 3D vector kernels (1, 2, 4, 8)
-    int vec = 2;                // 3D kernel form: 0 scalar, 1 vector (Nx % 4 == 0), 2 pipelined
+    int vec = 3;                // 3D kernel form: 0 one cell per thread, 2 pipelined two-pass,
+                                // 3 fused step (falls back to 2 where it does not apply)
     int persistent = 0;         // 0: one workgroup per tile; >0: that many workgroups per CU
     bool kernel_timing = false; // hipEvent pairs around the H/E launches (eager mode)
     bool snake = true;          // E half walks each XCD's tile chunk backwards
@@ -158,9 +165,15 @@ struct Runner {
     int ndims = 3;
     bool any_mats = false, any_full = false, any_int = false, any_mon = false, any_spectral = false;
     bool h_mats = false;        // some material has mu != 1: the H half reads the id
+    bool any_pml = false;       // some model has a CPML layer
     std::vector<Model *> dless_models;   // models this runner steps without stored D
     int maxtouched = 1;
     int kform = 0;              // 3D kernel form in use
+    bool fused = false;         // kform 3: one launch per step over ping-pong H/E buffers
+    int frows = 2;              // fused step: rows per lane
+    int nfc[3] = {0, 0, 0};     // fused step: tiles reaching x slabs only, other slabs, none
+    std::vector<DevModel> hd;   // host copies of d_models
+    std::vector<int> pp_seen;   // per model: the buffer parity d_models holds
     bool f64 = false;           // the models keep their state in FP64 (1D)
     bool split_state = false;   // stateful cells' E computed by the state pass
     bool tfsf_gather = false;   // TFSF corrections + touched integrals in one launch
@@ -189,6 +202,8 @@ struct Runner {
     int init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_t s);
     void release();
     int launch_step(bool timing, bool dft_flush);   // one step of every model, on `stream`
+    void advance(size_t k);           // host mirror of one step of model k
+    int refresh();                    // d_models takes buffer swaps made by other runners
     uint64_t dft_mask(int steps) const;
     int graph_len() const;
     int build_graph(int steps, uint64_t mask);

@@ -1,0 +1,747 @@
+// 3D Yee step in one z march: update_H_kernel! and update_E_kernel! 3D
+// (src/models.jl:735-1175) fused into one launch.
+//
+// Each step reads the previous step's E and H from one buffer set and writes the new E
+// and H into the other (ping-pong), so a step reads and writes each field once instead
+// of twice.  A tile is 64 x-columns (one per lane) by R rows (every lane holds all R
+// rows in registers) by a z range.  Each plane is done in one pass: the new H of the
+// tile's cells at that plane, then their new E, which needs the new H of the plane
+// below (kept from the previous plane) and of the row below and the column to the left.
+// Those two belong to other workgroups running at the same time, so the tile computes
+// them again from the old buffers: the row below on every lane, the column to the left
+// on R lanes, and the plane below the tile in a first pass over plane z0-1.  That
+// recomputation runs the same code as the owning tile (fused multiply-adds written out,
+// the file built with -ffp-contract=off so the compiler fuses nothing else), so the
+// results do not depend on the tiling.
+//
+// The H-half CPML psi arrays are ping-ponged like the fields (the halo cells read the
+// old psi); the E-half psi, D, the integrals and the material state stay in place.
+// A one-thread-per-model launch right after this one swaps the buffer pointers in the
+// model descriptors, so every later launch of the step sees the new fields where it
+// always looks (a per-tile atomic count instead, to let the last tile swap, serialises
+// thousands of same-address atomics per step).
+//
+// Every load of a plane is issued at once (with Ex, Ey and the id of the plane above,
+// which this plane's H and the next plane's stored-D choice need), then the plane is
+// computed: a wave keeps R rows of loads in flight and few registers, so a SIMD holds
+// many waves.  All addressing is a scalar base plus a 32-bit byte offset (models below
+// 2^30 cells), and x neighbours move between lanes by DPP.
+#include "common.hpp"
+#include "material.hpp"
+#include "tfsf.hpp"
+
+#include <algorithm>
+
+
+namespace mfd {
+namespace {
+
+typedef const __attribute__((address_space(1))) char gcC;
+typedef __attribute__((address_space(1))) char gC;
+
+// global accesses at a byte offset from a base pointer
+template <class T>
+__device__ __forceinline__ T ldb(const void *base, unsigned off) {
+    return *(const __attribute__((address_space(1))) T *)((gcC *)base + off);
+}
+template <class T>
+__device__ __forceinline__ void stb(void *base, unsigned off, T v) {
+    *(__attribute__((address_space(1))) T *)((gC *)base + off) = v;
+}
+__device__ __forceinline__ float ldnt(const void *base, unsigned off) {
+    return __builtin_nontemporal_load((const __attribute__((address_space(1))) float *)((gcC *)base + off));
+}
+__device__ __forceinline__ void stnt(void *base, unsigned off, float v) {
+    __builtin_nontemporal_store(v, (__attribute__((address_space(1))) float *)((gC *)base + off));
+}
+
+__device__ __forceinline__ float rdlane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+// value of lane + 1 / lane - 1 (DPP wave shifts; the edge lane's result is not used)
+__device__ __forceinline__ float from_next(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_prev(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+
+// profile values of one CPML slab at one position: B, A and 1/K (src/pml.jl:68-81)
+struct Prof {
+    float B, A, Ki;
+};
+
+// CPML slab of axis a at coordinate c of that axis: face index (-1 none) and profile index
+__device__ __forceinline__ int slab_of(const DevModel &m, int a, int c, int &p) {
+    const DevLayer &L0 = m.L[2 * a], &L1 = m.L[2 * a + 1];
+    if (L0.present && c <= L0.ib) { p = c; return 2 * a; }
+    if (L1.present && c >= L1.ib) { p = c - L1.ib; return 2 * a + 1; }
+    p = 0;
+    return -1;
+}
+
+__device__ __forceinline__ Prof prof_of(const DevModel &m, int f, int p) {
+    Prof r{0.0f, 0.0f, 1.0f};
+    if (f >= 0) {
+        const DevLayer &L = m.L[f];
+        r.B = G(L.B)[p];
+        r.A = G(L.A)[p];
+        r.Ki = G(L.Ki)[p];
+    }
+    return r;
+}
+
+struct Hv {
+    float x, y, z;
+};
+
+// H half-step of one cell (src/models.jl:755-876): its E, the E of its +x, +y, +z
+// neighbours (eyx/ezx, exy/ezy, exz/eyz) and its old H.  Present slabs get their psi
+// pairs updated in q (x: Eyx, Ezx; y: Exy, Ezy; z: Exz, Eyz), corrections before the
+// division by K, in the x, y, z order of the reference.
+__device__ __forceinline__ Hv h_cell(float Mh, float idx, float idy, float idz, float ex, float ey,
+                                     float ez, float eyx, float ezx, float exy, float ezy, float exz,
+                                     float eyz, float hx, float hy, float hz, bool bx, const Prof &px,
+                                     bool by, const Prof &py, bool bz, const Prof &pz, float (&q)[6]) {
+    float dExy = (exy - ex) * idy;
+    float dExz = (exz - ex) * idz;
+    float dEyx = (eyx - ey) * idx;
+    float dEyz = (eyz - ey) * idz;
+    float dEzx = (ezx - ez) * idx;
+    float dEzy = (ezy - ez) * idy;
+    if (bx) {   // src/models.jl:805-826
+        q[0] = __builtin_fmaf(px.B, q[0], px.A * dEyx);
+        q[1] = __builtin_fmaf(px.B, q[1], px.A * dEzx);
+        hy = __builtin_fmaf(Mh, q[1], hy);
+        hz = __builtin_fmaf(-Mh, q[0], hz);
+        dEyx = dEyx * px.Ki;
+        dEzx = dEzx * px.Ki;
+    }
+    if (by) {   // src/models.jl:827-848
+        q[2] = __builtin_fmaf(py.B, q[2], py.A * dExy);
+        q[3] = __builtin_fmaf(py.B, q[3], py.A * dEzy);
+        hx = __builtin_fmaf(-Mh, q[3], hx);
+        hz = __builtin_fmaf(Mh, q[2], hz);
+        dExy = dExy * py.Ki;
+        dEzy = dEzy * py.Ki;
+    }
+    if (bz) {   // src/models.jl:849-871
+        q[4] = __builtin_fmaf(pz.B, q[4], pz.A * dExz);
+        q[5] = __builtin_fmaf(pz.B, q[5], pz.A * dEyz);
+        hx = __builtin_fmaf(Mh, q[5], hx);
+        hy = __builtin_fmaf(-Mh, q[4], hy);
+        dExz = dExz * pz.Ki;
+        dEyz = dEyz * pz.Ki;
+    }
+    Hv r;   // src/models.jl:874-876
+    r.x = __builtin_fmaf(-Mh, dEzy - dEyz, hx);
+    r.y = __builtin_fmaf(-Mh, dExz - dEzx, hy);
+    r.z = __builtin_fmaf(-Mh, dEyx - dExy, hz);
+    return r;
+}
+
+// D (or, for a cell without stored D, E in E form) of one cell in the E half-step
+// (src/models.jl:920-1040): its new H, the new H of its -y (hxy/hzy), -x (hyx/hzx) and
+// -z (hxm/hym) neighbours.  Slab corrections before the Md1 scaling, as in the reference;
+// psi pairs in q (x: Hyx, Hzx; y: Hxy, Hzy; z: Hxz, Hyz).
+__device__ __forceinline__ void e_cell(float Md1, float Md2, float idx, float idy, float idz, float hx,
+                                       float hy, float hz, float hxy, float hzy, float hyx, float hzx,
+                                       float hxm, float hym, float &d0, float &d1, float &d2, bool bx,
+                                       const Prof &px, bool by, const Prof &py, bool bz, const Prof &pz,
+                                       float (&q)[6]) {
+    float dHxy = (hx - hxy) * idy;
+    float dHxz = (hx - hxm) * idz;
+    float dHyx = (hy - hyx) * idx;
+    float dHyz = (hy - hym) * idz;
+    float dHzx = (hz - hzx) * idx;
+    float dHzy = (hz - hzy) * idy;
+    if (bx) {   // src/models.jl:976-997
+        q[0] = __builtin_fmaf(px.B, q[0], px.A * dHyx);
+        q[1] = __builtin_fmaf(px.B, q[1], px.A * dHzx);
+        d1 = __builtin_fmaf(-Md2, q[1], d1);
+        d2 = __builtin_fmaf(Md2, q[0], d2);
+        dHyx = dHyx * px.Ki;
+        dHzx = dHzx * px.Ki;
+    }
+    if (by) {   // src/models.jl:998-1015
+        q[2] = __builtin_fmaf(py.B, q[2], py.A * dHxy);
+        q[3] = __builtin_fmaf(py.B, q[3], py.A * dHzy);
+        d0 = __builtin_fmaf(Md2, q[3], d0);
+        d2 = __builtin_fmaf(-Md2, q[2], d2);
+        dHxy = dHxy * py.Ki;
+        dHzy = dHzy * py.Ki;
+    }
+    if (bz) {   // src/models.jl:1016-1035
+        q[4] = __builtin_fmaf(pz.B, q[4], pz.A * dHxz);
+        q[5] = __builtin_fmaf(pz.B, q[5], pz.A * dHyz);
+        d0 = __builtin_fmaf(-Md2, q[5], d0);
+        d1 = __builtin_fmaf(Md2, q[4], d1);
+        dHxz = dHxz * pz.Ki;
+        dHyz = dHyz * pz.Ki;
+    }
+    d0 = __builtin_fmaf(Md1, d0, Md2 * (dHzy - dHyz));   // src/models.jl:1038-1040
+    d1 = __builtin_fmaf(Md1, d1, Md2 * (dHxz - dHzx));
+    d2 = __builtin_fmaf(Md1, d2, Md2 * (dHyx - dHxy));
+}
+
+// boundary rule for a neighbour outside the grid: 0 the loaded (periodic) value, 1 zero
+// (Dirichlet), 2 the cell's own value (Neumann), as in src/models.jl:755-802,920-973
+__device__ __forceinline__ float bcsel(int mode, float loaded, float own) {
+    return mode == 0 ? loaded : mode == 1 ? 0.0f : own;
+}
+__device__ __forceinline__ int bcmode(int bc) { return bc == BC_PERIODIC ? 0 : bc == BC_DIRICHLET ? 1 : 2; }
+
+}  // namespace
+
+template <int MODE, bool INTEG, int R, int PM>
+__global__ __launch_bounds__(64) void k_step3d_f(const DevModel *__restrict__ models, const Tile *__restrict__ tiles,
+                                                 int ntiles) {
+    __shared__ float4 ctab[128];
+    __shared__ float2 ktab[128];    // Kerr Pade (K2, K3)
+    __shared__ float cw[128];       // Me*Md2 (the E-form update without D), full mode
+    // incident values of the TFSF x faces for this tile's rows and planes:
+    // [side][H_u, H_v, E_u, E_v][row][plane - z0]
+    __shared__ float xinc[2][4][R][FUSED_XFACE_ZC];
+    const int lane = threadIdx.x;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const Tile tl = tiles[t];
+        if (tl.x < 0) continue;
+        const DevModel &m = models[tl.x];
+        if (*m.it >= m.Nt) continue;
+        const int nm = MODE == MODE_VAC ? 0 : m.nmat;
+        for (int k = lane; k <= nm; k += 64) {
+            const DevMat &mt = m.mats[k];
+            const float kind = (mt.flags & (MAT_DISP | MAT_PLASMA)) ? 2.0f : (mt.flags & MAT_KERR) ? 1.0f : 0.0f;
+            ctab[k] = make_float4(mt.Md1, mt.Md2, mt.Me, MODE == MODE_FULL ? kind : mt.MeMd2);
+            if (MODE == MODE_FULL) { ktab[k] = make_float2(mt.K2, mt.K3); cw[k] = mt.MeMd2; }
+        }
+        const int Nx = m.Nx, Ny = m.Ny, Nz = m.Nz;
+        const int x0 = tl.y, y0 = tl.z, z0 = tile_z0(tl), z1 = tile_z1(tl);
+        // TFSF x faces finished here, as in the two-pass E kernel: the incident values of
+        // the tile's x-face cells come into LDS once per tile
+        int xb[6] = {0, -2, 0, -1, 0, -1};
+        float xcm = 0.0f, xce = 0.0f, xcd = 0.0f;
+        bool xl[4] = {false, false, false, false};
+        bool xtile = false;
+        if (m.xtfsf) {
+            const __attribute__((address_space(1))) DevSource &S =
+                *(const __attribute__((address_space(1))) DevSource *)m.xtfsf;
+#pragma unroll
+            for (int q = 0; q < 6; q++) xb[q] = S.box[q];
+            xcm = S.cm[0]; xce = S.ce[0]; xcd = S.cd[0];
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) xl[kk] = tfsf_live(S, 0, kk);
+            const int nzt = z1 - z0;
+            xtile = ((xb[0] - 1 >= x0 && xb[0] - 1 < x0 + 64) || (xb[0] >= x0 && xb[0] < x0 + 64) ||
+                     (xb[1] >= x0 && xb[1] < x0 + 64)) &&
+                    y0 <= xb[3] && y0 + R - 1 >= xb[2] && z0 <= xb[5] && z1 - 1 >= xb[4];
+            if (xtile) {
+                const int it0 = *m.it;
+                for (int k = lane; k < 8 * R * nzt; k += 64) {
+                    const int zz = k % nzt, r = (k / nzt) % R, kk = (k / (nzt * R)) % 4, side = k / (nzt * R * 4);
+                    const int u = y0 + r, v = z0 + zz;
+                    const bool in = (kk == 0 || kk == 3) ? (u >= xb[2] && u <= xb[3] && v >= xb[4] && v <= xb[5] - 1)
+                                                         : (u >= xb[2] && u <= xb[3] - 1 && v >= xb[4] && v <= xb[5]);
+                    xinc[side][kk][r][zz] = in && xl[kk] ? tfsf_inc(S, Nz, 0, side, kk, u, v, it0) : 0.0f;
+                }
+            }
+        }
+        __syncthreads();
+
+        const int ix = x0 + lane;
+        const bool xin = ix < Nx;
+        const int ixc = xin ? ix : Nx - 1;
+        const int lx = min(63, Nx - 1 - x0);            // last lane inside the grid
+        const int nr = min(R, Ny - y0);                 // rows inside the grid
+        const unsigned syB = 4u * (unsigned)m.sy, szB = 4u * (unsigned)m.sz;
+        const float idx = m.idx, idy = m.idy, idz = m.idz, dtf = m.dtf;
+        const float Mh = m.mats[0].Mh;                  // every material has mu = 1
+        const float *Ex = m.F[EX], *Ey = m.F[EY], *Ez = m.F[EZ];          // previous step
+        const float *Hx = m.F[HX], *Hy = m.F[HY], *Hz = m.F[HZ];
+        float *Exw = m.Falt[EX], *Eyw = m.Falt[EY], *Ezw = m.Falt[EZ];    // this step
+        float *Hxw = m.Falt[HX], *Hyw = m.Falt[HY], *Hzw = m.Falt[HZ];
+        float *Dx = m.F[DX], *Dy = m.F[DY], *Dz = m.F[DZ];
+        float *Sa = m.Sa, *E2 = m.E2;
+        const bool doSa = INTEG && Sa, doE2 = INTEG && E2;
+        const unsigned char *id = m.id;
+        const bool dless = m.dless;
+        const bool useg = MODE != MODE_VAC || INTEG || dless;
+        // row below the tile (its new Hx, Hz feed E of row 0): mode 0 computed from the old
+        // buffers, 1 zero (Dirichlet), 2 row 0's own value (Neumann); ybn: the rule for
+        // that row's own +y neighbour (0: the tile's row 0)
+        int yb, ybm, ybn;
+        if (y0 > 0) { yb = y0 - 1; ybm = 0; ybn = 0; }
+        else if (m.bcs[2] == BC_PERIODIC) { yb = Ny - 1; ybm = 0; ybn = bcmode(m.bcs[3]); }
+        else { yb = 0; ybm = bcmode(m.bcs[2]); ybn = 0; }
+        // row above the last row (Ex, Ez for its H)
+        int yt, ytm;
+        if (y0 + nr < Ny) { yt = y0 + nr; ytm = 0; }
+        else { ytm = bcmode(m.bcs[3]); yt = ytm == 0 ? 0 : Ny - 1; }
+        // column left of lane 0 (its new Hy, Hz feed E of lane 0); xhn: the rule for that
+        // column's own +x neighbour (0: the tile's first column)
+        int xh, xhm, xhn;
+        if (x0 > 0) { xh = x0 - 1; xhm = 0; xhn = 0; }
+        else if (m.bcs[0] == BC_PERIODIC) { xh = Nx - 1; xhm = 0; xhn = bcmode(m.bcs[1]); }
+        else { xh = 0; xhm = bcmode(m.bcs[0]); xhn = 0; }
+        // column right of the last lane (Ey, Ez for its H)
+        int xe, xem;
+        if (x0 + lx + 1 < Nx) { xe = x0 + lx + 1; xem = 0; }
+        else { xem = bcmode(m.bcs[1]); xe = xem == 0 ? 0 : Nx - 1; }
+        // first plane of the march: z0 - 1 (H only) when E of plane z0 needs H below it
+        int pb, zbm;
+        if (z0 > 0) { pb = z0 - 1; zbm = 0; }
+        else if (m.bcs[4] == BC_PERIODIC) { pb = -1; zbm = 0; }
+        else { pb = 0; zbm = bcmode(m.bcs[4]); }
+        const int zhm = bcmode(m.bcs[5]);   // +z neighbour of plane Nz-1
+
+        const unsigned colB = 4u * (unsigned)ixc;
+        unsigned rowB[R];   // byte offset of each row's cell in a plane
+#pragma unroll
+        for (int r = 0; r < R; r++) rowB[r] = colB + syB * (unsigned)min(y0 + r, Ny - 1);
+        const unsigned bB = colB + syB * (unsigned)yb, tB = colB + syB * (unsigned)yt;
+        // lane-distributed halo columns: left column lane t = row t (lane nr: the row
+        // above), right column lane 0 = the row below, lane t = row t - 1
+        const int xhrow = lane < nr ? y0 + lane : yt;
+        const unsigned xhB = 4u * (unsigned)xh + syB * (unsigned)xhrow;
+        const int erow = lane == 0 ? yb : y0 + min(lane - 1, nr - 1);
+        const unsigned eB = 4u * (unsigned)xe + syB * (unsigned)erow;
+        auto wz = [&](int q) { return (unsigned)(q < 0 ? q + Nz : (q >= Nz ? q - Nz : q)); };
+
+        // cells on a TFSF face plane (they keep D): lane, row and plane parts of the test
+        const bool xf_l = ix == xb[0] - 1 || ix == xb[0] || ix == xb[1], xi_l = ix >= xb[0] && ix <= xb[1];
+        bool yf_r[R], yi_r[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int iy = y0 + r;
+            yf_r[r] = (iy == xb[2] - 1 || iy == xb[2] || iy == xb[3]);
+            yi_r[r] = iy >= xb[2] && iy <= xb[3];
+        }
+
+        // CPML slabs fixed over the march, with their profile values: x of this lane and
+        // of the left column, y of the rows, of the row below and of the left column's
+        // lane row (lane t: row t)
+        int fx = -1, px = 0, fxh = -1, pxh = 0, fyb = -1, pyb = 0, fyh = -1, pyh = 0;
+        int fy[R], py[R];
+        Prof xP{0.0f, 0.0f, 1.0f}, xhP = xP, ybP = xP, yhP = xP, yP[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) { fy[r] = -1; py[r] = 0; yP[r] = xP; }
+        if (PM) {
+            fx = slab_of(m, 0, ixc, px);
+            fxh = slab_of(m, 0, xh, pxh);
+            fyb = slab_of(m, 1, yb, pyb);
+            fyh = slab_of(m, 1, min(xhrow, Ny - 1), pyh);
+            xP = prof_of(m, fx, px);
+            xhP = prof_of(m, fxh, pxh);
+            ybP = prof_of(m, fyb, pyb);
+            yhP = prof_of(m, fyh, pyh);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                fy[r] = slab_of(m, 1, min(y0 + r, Ny - 1), py[r]);
+                yP[r] = prof_of(m, fy[r], py[r]);
+            }
+        }
+        const int hrow = min(xhrow, Ny - 1);
+        // x-slab psi arrays through one base (the lower face's, or the upper face's when
+        // the lower face has no layer): the upper face's array follows the lower face's in
+        // the same allocation, so a lane of either face needs only a byte offset
+        const DevLayer &LX = m.L[0].present ? m.L[0] : m.L[1];
+        const float *XE0 = LX.psiE0, *XE1 = LX.psiE1, *XH0 = LX.psiH0, *XH1 = LX.psiH1;
+        float *XE0w = LX.psiE0b, *XE1w = LX.psiE1b, *XH0w = LX.psiH0, *XH1w = LX.psiH1;
+        const unsigned nbx0 = (unsigned)m.L[0].nb, nbx1 = (unsigned)m.L[1].nb;
+        const unsigned x1B = m.L[0].present ? 4u * nbx0 * (unsigned)Ny * (unsigned)Nz : 0u;
+        auto xo = [&](int f, int p, int iy, int iz) {
+            const unsigned nb = f == 0 ? nbx0 : nbx1;
+            return (f == 1 ? x1B : 0u) + 4u * ((unsigned)p + nb * ((unsigned)iy + (unsigned)Ny * (unsigned)iz));
+        };
+        // the same for the y and z slabs
+        const DevLayer &LY = m.L[2].present ? m.L[2] : m.L[3];
+        const float *YE0 = LY.psiE0, *YE1 = LY.psiE1, *YH0 = LY.psiH0, *YH1 = LY.psiH1;
+        float *YE0w = LY.psiE0b, *YE1w = LY.psiE1b, *YH0w = LY.psiH0, *YH1w = LY.psiH1;
+        const unsigned nby0 = (unsigned)m.L[2].nb, nby1 = (unsigned)m.L[3].nb;
+        const unsigned y1B = m.L[2].present ? 4u * (unsigned)Nx * nby0 * (unsigned)Nz : 0u;
+        auto yo = [&](int f, int p, int ix_, int iz) {
+            const unsigned nb = f == 2 ? nby0 : nby1;
+            return (f == 3 ? y1B : 0u) + 4u * ((unsigned)ix_ + (unsigned)Nx * ((unsigned)p + nb * (unsigned)iz));
+        };
+        const DevLayer &LZ = m.L[4].present ? m.L[4] : m.L[5];
+        const float *ZE0 = LZ.psiE0, *ZE1 = LZ.psiE1, *ZH0 = LZ.psiH0, *ZH1 = LZ.psiH1;
+        float *ZE0w = LZ.psiE0b, *ZE1w = LZ.psiE1b, *ZH0w = LZ.psiH0, *ZH1w = LZ.psiH1;
+        const unsigned z1B = m.L[4].present ? 4u * (unsigned)m.L[4].nb * (unsigned)m.sz : 0u;
+        auto zof = [&](int f, int p, unsigned cellB) {   // cellB: byte offset of the cell in its plane
+            return (f == 5 ? z1B : 0u) + cellB + szB * (unsigned)p;
+        };
+        // carried from one plane to the next: Ex, Ey and the id of the rows (loaded with
+        // the plane below as its +z neighbour), Ey of the row below, Ex of the left column
+        float ex[R], ey[R], bey, hex;
+        int gid[R];
+        {
+            const unsigned zo = szB * wz(pb);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                ex[r] = ldb<float>(Ex, rowB[r] + zo);
+                ey[r] = ldb<float>(Ey, rowB[r] + zo);
+                gid[r] = useg ? (int)ldb<unsigned char>(id, (rowB[r] + zo) >> 2) : 0;
+            }
+            bey = ldb<float>(Ey, bB + zo);
+            hex = ldb<float>(Ex, xhB + zo);
+        }
+        float hxm[R], hym[R];     // new H of the plane below
+        bool zm_own = false;
+#pragma unroll
+        for (int r = 0; r < R; r++) { hxm[r] = 0.0f; hym[r] = 0.0f; }
+        if (zbm == 2) zm_own = true;
+        int nonfin = 0;
+        // x-face TFSF columns of this lane
+        const bool xcol = xtile && xf_l;
+        const int xside = ix == xb[1] ? 1 : 0;
+
+        for (int p = pb; p < z1; p++) {
+            const int zg = p < 0 ? p + Nz : p;
+            const int ztm = zg == Nz - 1 ? zhm : 0;   // rule for the +z neighbour of this plane
+            const bool pro = p < z0;                   // the plane below the tile: H only
+            const unsigned zo = szB * (unsigned)zg, zn = szB * wz(p + 1);
+            int fz = -1, pz = 0;
+            Prof zP{0.0f, 0.0f, 1.0f};
+            if (PM & 4) {
+                fz = slab_of(m, 2, zg, pz);
+                zP = prof_of(m, fz, pz);
+            }
+            const bool bz = (PM & 4) && fz >= 0, bx = (PM & 1) && fx >= 0;
+            const bool zf_p = zg == xb[4] - 1 || zg == xb[4] || zg == xb[5], zi_p = zg >= xb[4] && zg <= xb[5];
+            // ---- every load of the plane, issued together: fields, integrals, stored D,
+            // psi pairs, the halo rows and columns
+            float exn[R], eyn[R], ez[R], hx[R], hy[R], hz[R], sa[R], e2[R], dd[R][3];
+            float hq[R][6], eq[R][6];
+            int gidn[R];
+            bool dc[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const unsigned o = rowB[r] + zo, on = rowB[r] + zn;
+                const int iyc = min(y0 + r, Ny - 1);
+                exn[r] = ldb<float>(Ex, on);
+                eyn[r] = ldb<float>(Ey, on);
+                gidn[r] = useg ? (int)ldb<unsigned char>(id, on >> 2) : 0;
+                ez[r] = ldb<float>(Ez, o);
+                hx[r] = ldb<float>(Hx, o);
+                hy[r] = ldb<float>(Hy, o);
+                hz[r] = ldb<float>(Hz, o);
+                sa[r] = 0.0f;
+                e2[r] = 0.0f;
+#pragma unroll
+                for (int c = 0; c < 6; c++) { hq[r][c] = 0.0f; eq[r][c] = 0.0f; }
+                // cells that keep D: on a TFSF face plane and, from the id, cells a source
+                // writes and Kerr / ADE / plasma cells
+                bool d = false;
+                if (!pro) {
+                    d = !dless || (xf_l && yi_r[r] && zi_p) || (yf_r[r] && xi_l && zi_p) || (zf_p && xi_l && yi_r[r]);
+                    if (useg && dless && !d) {
+                        const int g = gid[r];
+                        d = (g & ID_TOUCHED) || (MODE == MODE_FULL && ctab[g & ID_MASK].w != 0.0f);
+                    }
+                    if (d) { dd[r][0] = ldb<float>(Dx, o); dd[r][1] = ldb<float>(Dy, o); dd[r][2] = ldb<float>(Dz, o); }
+                    if (doSa) sa[r] = ldnt(Sa, o);
+                    if (doE2) e2[r] = ldnt(E2, o);
+                }
+                dc[r] = d;
+                if (bx) {
+                    const unsigned j = xo(fx, px, iyc, zg);
+                    hq[r][0] = ldb<float>(XE0, j);
+                    hq[r][1] = ldb<float>(XE1, j);
+                    if (!pro) { eq[r][0] = ldb<float>(XH0, j); eq[r][1] = ldb<float>(XH1, j); }
+                }
+                if ((PM & 2) && fy[r] >= 0) {
+                    const unsigned j = yo(fy[r], py[r], ixc, zg);
+                    hq[r][2] = ldb<float>(YE0, j);
+                    hq[r][3] = ldb<float>(YE1, j);
+                    if (!pro) { eq[r][2] = ldb<float>(YH0, j); eq[r][3] = ldb<float>(YH1, j); }
+                }
+                if (bz) {
+                    const unsigned j = zof(fz, pz, rowB[r]);
+                    hq[r][4] = ldb<float>(ZE0, j);
+                    hq[r][5] = ldb<float>(ZE1, j);
+                    if (!pro) { eq[r][4] = ldb<float>(ZH0, j); eq[r][5] = ldb<float>(ZH1, j); }
+                }
+            }
+            const float beyn = ldb<float>(Ey, bB + zn), hexn = ldb<float>(Ex, xhB + zn);
+            const float tex = ldb<float>(Ex, tB + zo), tez = ldb<float>(Ez, tB + zo);
+            const float cey = ldb<float>(Ey, eB + zo), cez = ldb<float>(Ez, eB + zo);
+            // the row below: Ex, Ez, old Hx, Hz and the psi values its Hx, Hz need
+            float bexv = 0.0f, bez = 0.0f, bhx = 0.0f, bhz = 0.0f;
+            float bq[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            const bool bby = (PM & 2) && fyb >= 0;
+            if (!pro && ybm == 0) {
+                bexv = ldb<float>(Ex, bB + zo); bez = ldb<float>(Ez, bB + zo);
+                bhx = ldb<float>(Hx, bB + zo); bhz = ldb<float>(Hz, bB + zo);
+                if (bx) bq[0] = ldb<float>(XE0, xo(fx, px, yb, zg));
+                if (bby) {
+                    const unsigned j = yo(fyb, pyb, ixc, zg);
+                    bq[2] = ldb<float>(YE0, j);
+                    bq[3] = ldb<float>(YE1, j);
+                }
+                if (bz) bq[5] = ldb<float>(ZE1, zof(fz, pz, bB));
+            }
+            // the left column (lane t: row t): Ey, Ez, old Hy, Hz and the psi values its Hy,
+            // Hz need
+            float hey = 0.0f, hez = 0.0f, hhy = 0.0f, hhz = 0.0f;
+            float xq[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            const bool xbx = (PM & 1) && fxh >= 0, xby = (PM & 2) && fyh >= 0;
+            if (!pro && xhm == 0) {
+                hey = ldb<float>(Ey, xhB + zo); hez = ldb<float>(Ez, xhB + zo);
+                hhy = ldb<float>(Hy, xhB + zo); hhz = ldb<float>(Hz, xhB + zo);
+                if (PM && lane < nr) {
+                        if (xbx) {
+                        const unsigned j = xo(fxh, pxh, hrow, zg);
+                        xq[0] = ldb<float>(XE0, j);
+                        xq[1] = ldb<float>(XE1, j);
+                    }
+                    if (xby) xq[2] = ldb<float>(YE0, yo(fyh, pyh, xh, zg));
+                    if (bz) xq[4] = ldb<float>(ZE0, zof(fz, pz, 4u * (unsigned)xh + syB * (unsigned)hrow));
+                }
+            }
+
+            // ---- new H of the tile's rows (src/models.jl:735-889)
+            Hv Hn[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                float eyx = from_next(ey[r]), ezx = from_next(ez[r]);
+                const float cy = rdlane(cey, r + 1), cz = rdlane(cez, r + 1);
+                if (lane == lx) {
+                    eyx = bcsel(xem, cy, ey[r]);
+                    ezx = bcsel(xem, cz, ez[r]);
+                }
+                float exy, ezy;
+                if (r + 1 < R && r + 1 < nr) { exy = ex[r + 1]; ezy = ez[r + 1]; }
+                else { exy = bcsel(ytm, tex, ex[r]); ezy = bcsel(ytm, tez, ez[r]); }
+                const float exz = ztm == 0 ? exn[r] : bcsel(ztm, 0.0f, ex[r]);
+                const float eyz = ztm == 0 ? eyn[r] : bcsel(ztm, 0.0f, ey[r]);
+                const bool by = (PM & 2) && fy[r] >= 0;
+                Hn[r] = h_cell(Mh, idx, idy, idz, ex[r], ey[r], ez[r], eyx, ezx, exy, ezy, exz, eyz, hx[r], hy[r],
+                               hz[r], bx, xP, by, yP[r], bz, zP, hq[r]);
+                if (PM && !pro && xin && r < nr) {
+                    const int iyc = min(y0 + r, Ny - 1);
+                    if (bx) {
+                        const unsigned j = xo(fx, px, iyc, zg);
+                        stb<float>(XE0w, j, hq[r][0]);
+                        stb<float>(XE1w, j, hq[r][1]);
+                    }
+                    if (by) {
+                        const unsigned j = yo(fy[r], py[r], ixc, zg);
+                        stb<float>(YE0w, j, hq[r][2]);
+                        stb<float>(YE1w, j, hq[r][3]);
+                    }
+                    if (bz) {
+                        const unsigned j = zof(fz, pz, rowB[r]);
+                        stb<float>(ZE0w, j, hq[r][4]);
+                        stb<float>(ZE1w, j, hq[r][5]);
+                    }
+                }
+            }
+            if (!pro) {
+                // ---- new Hx, Hz of the row below (recomputed; the other tile stores them)
+                float hxb = 0.0f, hzb = 0.0f;
+                if (ybm == 0) {
+                    float eyx = from_next(bey);
+                    const float cy = rdlane(cey, 0);
+                    if (lane == lx) eyx = bcsel(xem, cy, bey);
+                    const float exy = bcsel(ybn, ex[0], bexv), ezy = bcsel(ybn, ez[0], bez);
+                    const float eyz = ztm == 0 ? beyn : bcsel(ztm, 0.0f, bey);
+                    const Hv hb = h_cell(Mh, idx, idy, idz, bexv, bey, bez, eyx, bez, exy, ezy, bexv, eyz, bhx,
+                                         0.0f, bhz, bx, xP, bby, ybP, bz, zP, bq);
+                    hxb = hb.x;
+                    hzb = hb.z;
+                }
+                // ---- new Hy, Hz of the column left of lane 0, on lanes 0..nr-1 (lane t: row t)
+                float hyh = 0.0f, hzh = 0.0f;
+                if (xhm == 0) {
+                    float ezr = 0.0f, eyr = 0.0f;   // E of the tile's first column, row t
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const float vz = rdlane(ez[r], 0), vy = rdlane(ey[r], 0);
+                        if (lane == r) { ezr = vz; eyr = vy; }
+                    }
+                    ezr = bcsel(xhn, ezr, hez);
+                    eyr = bcsel(xhn, eyr, hey);
+                    float exy = from_next(hex);
+                    if (lane == nr - 1) exy = bcsel(ytm, exy, hex);
+                    const float exz = ztm == 0 ? hexn : bcsel(ztm, 0.0f, hex);
+                    const Hv hh = h_cell(Mh, idx, idy, idz, hex, hey, hez, eyr, ezr, exy, hez, exz, hey, 0.0f, hhy,
+                                         hhz, xbx, xhP, xby, yhP, bz, zP, xq);
+                    hyh = hh.y;
+                    hzh = hh.z;
+                }
+                // ---- new E of the tile's rows (src/models.jl:895-1175)
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int iy = y0 + r;
+                    const bool act = xin && r < nr;
+                    const unsigned o = rowB[r] + zo;
+                    if (zm_own) { hxm[r] = Hn[r].x; hym[r] = Hn[r].y; }
+                    float hxy, hzy;
+                    if (r > 0) { hxy = Hn[r - 1].x; hzy = Hn[r - 1].z; }
+                    else { hxy = bcsel(ybm, hxb, Hn[0].x); hzy = bcsel(ybm, hzb, Hn[0].z); }
+                    float hyx = from_prev(Hn[r].y), hzx = from_prev(Hn[r].z);
+                    const float vy = rdlane(hyh, r), vz = rdlane(hzh, r);
+                    if (lane == 0) {
+                        hyx = bcsel(xhm, vy, Hn[r].y);
+                        hzx = bcsel(xhm, vz, Hn[r].z);
+                    }
+                    const int gg = gid[r];
+                    const int g = gg & ID_MASK;
+                    const float4 cf = MODE != MODE_VAC ? ctab[g] : ctab[0];
+                    const bool dcr = dc[r];
+                    const float Md1 = cf.x, Md2 = dcr ? cf.y : (MODE == MODE_FULL ? cw[g] : cf.w);
+                    float d0 = dcr ? dd[r][0] : ex[r];
+                    float d1 = dcr ? dd[r][1] : ey[r];
+                    float d2 = dcr ? dd[r][2] : ez[r];
+                    const bool by = (PM & 2) && fy[r] >= 0;
+                    e_cell(Md1, Md2, idx, idy, idz, Hn[r].x, Hn[r].y, Hn[r].z, hxy, hzy, hyx, hzx, hxm[r], hym[r],
+                           d0, d1, d2, bx, xP, by, yP[r], bz, zP, eq[r]);
+                    if (PM && act) {
+                        const int iyc = min(iy, Ny - 1);
+                        if (bx) {
+                            const unsigned j = xo(fx, px, iyc, zg);
+                            stb<float>(XH0w, j, eq[r][0]);
+                            stb<float>(XH1w, j, eq[r][1]);
+                        }
+                        if (by) {
+                            const unsigned j = yo(fy[r], py[r], ixc, zg);
+                            stb<float>(YH0w, j, eq[r][2]);
+                            stb<float>(YH1w, j, eq[r][3]);
+                        }
+                        if (bz) {
+                            const unsigned j = zof(fz, pz, rowB[r]);
+                            stb<float>(ZH0w, j, eq[r][4]);
+                            stb<float>(ZH1w, j, eq[r][5]);
+                        }
+                    }
+                    float e[3];
+                    if (dcr) { e[0] = d0 * cf.z; e[1] = d1 * cf.z; e[2] = d2 * cf.z; }
+                    else { e[0] = d0; e[1] = d1; e[2] = d2; }       // already E
+                    bool defer = false;
+                    if (MODE == MODE_FULL && cf.w != 0.0f) {
+                        if (cf.w > 1.5f) {
+                            defer = true;
+                        } else {   // Kerr Pade, src/models.jl:1140-1149
+                            const float2 kk = ktab[g];
+                            const float du[3] = {d0, d1, d2};
+#pragma unroll
+                            for (int c = 0; c < 3; c++) {
+                                const float u = du[c];
+                                e[c] = (1.0f + kk.x * u + 2.0f * kk.y * (u * u)) /
+                                       (1.0f + 2.0f * kk.x * u + 3.0f * kk.y * (u * u)) * u * cf.z;
+                            }
+                        }
+                    }
+                    // x-face TFSF epilogue (src/sources.jl:725-745); the H correction of the
+                    // H-out cells enters only the integrals here, the face kernel writes it
+                    float hyI = Hn[r].y, hzI = Hn[r].z;
+                    bool xh2 = false;
+                    if (xcol && iy >= xb[2] && iy <= xb[3] && zg >= xb[4] && zg <= xb[5]) {
+                        xh2 = tfsf_xhandled_box(xb, ix, iy, zg, defer);
+                        if (xh2) {
+                            const int u = iy, v = zg, side = xside, zz = zg - z0;
+                            const int U2 = xb[3], V2 = xb[5];
+                            const int nH = side ? xb[1] : xb[0] - 1, nE = side ? xb[1] : xb[0];
+                            const float s0 = side ? 1.0f : -1.0f;
+                            if (ix == nH) {
+                                if (u <= U2 && v <= V2 - 1 && xl[3]) hyI = __builtin_fmaf(s0 * xcm, xinc[side][3][r][zz], hyI);
+                                if (u <= U2 - 1 && v <= V2 && xl[2]) hzI = __builtin_fmaf(-s0 * xcm, xinc[side][2][r][zz], hzI);
+                            }
+                            if (ix == nE) {
+                                if (u <= U2 - 1 && v <= V2 && xl[1]) {
+                                    const float iv = xinc[side][1][r][zz];
+                                    e[1] = __builtin_fmaf(-s0 * xce, iv, e[1]);
+                                    d1 = __builtin_fmaf(-s0 * xcd, iv, d1);
+                                }
+                                if (u <= U2 && v <= V2 - 1 && xl[0]) {
+                                    const float iv = xinc[side][0][r][zz];
+                                    e[2] = __builtin_fmaf(s0 * xce, iv, e[2]);
+                                    d2 = __builtin_fmaf(s0 * xcd, iv, d2);
+                                }
+                            }
+                        }
+                    }
+                    if (act) {
+                        stb<float>(Hxw, o, Hn[r].x); stb<float>(Hyw, o, Hn[r].y); stb<float>(Hzw, o, Hn[r].z);
+                        if (dcr) { stb<float>(Dx, o, d0); stb<float>(Dy, o, d1); stb<float>(Dz, o, d2); }
+                        // a cell whose E the state pass finishes keeps E^n, which that pass reads
+                        if (defer) { e[0] = ex[r]; e[1] = ey[r]; e[2] = ez[r]; }
+                        stb<float>(Exw, o, e[0]); stb<float>(Eyw, o, e[1]); stb<float>(Ezw, o, e[2]);
+                        if (!defer && !isfinite(e[0])) nonfin |= isnan(e[0]) ? 1 : 2;
+                        if (INTEG && !defer && (!(gg & ID_TOUCHED) || xh2)) {   // src/outputs.jl:480-483
+                            if (doSa) {
+                                const float a = e[1] * hzI - e[2] * hyI, b = e[2] * Hn[r].x - e[0] * hzI;
+                                const float cc = e[0] * hyI - e[1] * Hn[r].x;
+                                stnt(Sa, o, sa[r] + sa_sqrt(a * a + b * b + cc * cc) * dtf);
+                            }
+                            if (doE2) stnt(E2, o, e2[r] + (e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) * dtf);
+                        }
+                    }
+                }
+                zm_own = false;
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                hxm[r] = Hn[r].x;
+                hym[r] = Hn[r].y;
+                ex[r] = exn[r];
+                ey[r] = eyn[r];
+                gid[r] = gidn[r];
+            }
+            bey = beyn;
+            hex = hexn;
+        }
+        if (__any(nonfin != 0) && nonfin) atomicOr(m.flag, nonfin);
+        // the LDS tables are rewritten by the next tile
+        __syncthreads();
+    }
+}
+
+// Swaps the H/E buffers and the H-half psi arrays of every model the fused launch just
+// stepped (one thread per model; models that had finished did not step).
+__global__ void k_swap_pp(DevModel *models, int nmodels) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nmodels) return;
+    DevModel &m = models[k];
+    if (*m.it >= m.Nt) return;
+    const int cs[6] = {HX, HY, HZ, EX, EY, EZ};
+    for (int c = 0; c < 6; c++) {
+        float *a = m.F[cs[c]];
+        m.F[cs[c]] = m.Falt[cs[c]];
+        m.Falt[cs[c]] = a;
+    }
+    for (int f = 0; f < 6; f++) {
+        DevLayer &L = m.L[f];
+        float *a0 = L.psiE0, *a1 = L.psiE1;
+        L.psiE0 = L.psiE0b; L.psiE1 = L.psiE1b;
+        L.psiE0b = a0; L.psiE1b = a1;
+    }
+}
+
+template <int R, int PM>
+static void lf(DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, hipStream_t s) {
+#define SF_(MD, IN) hipLaunchKernelGGL((k_step3d_f<MD, IN, R, PM>), dim3(grid), dim3(64), 0, s, m, tiles, ntiles)
+    if (mode == MODE_VAC) { if (integ) SF_(MODE_VAC, true); else SF_(MODE_VAC, false); }
+    else if (mode == MODE_SIMPLE) { if (integ) SF_(MODE_SIMPLE, true); else SF_(MODE_SIMPLE, false); }
+    else { if (integ) SF_(MODE_FULL, true); else SF_(MODE_FULL, false); }
+#undef SF_
+}
+
+void launch_swap_pp(DevModel *m, int nmodels, hipStream_t s) {
+    hipLaunchKernelGGL(k_swap_pp, dim3((nmodels + 63) / 64), dim3(64), 0, s, m, nmodels);
+}
+
+// pmask: the CPML axes the tiles' cells (and recomputed halo cells) may lie in; the
+// kernel is built for 0 (none), 1 (x only) or 7 (any)
+void launch_step3d_f(DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int rows,
+                     int pmask, hipStream_t s) {
+    if (ntiles <= 0) return;
+#define LF_(RR) (pmask == 0 ? lf<RR, 0>(m, tiles, ntiles, grid, mode, integ, s)   \
+                 : pmask == 1 ? lf<RR, 1>(m, tiles, ntiles, grid, mode, integ, s) \
+                              : lf<RR, 7>(m, tiles, ntiles, grid, mode, integ, s))
+    if (rows == 2) LF_(2);
+    else LF_(4);
+#undef LF_
+}
+
+}  // namespace mfd

@@ -484,20 +484,20 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
                         const float s0 = side ? 1.0f : -1.0f;
                         if (ix == nH) {
                             if (u <= U2 && v <= V2 - 1 && xl[3])
-                                hyI = hyI + (s0 * xcm) * xinc[side][3][ty][zz];
+                                hyI = __builtin_fmaf(s0 * xcm, xinc[side][3][ty][zz], hyI);
                             if (u <= U2 - 1 && v <= V2 && xl[2])
-                                hzI = hzI + (-s0 * xcm) * xinc[side][2][ty][zz];
+                                hzI = __builtin_fmaf(-s0 * xcm, xinc[side][2][ty][zz], hzI);
                         }
                         if (ix == nE) {
                             if (u <= U2 - 1 && v <= V2 && xl[1]) {
                                 const float iv = xinc[side][1][ty][zz];
-                                e[1] = e[1] + (-s0 * xce) * iv;
-                                d1 = d1 + (-s0 * xcd) * iv;
+                                e[1] = __builtin_fmaf(-s0 * xce, iv, e[1]);
+                                d1 = __builtin_fmaf(-s0 * xcd, iv, d1);
                             }
                             if (u <= U2 && v <= V2 - 1 && xl[0]) {
                                 const float iv = xinc[side][0][ty][zz];
-                                e[2] = e[2] + (s0 * xce) * iv;
-                                d2 = d2 + (s0 * xcd) * iv;
+                                e[2] = __builtin_fmaf(s0 * xce, iv, e[2]);
+                                d2 = __builtin_fmaf(s0 * xcd, iv, d2);
                             }
                         }
                     }

@@ -271,29 +271,29 @@ __global__ __launch_bounds__(256) void k_tfsf_axis(const DevModel *__restrict__ 
         const float iv = inc(3);
         float *F = m.F[cHu];
         const long long c = cell(nH);
-        F[c] = F[c] + (s0 * cm) * iv;
+        F[c] = __builtin_fmaf(s0 * cm, iv, F[c]);
     }
     if (u <= U2 - 1 && v <= V2 && live(2)) {     // H_v += (-s0) Cm/d * inc E_u
         const float iv = inc(2);
         float *F = m.F[cHv];
         const long long c = cell(nH);
-        F[c] = F[c] + (-s0 * cm) * iv;
+        F[c] = __builtin_fmaf(-s0 * cm, iv, F[c]);
     }
     if (u <= U2 - 1 && v <= V2 && live(1)) {     // E_u, D_u += (-s0) Ce/d * inc H_v
         const float iv = inc(1);
         const long long c = cell(nE);
         float *E = m.F[cEu];
         float *D = m.F[cDu];
-        E[c] = E[c] + (-s0 * ce) * iv;
-        D[c] = D[c] + (-s0 * cd) * iv;
+        E[c] = __builtin_fmaf(-s0 * ce, iv, E[c]);
+        D[c] = __builtin_fmaf(-s0 * cd, iv, D[c]);
     }
     if (u <= U2 && v <= V2 - 1 && live(0)) {     // E_v, D_v += (s0) Ce/d * inc H_u
         const float iv = inc(0);
         const long long c = cell(nE);
         float *E = m.F[cEv];
         float *D = m.F[cDv];
-        E[c] = E[c] + (s0 * ce) * iv;
-        D[c] = D[c] + (s0 * cd) * iv;
+        E[c] = __builtin_fmaf(s0 * ce, iv, E[c]);
+        D[c] = __builtin_fmaf(s0 * cd, iv, D[c]);
     }
 }
 
@@ -354,25 +354,25 @@ __global__ __launch_bounds__(256) void k_tfsf_gather(const DevModel *__restrict_
                 const float s0 = (axis == 1 ? 1.0f : -1.0f) * sg;
                 if (n == nH) {
                     if (u <= U2 && v <= V2 - 1 && tfsf_live(S, axis, 3)) {
-                        f[cHu] = f[cHu] + (s0 * cm) * tfsf_inc(S, m.Nz, axis, side, 3, u, v, it0);
+                        f[cHu] = __builtin_fmaf(s0 * cm, tfsf_inc(S, m.Nz, axis, side, 3, u, v, it0), f[cHu]);
                         dirty |= 1u << cHu;
                     }
                     if (u <= U2 - 1 && v <= V2 && tfsf_live(S, axis, 2)) {
-                        f[cHv] = f[cHv] + (-s0 * cm) * tfsf_inc(S, m.Nz, axis, side, 2, u, v, it0);
+                        f[cHv] = __builtin_fmaf(-s0 * cm, tfsf_inc(S, m.Nz, axis, side, 2, u, v, it0), f[cHv]);
                         dirty |= 1u << cHv;
                     }
                 }
                 if (n == nE && !xh) {
                     if (u <= U2 - 1 && v <= V2 && tfsf_live(S, axis, 1)) {
                         const float iv = tfsf_inc(S, m.Nz, axis, side, 1, u, v, it0);
-                        f[cEu] = f[cEu] + (-s0 * ce) * iv;
-                        f[cDu] = f[cDu] + (-s0 * cd) * iv;
+                        f[cEu] = __builtin_fmaf(-s0 * ce, iv, f[cEu]);
+                        f[cDu] = __builtin_fmaf(-s0 * cd, iv, f[cDu]);
                         dirty |= (1u << cEu) | (1u << cDu);
                     }
                     if (u <= U2 && v <= V2 - 1 && tfsf_live(S, axis, 0)) {
                         const float iv = tfsf_inc(S, m.Nz, axis, side, 0, u, v, it0);
-                        f[cEv] = f[cEv] + (s0 * ce) * iv;
-                        f[cDv] = f[cDv] + (s0 * cd) * iv;
+                        f[cEv] = __builtin_fmaf(s0 * ce, iv, f[cEv]);
+                        f[cDv] = __builtin_fmaf(s0 * cd, iv, f[cDv]);
                         dirty |= (1u << cEv) | (1u << cDv);
                     }
                 }

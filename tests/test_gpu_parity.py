@@ -2,6 +2,8 @@
 same C-ABI call sequence.  Bar: relative L2 <= 1e-4 on the H, D and E vector fields, on
 monitors, on DFT spectra and on integral outputs (the FP32-vs-FP64 tolerance of the
 per-step update), plus the reference's own analytic checks."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -157,7 +159,7 @@ def test_slab_1d():
 
 
 @pytest.mark.parametrize("kmax", [3.0, 5.0])
-@pytest.mark.parametrize("ndims,kform", [(1, None), (2, None), (3, 2), (3, 0)])
+@pytest.mark.parametrize("ndims,kform", [(1, None), (2, None), (3, 3), (3, 2), (3, 0)])
 def test_cpml_kappa(ndims, kform, kmax):
     """CPML with a kappa profile (kmax > 1): the slab updates divide the E/H differences by
     K after the psi recursion (src/models.jl:812-871, src/pml.jl:68-81).  With kmax = 1 the
@@ -180,7 +182,7 @@ def test_cpml_kappa(ndims, kform, kmax):
     assert_parity(hg, ho, desc, model)
 
 
-@pytest.mark.parametrize("ndims,kform", [(1, None), (2, None), (3, 2), (3, 0)])
+@pytest.mark.parametrize("ndims,kform", [(1, None), (2, None), (3, 3), (3, 2), (3, 0)])
 def test_tfsf_replay_tables(ndims, kform):
     """TFSF from replay tables (MFD_TFSF_TABLES), the form in which the reference's
     TFSFSource reaches add_source! (src/sources.jl:269-346,400-497,565-776): 4 tables in
@@ -193,7 +195,7 @@ def test_tfsf_replay_tables(ndims, kform):
     assert np.max(np.abs(ho.field("Ex"))) > 1e-2
 
 
-@pytest.mark.parametrize("kform", [2, 0])
+@pytest.mark.parametrize("kform", [3, 2, 0])
 def test_tfsf_faces_through_materials(kform):
     """TFSF as the only source (one gather launch per step; with the pipelined kernels
     the E kernel finishes x-face cells itself) with face cells holding stateless,
@@ -325,6 +327,65 @@ def test_h_tile_list_is_bitwise_neutral(kform):
     for o in outs[1:]:
         for a, b in zip(outs[0], o):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("bc", ["periodic", "dirichlet", "neumann"])
+def test_fused_step_tiling_is_bitwise_neutral(bc):
+    """The fused step (one z march per step over ping-pong H/E buffers, H recomputed one
+    cell below each tile in x, y and z) gives the same bits whatever the tile shape: 2 or
+    4 rows per lane, z chunks of 3 to 32 planes, one launch or three CPML-class launches
+    (one after the other or side by side), any XCD block.  The grid is 150 x 41 x 37 so
+    that tiles are partial in x, y and z, and the faces carry `bc`.  The result also
+    matches the FP64 oracle and the two-pass form to rounding."""
+    model, mons = cases.spheres_3d(n=37, nx=150, steps=60, bc=bc)
+    desc = model.describe(monitors=mons)
+    outs = []
+    for rows, zc, split, blk in ((4, 16, 0, 64), (2, 3, 1, 7), (4, 32, 2, 1), (2, 8, 0, 0)):
+        h = Handle(desc, _lib.engine())
+        for opt, v in ((_lib.MFD_OPT_KERNEL_FORM, 3), (_lib.MFD_OPT_ROWS, rows), (_lib.MFD_OPT_ZCHUNK, zc),
+                       (_lib.MFD_OPT_PML_SPLIT, split), (_lib.MFD_OPT_XCD_BLOCK, blk)):
+            h.lib.check(h.lib.model_set_options(h.h, opt, v), "set_options")
+        h.step(model.Nt)
+        outs.append([h.field(c) for c in ("Hx", "Hy", "Hz", "Dx", "Ex", "Ey", "Ez")] + [h.output(0), h.output(1)]
+                    + [np.ravel(h.monitor(q, "Ez")) for q in range(len(mons))])
+        if len(outs) == 1:
+            ho = Handle(desc, oracle("f64"))
+            ho.step(model.Nt)
+            assert_parity(h, ho, desc, model)
+            two = Handle(desc, _lib.engine())
+            two.lib.check(two.lib.model_set_options(two.h, _lib.MFD_OPT_KERNEL_FORM, 2), "set_options")
+            two.step(model.Nt)
+            errs = field_errors(h, two, 3)
+            assert max(errs.values()) <= 1e-5, errs
+        h.close()
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
+
+
+def test_fused_step_buffers_follow_the_model():
+    """The fused step swaps each model's H/E buffers every step.  A model stepped alone,
+    then in a batch, then alone again (three runners, each holding its own copy of the
+    model descriptor) and read back between the runs gives the bits of one uninterrupted
+    run, for an odd and an even number of steps per leg."""
+    model, mons = cases.spheres_3d(n=30, steps=77)
+    desc = model.describe(monitors=mons)
+    ref = Handle(desc, _lib.engine())
+    ref.step(model.Nt)
+    h = Handle(desc, _lib.engine())
+    h.step(13)
+    h.field("Ex")
+    lib = h.lib
+    b = ctypes.c_void_p()
+    lib.check(lib.batch_create(ctypes.byref(b)), "batch_create")
+    lib.check(lib.batch_add(b, h.h), "batch_add")
+    lib.check(lib.batch_run(b, 24), "batch_run")
+    lib.check(lib.batch_destroy(b), "batch_destroy")
+    h.field("Hy")
+    h.step(model.Nt - 37)
+    for c in ("Hx", "Hy", "Hz", "Dz", "Ex", "Ey", "Ez"):
+        assert np.array_equal(ref.field(c), h.field(c)), c
+    assert np.array_equal(np.ravel(ref.monitor(2, "Ex")), np.ravel(h.monitor(2, "Ex")))
 
 
 def test_chunked_stepping_is_bitwise_identical():
