@@ -191,7 +191,7 @@ def bench_3d(args, lib):
     gsteps = graph_steps_for(args.steps, args.graph_steps)
     set_opts(lib, h, {0: gsteps, 1: args.zchunk, 5: args.ty, 6: args.kform,
                       10: args.snake, 11: args.dless, 12: args.pml_split, 14: args.zchunk_h,
-                      15: args.xcd_h, 16: args.xcd_block})
+                      15: args.xcd_h, 16: args.xcd_block, 18: args.tfsf_fold})
     lib.check(lib.model_upload(h.h), "upload")
     # the first step call builds, instantiates and uploads the step graph (whatever the
     # warm-up length), so the timed steps are whole graph replays
@@ -235,7 +235,8 @@ def bench_3d(args, lib):
                      "cells": N, "model_steps": desc.Nt, "graph_steps": gsteps,
                      "zchunk": args.zchunk, "rows_per_workgroup": args.ty, "kernel_form": args.kform,
                      "snake_order": args.snake, "plain_cells_without_d": args.dless,
-                     "cpml_split": args.pml_split, "zchunk_h": args.zchunk_h, "xcd_chunked_h": args.xcd_h, "xcd_block": args.xcd_block}
+                     "cpml_split": args.pml_split, "zchunk_h": args.zchunk_h, "xcd_chunked_h": args.xcd_h, "xcd_block": args.xcd_block,
+                     "tfsf_fold": args.tfsf_fold}
     if fused:
         achieved = N * (bh + be) / (h_ms * 1e-3) / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -612,6 +613,8 @@ def main():
                     help="3D kernel form: 3 fused step (default), 2 two-pass, 0 one cell per thread")
     ap.add_argument("--snake", type=int, default=1)
     ap.add_argument("--dless", type=int, default=1, help="plain cells keep no D (default 1)")
+    ap.add_argument("--tfsf-fold", type=int, default=1,
+                    help="fused step stores the TFSF x-face H corrections itself (0: the face launch does)")
     ap.add_argument("--pml-split", type=int, default=0,
                     help="tiles outside every CPML layer run the slab-free kernels (default 0)")
     ap.add_argument("--count", type=int, default=None, help="instances / requests")

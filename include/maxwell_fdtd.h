@@ -215,23 +215,32 @@ enum {
     MFD_OPT_PERSISTENT = 2,    /* workgroups per CU of the persistent form (0 = off)       */
     MFD_OPT_KERNEL_TIMING = 3, /* hipEvent pairs around every H and E launch (eager)       */
     MFD_OPT_DFT_BLOCK = 4,     /* time steps per DFT flush (1..64, before upload)          */
-    MFD_OPT_ROWS = 5,          /* y rows per 3D workgroup (1, 2, 4, 8)                     */
-    MFD_OPT_KERNEL_FORM = 6,   /* 3D kernel form: 0 one cell per thread, 2 z march (default)*/
+    MFD_OPT_ROWS = 5,          /* y rows per 3D workgroup (1, 2, 4, 8); fused form: rows  */
+                               /* per lane (2 or 4)                                       */
+    MFD_OPT_KERNEL_FORM = 6,   /* 3D kernel form: 0 one cell per thread, 2 two-pass z      */
+                               /* march, 3 fused H+E step over two buffer sets (default;  */
+                               /* FP32 3D models with mu = 1 everywhere, else form 2)      */
     MFD_OPT_STAGGER = 7,       /* byte offset between field arrays (before upload)         */
     MFD_OPT_QUANTUM = 8,       /* server: steps per scheduling round (default 64)          */
     MFD_OPT_FILL_MCELLS = 9,   /* server: admit while active cells < value * 2^20 (16)     */
     MFD_OPT_SNAKE = 10,        /* E half walks the tile list backwards (default 1)         */
     MFD_OPT_DLESS = 11,        /* plain cells keep no D (rebuilt as E/Me on read; default 1) */
     MFD_OPT_PML_SPLIT = 12,    /* CPML-free tiles in slab-free kernels: 0 off, 1 after the */
-                               /* CPML tiles, 2 beside them on a second stream             */
+                               /* CPML tiles, 2 beside them on a second stream; the fused  */
+                               /* form splits into x-slab-only, other CPML and free tiles */
     MFD_OPT_SCHED = 13,        /* server: admit 0 in arrival order, 1 least work first (1) */
     MFD_OPT_ZCHUNK_H = 14,     /* pipelined 3D H half z planes per tile (8; 0 = ZCHUNK)    */
     MFD_OPT_XCD_H = 15,        /* 3D H tiles dealt to the XCDs like E's (1) or grid order  */
     MFD_OPT_XCD_BLOCK = 16,    /* 3D tiles per block dealt to the XCDs (64; 0: one per XCD) */
-    MFD_OPT_F64 = 17           /* 1D models: 1 keeps fields, CPML, coefficients, ADE /
+    MFD_OPT_F64 = 17,          /* 1D models: 1 keeps fields, CPML, coefficients, ADE /
                                   plasma state, source waveforms, TFSF tables and Sa/E2
                                   in FP64 (the reference's CPU{Float64} run; set before
                                   upload); monitor samples are stored in FP32           */
+    MFD_OPT_TFSF_FOLD = 18     /* fused 3D step with a TFSF source alone: 1 (default)
+                                  stores the x-face H corrections in the step launch, so
+                                  the face launch after it walks only the y/z-face cells;
+                                  0 leaves them to the face launch; results are bitwise
+                                  equal                                                  */
 };
 /* Changing an option rebuilds the launch plan; field state is kept.                   */
 int32_t mfd_model_set_options(mfd_model *m, int32_t opt, int32_t value);

@@ -26,6 +26,19 @@ __device__ __forceinline__ gp<const T> G(const T *p) { return (gp<const T>)p; }
 // square root (v_sqrt_f32, within 1 ulp) rather than the correctly rounded library
 // sequence, which costs about 15 more instructions per cell and step
 __device__ __forceinline__ float sa_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+// Sa and E2 of a 3D cell advanced by one step (src/outputs.jl:475-485), the multiply-adds
+// written out so that the fused step and the TFSF face launch round them alike whatever
+// each file's contraction setting
+__device__ __forceinline__ float sa_step3(float sa, float ex, float ey, float ez, float hx, float hy, float hz,
+                                          float dt) {
+    const float a = __builtin_fmaf(ey, hz, -(ez * hy));
+    const float b = __builtin_fmaf(ez, hx, -(ex * hz));
+    const float c = __builtin_fmaf(ex, hy, -(ey * hx));
+    return __builtin_fmaf(sa_sqrt(__builtin_fmaf(a, a, __builtin_fmaf(b, b, c * c))), dt, sa);
+}
+__device__ __forceinline__ float e2_step3(float e2, float ex, float ey, float ez, float dt) {
+    return __builtin_fmaf(__builtin_fmaf(ex, ex, __builtin_fmaf(ey, ey, ez * ez)), dt, e2);
+}
 // square root in the precision of x: the same hardware form for FP32, correctly rounded
 // for FP64
 __device__ __forceinline__ float sqrt_of(float x) { return sa_sqrt(x); }
@@ -67,16 +80,17 @@ struct DevMat {
     int flags;                  // MAT_DISP | MAT_KERR | MAT_PLASMA
     int nq;                     // ADE poles
     float K2, K3;               // Kerr Pade: Mk2*Me^2, Mk3*Me^3 (src/models.jl:1140-1149)
-    const float *Aq, *Bq, *Cq;  // ade_coefficients (src/materials.jl:509-577)
-    float2 *P;                  // [nq][nc][cnt]: (P^n, P^{n-1})
+    // ADE poles and plasma current: coefficients and state in FP64 (material_e)
+    const double *Aq, *Bq, *Cq; // ade_coefficients (src/materials.jl:509-577)
+    double2 *P;                 // [nq][nc][cnt]: (P^n, P^{n-1})
     int cnt;                    // state slots (cells of this material)
     int rate_kind, rate_n, pad;
     double Rava, ksi, Cprho0, Marho0, rate_c, rate_k, rho0;
-    float Ap, Bp;
+    double Ap, Bp;
     const double *rate_x, *rate_y;
     double2 *rhod;              // [cnt]: (rho, drho), FP64
-    float2 *Pp;                 // [nc][cnt]: (Pp^n, Pp^{n-1})
-    float *Pa;                  // [nc][cnt]
+    double2 *Pp;                // [nc][cnt]: (Pp^n, Pp^{n-1})
+    double *Pa;                 // [nc][cnt]
     float *JE;                  // [cnt]
     const int *cells;           // [cnt]: linear cell index of each state slot (stateful materials)
 };
@@ -172,6 +186,8 @@ struct DevModel {
                                 // (TOUCHED_STATEFUL): the cell's material has ADE/plasma state
     int ntouched;
     const DevSource *xtfsf;     // TFSF source whose x faces the E kernel finishes, or null
+    int xface_h;                // fused 3D step: it also stores the H corrections of the
+                                // x-face cells it finishes (touched then lists the others)
     int dless;                  // plain cells keep no D: D = E/Me is rebuilt on demand
     // FP64 state (1D models with MFD_OPT_F64): the fields are F64[] (F[] is null), the
     // material table is mats64 (mats keeps the FP32 table for the id lookups of the
@@ -250,6 +266,7 @@ void launch_e3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, in
 void launch_step3d_f(DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int rows,
                      int pmask, hipStream_t s);
 void launch_swap_pp(DevModel *m, int nmodels, hipStream_t s);
+void launch_finish_swap(DevModel *m, int nmodels, hipStream_t s);
 void launch_h2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int tw, hipStream_t s);
 void launch_e2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int tw, hipStream_t s);
 void launch_h1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool f64,
@@ -262,7 +279,7 @@ void launch_tfsf(const DevModel *m, int nmodels, int slot, int ndims, int maxfac
                  bool aux);
 void launch_tfsf_aux(const DevModel *m, int nmodels, int slot, hipStream_t s);
 void launch_tfsf_gather(const DevModel *m, int nmodels, int slot, int maxn, bool integ,
-                        hipStream_t s);
+                        hipStream_t s, bool pp = false);
 void launch_tfsf_aux_table(const DevSource *S, int Nz, int Nt, int bcl, int bcr, float *tab,
                            hipStream_t s);
 void launch_monitors(const DevModel *m, int nmodels, int maxn, hipStream_t s);

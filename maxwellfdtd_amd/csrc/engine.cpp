@@ -57,8 +57,8 @@ int64_t Model::footprint() const {
         const auto &md = mats[k].d;
         const long long c = cnt[k + 1];
         if (md.nq > 0 || md.isplasma) { stateful = true; b += c * 4; }
-        b += (int64_t)md.nq * ncomp() * c * 8;
-        if (md.isplasma) b += c * (16 + ncomp() * 12 + 4);
+        b += (int64_t)md.nq * ncomp() * c * 16;
+        if (md.isplasma) b += c * (16 + ncomp() * 24 + (f64 ? 8 : 4));
     }
     if (stateful) b += N * 4;
     for (int f = 0; f < 6; f++)
@@ -298,18 +298,18 @@ int Model::upload() {
             v.cells = cp;
         }
         if (hm.d.nq > 0) {
-            float *p;
-            UP(upload_vec(arena, &p, to_f32(hm.Aq), io)); v.Aq = p;
-            UP(upload_vec(arena, &p, to_f32(hm.Bq), io)); v.Bq = p;
-            UP(upload_vec(arena, &p, to_f32(hm.Cq), io)); v.Cq = p;
+            double *p;
+            UP(upload_vec(arena, &p, hm.Aq, io)); v.Aq = p;
+            UP(upload_vec(arena, &p, hm.Bq, io)); v.Bq = p;
+            UP(upload_vec(arena, &p, hm.Cq, io)); v.Cq = p;
             UP(arena.alloc(&v.P, (size_t)hm.d.nq * nc * c));
-            UP(hipMemsetAsync(v.P, 0, (size_t)hm.d.nq * nc * c * sizeof(float2), io));
+            UP(hipMemsetAsync(v.P, 0, (size_t)hm.d.nq * nc * c * sizeof(double2), io));
         }
         if (hm.d.isplasma) {
             v.Rava = hm.d.Rava; v.ksi = hm.d.ksi; v.rho0 = hm.d.rho0;
             v.Cprho0 = hm.d.Cp * hm.d.rho0;
             v.Marho0 = hm.d.Ma * hm.d.rho0;
-            v.Ap = (float)hm.d.Ap; v.Bp = (float)hm.d.Bp;
+            v.Ap = hm.d.Ap; v.Bp = hm.d.Bp;
             v.rate_kind = hm.d.rate_kind; v.rate_c = hm.d.rate_c; v.rate_k = hm.d.rate_k;
             v.rate_n = (int)hm.rx.size();
             double *dp;
@@ -318,9 +318,9 @@ int Model::upload() {
             UP(arena.alloc(&v.rhod, c));
             UP(hipMemsetAsync(v.rhod, 0, c * sizeof(double2), io));
             UP(arena.alloc(&v.Pp, (size_t)nc * c));
-            UP(hipMemsetAsync(v.Pp, 0, (size_t)nc * c * sizeof(float2), io));
+            UP(hipMemsetAsync(v.Pp, 0, (size_t)nc * c * sizeof(double2), io));
             UP(arena.alloc(&v.Pa, (size_t)nc * c));
-            UP(hipMemsetAsync(v.Pa, 0, (size_t)nc * c * sizeof(float), io));
+            UP(hipMemsetAsync(v.Pa, 0, (size_t)nc * c * sizeof(double), io));
             UP(arena.alloc(&v.JE, c));
             UP(hipMemsetAsync(v.JE, 0, c * sizeof(float), io));
         }
@@ -635,6 +635,32 @@ int Model::ensure_pp() {
     return MFD_OK;
 }
 
+// The touched list without the x-face cells that a fused step finishes, H corrections
+// included (tfsf_xhandled_box: on an x face plane, on no y or z face plane, no ADE or
+// plasma state); the cells keep the stateful bit of the full list.
+int Model::ensure_touched_yz() {
+    if (n_touched_yz >= 0) return MFD_OK;
+    const int *b = hsrc[0].box;
+    std::vector<int> keep;
+    for (int c : touched) {
+        const int ix = (int)(c % d.Nx), iy = (int)((c / d.Nx) % d.Ny), iz = (int)(c / ((long long)d.Nx * d.Ny));
+        const int g = geom[c];
+        const bool st = g > 0 && (mats[g - 1].d.nq > 0 || mats[g - 1].d.isplasma);
+        const bool onx = (ix == b[0] - 1 || ix == b[0] || ix == b[1]) && iy >= b[2] && iy <= b[3] && iz >= b[4] &&
+                         iz <= b[5];
+        const bool ony = (iy == b[2] - 1 || iy == b[2] || iy == b[3]) && ix >= b[0] && ix <= b[1];
+        const bool onz = (iz == b[4] - 1 || iz == b[4] || iz == b[5]) && ix >= b[0] && ix <= b[1];
+        if (onx && !ony && !onz && !st) continue;
+        keep.push_back(st ? (c | TOUCHED_STATEFUL) : c);
+    }
+    int *p = nullptr;
+    MFD_HIP(arena.alloc(&p, keep.size()));
+    if (!keep.empty()) MFD_HIP(copy_sync(p, keep.data(), keep.size() * sizeof(int), hipMemcpyHostToDevice, io));
+    d_touched_yz = p;
+    n_touched_yz = (int)keep.size();
+    return MFD_OK;
+}
+
 // Host mirror of the buffer swap the fused step's last tile makes on the device.
 void Model::swap_pp() {
     const int cs[6] = {HX, HY, HZ, EX, EY, EZ};
@@ -861,6 +887,23 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     const bool xface = tfsf_gather && kform >= 2 && opt.zchunk <= (fused ? FUSED_XFACE_ZC : XFACE_ZC);
     if (xface)
         for (size_t k = 0; k < ms.size(); k++) hd[k].xtfsf = ms[k]->dm.src;
+    // the fused step stores the x faces' H corrections itself, and the gather skips
+    // those cells: their sectors lie 4 * Nx bytes apart, most of the gather's traffic
+    xface_h = xface && fused && opt.tfsf_fold && !aux_needed;
+    if (xface_h) {
+        maxtouched = 1;
+        for (size_t k = 0; k < ms.size(); k++) {
+            int st = ms[k]->ensure_touched_yz();
+            if (st != MFD_OK) return st;
+            hd[k].xface_h = 1;
+            hd[k].touched = ms[k]->d_touched_yz;
+            hd[k].ntouched = ms[k]->n_touched_yz;
+            maxtouched = std::max(maxtouched, ms[k]->n_touched_yz);
+        }
+    }
+    // no launch of the step but the gather (which then reads the new buffers where the
+    // fused launch left them) reads the fields after the fused launch: k_finish swaps
+    finish_swap = fused && !(split_state && nchunks) && !any_mon && (tfsf_gather || (maxsrc == 0 && !any_int));
     dless_models.clear();
     // plain cells without stored D, for a state the caller has not written, on the
     // pipelined 3D kernels and the 2D kernels: D stays stored where the id says Kerr,
@@ -1036,7 +1079,7 @@ int Runner::launch_step(bool timing, bool dft_flush) {
         launch_step3d_f(d_models, d_tiles + n1 + n7, n0, grid_of(n0), mode, any_int, frows, 0, stream);
         launch_step3d_f(d_models, d_tiles + n1, n7, grid_of(n7), mode, any_int, frows, 7, stream);
         if (side) { MFD_HIP(hipEventRecord(join_ev, side)); MFD_HIP(hipStreamWaitEvent(stream, join_ev, 0)); }
-        launch_swap_pp(d_models, nm, stream);
+        if (!finish_swap) launch_swap_pp(d_models, nm, stream);
     }
     else if (ndims == 3 && kform == 2) {
         // side by side (MFD_OPT_PML_SPLIT = 2): the CPML tiles run on the side stream,
@@ -1067,7 +1110,7 @@ int Runner::launch_step(bool timing, bool dft_flush) {
     if (timing) MFD_HIP(hipEventRecord(ev[3], stream));
     if (tfsf_gather) {
         if (aux_needed) launch_tfsf_aux(d_models, nm, 0, stream);
-        launch_tfsf_gather(d_models, nm, 0, maxtouched, any_int, stream);
+        launch_tfsf_gather(d_models, nm, 0, maxtouched, any_int, stream, finish_swap);
     } else {
         for (int s = 0; s < maxsrc; s++) {
             if (src_kind_mask[s] & 1) launch_point_sources(d_models, nm, s, maxsrc_n, stream);
@@ -1077,7 +1120,8 @@ int Runner::launch_step(bool timing, bool dft_flush) {
     if (any_mon) launch_monitors(d_models, nm, maxmon_n, stream);
     if (dft_flush) launch_dft_flush(d_models, nm, maxmon_n, false, stream);
     if (any_int && !tfsf_gather) launch_integrals_touched(d_models, nm, maxtouched, ndims, stream);
-    launch_finish(d_models, nm, stream);
+    if (finish_swap) launch_finish_swap(d_models, nm, stream);
+    else launch_finish(d_models, nm, stream);
     MFD_HIP(hipGetLastError());
     if (timing) {
         MFD_HIP(hipEventSynchronize(ev[3]));
@@ -1297,6 +1341,7 @@ int set_option(RunOptions &o, Model *m, int opt, int value) {
             }
             m->f64 = value != 0;
             break;
+        case MFD_OPT_TFSF_FOLD: o.tfsf_fold = value != 0; break;
         case MFD_OPT_STAGGER:
             if (!m || m->uploaded || value < 0) { set_error("stagger must be >= 0, set on a model before upload"); return MFD_ERR_ARG; }
             m->stagger = value;

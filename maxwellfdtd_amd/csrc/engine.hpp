@@ -105,6 +105,10 @@ struct Model {
     std::vector<int> touched;   // cells written by sources (sorted)
     std::vector<std::vector<int>> state_cells;   // per material: cells holding ADE/plasma state
     bool has_mats = false, has_tfsf = false, has_spectral = false;
+    // the touched cells less the x-face cells a fused step finishes with their H
+    // corrections (the face launch's list then), built on first use
+    int *d_touched_yz = nullptr;
+    int n_touched_yz = -1;
 
     int status = 0;             // MFD_OK or the model's non-finite status
     bool user_set = false;      // the caller wrote a field component
@@ -122,6 +126,7 @@ struct Model {
     int ensure_io();
     int upload();
     int ensure_pp();
+    int ensure_touched_yz();
     void swap_pp();
     int sync();
     int materialize_d();
@@ -149,6 +154,7 @@ struct RunOptions {
     int pml_split = 0;
     bool xcd_h = true;          // 3D H tile list dealt to the XCDs like E's (else grid order)
     int xcd_block = 64;         // 3D: tiles per block dealt to the XCDs (0: one chunk per XCD)
+    bool tfsf_fold = true;      // fused 3D step: x-face H corrections inside the launch
 };
 
 // Advances a fixed set of uploaded models together.
@@ -177,6 +183,10 @@ struct Runner {
     bool f64 = false;           // the models keep their state in FP64 (1D)
     bool split_state = false;   // stateful cells' E computed by the state pass
     bool tfsf_gather = false;   // TFSF corrections + touched integrals in one launch
+    bool xface_h = false;       // the fused step stores the x faces' H corrections too;
+                                // the gather walks the other touched cells only
+    bool finish_swap = false;   // fused step: nothing reads the fields between the step
+                                // launch and k_finish, which then swaps the buffers too
     bool aux_needed = false;    // some TFSF aux run is stepped on the device each step
     int4 *d_chunks = nullptr;   // state-pass work list {model, material, s0, s1}
     int *d_flags = nullptr;     // per-model non-finite flags, gathered

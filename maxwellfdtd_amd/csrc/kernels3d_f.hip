@@ -391,8 +391,9 @@ __global__ __launch_bounds__(64) void k_step3d_f(const DevModel *__restrict__ mo
         for (int r = 0; r < R; r++) { hxm[r] = 0.0f; hym[r] = 0.0f; }
         if (zbm == 2) zm_own = true;
         int nonfin = 0;
-        // x-face TFSF columns of this lane
+        // x-face TFSF columns of this lane; xhs: their H corrections stored here
         const bool xcol = xtile && xf_l;
+        const bool xhs = m.xface_h != 0;
         const int xside = ix == xb[1] ? 1 : 0;
 
         for (int p = pb; p < z1; p++) {
@@ -633,7 +634,8 @@ __global__ __launch_bounds__(64) void k_step3d_f(const DevModel *__restrict__ mo
                         }
                     }
                     // x-face TFSF epilogue (src/sources.jl:725-745); the H correction of the
-                    // H-out cells enters only the integrals here, the face kernel writes it
+                    // H-out cells enters the integrals here, and the stored H too when
+                    // m.xface_h is set (else the face kernel writes it)
                     float hyI = Hn[r].y, hzI = Hn[r].z;
                     bool xh2 = false;
                     if (xcol && iy >= xb[2] && iy <= xb[3] && zg >= xb[4] && zg <= xb[5]) {
@@ -662,19 +664,17 @@ __global__ __launch_bounds__(64) void k_step3d_f(const DevModel *__restrict__ mo
                         }
                     }
                     if (act) {
-                        stb<float>(Hxw, o, Hn[r].x); stb<float>(Hyw, o, Hn[r].y); stb<float>(Hzw, o, Hn[r].z);
+                        stb<float>(Hxw, o, Hn[r].x);
+                        stb<float>(Hyw, o, xhs ? hyI : Hn[r].y);
+                        stb<float>(Hzw, o, xhs ? hzI : Hn[r].z);
                         if (dcr) { stb<float>(Dx, o, d0); stb<float>(Dy, o, d1); stb<float>(Dz, o, d2); }
                         // a cell whose E the state pass finishes keeps E^n, which that pass reads
                         if (defer) { e[0] = ex[r]; e[1] = ey[r]; e[2] = ez[r]; }
                         stb<float>(Exw, o, e[0]); stb<float>(Eyw, o, e[1]); stb<float>(Ezw, o, e[2]);
                         if (!defer && !isfinite(e[0])) nonfin |= isnan(e[0]) ? 1 : 2;
                         if (INTEG && !defer && (!(gg & ID_TOUCHED) || xh2)) {   // src/outputs.jl:480-483
-                            if (doSa) {
-                                const float a = e[1] * hzI - e[2] * hyI, b = e[2] * Hn[r].x - e[0] * hzI;
-                                const float cc = e[0] * hyI - e[1] * Hn[r].x;
-                                stnt(Sa, o, sa[r] + sa_sqrt(a * a + b * b + cc * cc) * dtf);
-                            }
-                            if (doE2) stnt(E2, o, e2[r] + (e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) * dtf);
+                            if (doSa) stnt(Sa, o, sa_step3(sa[r], e[0], e[1], e[2], Hn[r].x, hyI, hzI, dtf));
+                            if (doE2) stnt(E2, o, e2_step3(e2[r], e[0], e[1], e[2], dtf));
                         }
                     }
                 }
@@ -718,6 +718,30 @@ __global__ void k_swap_pp(DevModel *models, int nmodels) {
     }
 }
 
+// k_finish and k_swap_pp in one launch, for steps in which nothing reads the fields
+// after the fused launch: the buffer swap, the step counter and the DFT block position
+__global__ void k_finish_swap(DevModel *models, int nmodels) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nmodels) return;
+    DevModel &m = models[k];
+    const int it = *m.it;
+    if (it >= m.Nt) return;
+    const int cs[6] = {HX, HY, HZ, EX, EY, EZ};
+    for (int c = 0; c < 6; c++) {
+        float *a = m.F[cs[c]];
+        m.F[cs[c]] = m.Falt[cs[c]];
+        m.Falt[cs[c]] = a;
+    }
+    for (int f = 0; f < 6; f++) {
+        DevLayer &L = m.L[f];
+        float *a0 = L.psiE0, *a1 = L.psiE1;
+        L.psiE0 = L.psiE0b; L.psiE1 = L.psiE1b;
+        L.psiE0b = a0; L.psiE1b = a1;
+    }
+    *m.it = it + 1;
+    if (it + 1 - *m.dftdone == m.dft_block) *m.dftdone = it + 1;
+}
+
 template <int R, int PM>
 static void lf(DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, hipStream_t s) {
 #define SF_(MD, IN) hipLaunchKernelGGL((k_step3d_f<MD, IN, R, PM>), dim3(grid), dim3(64), 0, s, m, tiles, ntiles)
@@ -729,6 +753,10 @@ static void lf(DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, b
 
 void launch_swap_pp(DevModel *m, int nmodels, hipStream_t s) {
     hipLaunchKernelGGL(k_swap_pp, dim3((nmodels + 63) / 64), dim3(64), 0, s, m, nmodels);
+}
+
+void launch_finish_swap(DevModel *m, int nmodels, hipStream_t s) {
+    hipLaunchKernelGGL(k_finish_swap, dim3((nmodels + 63) / 64), dim3(64), 0, s, m, nmodels);
 }
 
 // pmask: the CPML axes the tiles' cells (and recomputed halo cells) may lie in; the

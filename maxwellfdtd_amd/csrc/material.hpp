@@ -35,33 +35,33 @@ __device__ inline double ionrate(MR &mt, double I) {
     return 0.0;
 }
 
-template <class T> struct vec2;
-template <> struct vec2<float> { typedef v2f type; };
-template <> struct vec2<double> { typedef v2d type; };
-
 // d[c]: updated D; e[c]: on entry E^n (read only when the material has ADE/plasma
 // state), on exit E^{n+1}.  s is the cell's state slot within its material.  mt is a
 // DevMat (T = float) or a DevMat64 (T = double), generic or global-address-space; dtT
-// is dt in T.  The plasma density and rate are FP64 either way.
+// is dt in T.  Whatever T, the pole and plasma-current recursions run in FP64 on FP64
+// state and coefficients, and D - sum P is formed in FP64: a Drude pole (and the plasma
+// current) has a root at exactly 1, whose FP32 rounding of the coefficients (A + B off 1
+// by ~1e-7) and of the state the other root, 1 - gamma dt, amplifies ~1/(gamma dt) times
+// into a drift of P and D over thousands of steps.  The plasma density and rate are
+// FP64 as well.
 template <int NC, class T, class MR>
 __device__ __forceinline__ void material_e(MR &mt, int s, const T (&d)[NC], T (&e)[NC],
                                            double dt, T dtT) {
-    typedef typename vec2<T>::type V2;
     const int flags = mt.flags;
-    T sumP[NC];
+    double sumP[NC];
 #pragma unroll
-    for (int c = 0; c < NC; c++) sumP[c] = (T)0;
+    for (int c = 0; c < NC; c++) sumP[c] = 0.0;
     if (flags & MAT_DISP) {
         const int nq = mt.nq, cnt = mt.cnt;
         for (int q = 0; q < nq; q++) {
-            const T A = (T)G(mt.Aq)[q], B = (T)G(mt.Bq)[q], Cc = (T)G(mt.Cq)[q];
+            const double A = G(mt.Aq)[q], B = G(mt.Bq)[q], Cc = G(mt.Cq)[q];
 #pragma unroll
             for (int c = 0; c < NC; c++) {
                 // pole state is read and written once per step: streamed past the caches
-                gp<V2> pp = (gp<V2>)(mt.P + (size_t)(q * NC + c) * cnt + s);
-                const V2 p = __builtin_nontemporal_load(pp);
-                const T pn = A * p.x + B * p.y + Cc * e[c];
-                V2 np;
+                gp<v2d> pp = (gp<v2d>)(mt.P + (size_t)(q * NC + c) * cnt + s);
+                const v2d p = __builtin_nontemporal_load(pp);
+                const double pn = A * p.x + B * p.y + Cc * (double)e[c];
+                v2d np;
                 np.x = pn;
                 np.y = p.x;
                 __builtin_nontemporal_store(np, pp);
@@ -77,26 +77,26 @@ __device__ __forceinline__ void material_e(MR &mt, int s, const T (&d)[NC], T (&
         const double II = mt.ksi * E2;
         gp<v2d> prd = (gp<v2d>)(mt.rhod + s);
         v2d rd = __builtin_nontemporal_load(prd);
-        const T cp = (T)(mt.Cprho0 * rd.x);
-        T J[NC];
+        const double cp = mt.Cprho0 * rd.x;
+        double J[NC];
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            gp<V2> pp = (gp<V2>)(mt.Pp + (size_t)c * cnt + s);
-            const V2 p = __builtin_nontemporal_load(pp);
-            const T pn = (T)mt.Ap * p.x + (T)mt.Bp * p.y + cp * e[c];
-            V2 np;
+            gp<v2d> pp = (gp<v2d>)(mt.Pp + (size_t)c * cnt + s);
+            const v2d p = __builtin_nontemporal_load(pp);
+            const double pn = (double)mt.Ap * p.x + (double)mt.Bp * p.y + cp * (double)e[c];
+            v2d np;
             np.x = pn;
             np.y = p.x;
             __builtin_nontemporal_store(np, pp);
             sumP[c] += pn;
-            J[c] = (pn - p.y) / ((T)2 * dtT);
+            J[c] = (pn - p.y) / (2.0 * dt);
         }
         const double invE2 = (E2 >= DBL_EPS_) ? 1.0 / E2 : 0.0;
-        const T ca = (T)(mt.Marho0 * rd.y * invE2);
+        const double ca = mt.Marho0 * rd.y * invE2;
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            gp<T> pa = G(mt.Pa) + (size_t)c * cnt + s;
-            const T v = __builtin_nontemporal_load(pa) + ca * e[c];
+            gp<double> pa = G(mt.Pa) + (size_t)c * cnt + s;
+            const double v = __builtin_nontemporal_load(pa) + ca * (double)e[c];
             __builtin_nontemporal_store(v, pa);
             sumP[c] += v;
         }
@@ -114,21 +114,21 @@ __device__ __forceinline__ void material_e(MR &mt, int s, const T (&d)[NC], T (&
         __builtin_nontemporal_store(rd, prd);
         T je = (T)0;
 #pragma unroll
-        for (int c = 0; c < NC; c++) je += J[c] * e[c];
+        for (int c = 0; c < NC; c++) je += (T)J[c] * e[c];
         __builtin_nontemporal_store(__builtin_nontemporal_load(&G(mt.JE)[s]) + je * dtT, &G(mt.JE)[s]);
     }
-    const T Me = (T)mt.Me;
+    const double Me = (double)mt.Me;
     if (flags & MAT_KERR) {
-        const T K2 = (T)mt.K2, K3 = (T)mt.K3;
+        const T K2 = (T)mt.K2, K3 = (T)mt.K3, MeT = (T)mt.Me;
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            const T u = d[c] - sumP[c];
+            const T u = (T)((double)d[c] - sumP[c]);
             e[c] = ((T)1 + K2 * u + (T)2 * K3 * (u * u)) /
-                   ((T)1 + (T)2 * K2 * u + (T)3 * K3 * (u * u)) * u * Me;
+                   ((T)1 + (T)2 * K2 * u + (T)3 * K3 * (u * u)) * u * MeT;
         }
     } else {
 #pragma unroll
-        for (int c = 0; c < NC; c++) e[c] = (d[c] - sumP[c]) * Me;
+        for (int c = 0; c < NC; c++) e[c] = (T)(((double)d[c] - sumP[c]) * Me);
     }
 }
 

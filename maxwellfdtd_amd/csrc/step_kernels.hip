@@ -306,7 +306,7 @@ __global__ __launch_bounds__(256) void k_tfsf_axis(const DevModel *__restrict__ 
 // kernel finishes x-face cells itself (m.xtfsf set), such a cell only gets its pending
 // H correction here.
 __global__ __launch_bounds__(256) void k_tfsf_gather(const DevModel *__restrict__ models,
-                                                     int slot, int integ) {
+                                                     int slot, int integ, int pp) {
     const DevModel &m = models[blockIdx.y];
     const int it0 = *m.it;
     if (it0 >= m.Nt) return;
@@ -322,76 +322,30 @@ __global__ __launch_bounds__(256) void k_tfsf_gather(const DevModel *__restrict_
                            (int)(i / ((unsigned)Nx * (unsigned)Ny))};
         bool xh = false;
         if (m.xtfsf) xh = tfsf_xhandled(S, c3[0], c3[1], c3[2], (tk & TOUCHED_STATEFUL) != 0);
+        // pp: a fused step ran and its buffer swap comes after this launch, so this
+        // step's H and E are in Falt (D stays in place)
+        float *fp[9];
+#pragma unroll
+        for (int c = 0; c < 9; c++) fp[c] = pp && (c < DX || c > DZ) ? m.Falt[c] : m.F[c];
         float f[9];
         if (xh) {
-            f[HY] = G((const float *)m.F[HY])[i];
-            f[HZ] = G((const float *)m.F[HZ])[i];
+            f[HY] = G((const float *)fp[HY])[i];
+            f[HZ] = G((const float *)fp[HZ])[i];
         } else {
 #pragma unroll
-            for (int c = 0; c < 9; c++) f[c] = G((const float *)m.F[c])[i];
+            for (int c = 0; c < 9; c++) f[c] = G((const float *)fp[c])[i];
         }
-        unsigned dirty = 0;
-#pragma unroll
-        for (int axis = 0; axis < 3; axis++) {
-            const int ua = axis == 0 ? 1 : 0, va = axis == 2 ? 1 : 2;
-            const int U1 = S.box[2 * ua], U2 = S.box[2 * ua + 1];
-            const int V1 = S.box[2 * va], V2 = S.box[2 * va + 1];
-            const int N1 = S.box[2 * axis], N2 = S.box[2 * axis + 1];
-            const int u = c3[ua], v = c3[va], n = c3[axis];
-            if (u < U1 || u > U2 || v < V1 || v > V2) continue;
-            if (xh && axis > 0) continue;
-            constexpr int cH[3][2] = {{HY, HZ}, {HX, HZ}, {HX, HY}};
-            constexpr int cE[3][2] = {{EY, EZ}, {EX, EZ}, {EX, EY}};
-            constexpr int cD[3][2] = {{DY, DZ}, {DX, DZ}, {DX, DY}};
-            const int cHu = cH[axis][0], cHv = cH[axis][1], cEu = cE[axis][0], cEv = cE[axis][1];
-            const int cDu = cD[axis][0], cDv = cD[axis][1];
-            const float cm = S.cm[axis], ce = S.ce[axis], cd = S.cd[axis];
-#pragma unroll
-            for (int side = 0; side < 2; side++) {
-                const int nH = side ? N2 : N1 - 1, nE = side ? N2 : N1;
-                if (n != nH && n != nE) continue;
-                const float sg = side ? -1.0f : 1.0f;
-                const float s0 = (axis == 1 ? 1.0f : -1.0f) * sg;
-                if (n == nH) {
-                    if (u <= U2 && v <= V2 - 1 && tfsf_live(S, axis, 3)) {
-                        f[cHu] = __builtin_fmaf(s0 * cm, tfsf_inc(S, m.Nz, axis, side, 3, u, v, it0), f[cHu]);
-                        dirty |= 1u << cHu;
-                    }
-                    if (u <= U2 - 1 && v <= V2 && tfsf_live(S, axis, 2)) {
-                        f[cHv] = __builtin_fmaf(-s0 * cm, tfsf_inc(S, m.Nz, axis, side, 2, u, v, it0), f[cHv]);
-                        dirty |= 1u << cHv;
-                    }
-                }
-                if (n == nE && !xh) {
-                    if (u <= U2 - 1 && v <= V2 && tfsf_live(S, axis, 1)) {
-                        const float iv = tfsf_inc(S, m.Nz, axis, side, 1, u, v, it0);
-                        f[cEu] = __builtin_fmaf(-s0 * ce, iv, f[cEu]);
-                        f[cDu] = __builtin_fmaf(-s0 * cd, iv, f[cDu]);
-                        dirty |= (1u << cEu) | (1u << cDu);
-                    }
-                    if (u <= U2 && v <= V2 - 1 && tfsf_live(S, axis, 0)) {
-                        const float iv = tfsf_inc(S, m.Nz, axis, side, 0, u, v, it0);
-                        f[cEv] = __builtin_fmaf(s0 * ce, iv, f[cEv]);
-                        f[cDv] = __builtin_fmaf(s0 * cd, iv, f[cDv]);
-                        dirty |= (1u << cEv) | (1u << cDv);
-                    }
-                }
-            }
-        }
+        const unsigned dirty = tfsf_cell_fix(S, m.Nz, it0, c3, xh, f);
 #pragma unroll
         for (int c = 0; c < 9; c++)
-            if (dirty & (1u << c)) G(m.F[c])[i] = f[c];
+            if (dirty & (1u << c)) G(fp[c])[i] = f[c];
         if (xh) {
             active = false;     // integrals and the Ex check were done by the E kernel
         } else {
             if (integ) {   // src/outputs.jl:475-485
                 const float dt = m.dtf;
-                if (m.Sa) {
-                    const float a = f[EY] * f[HZ] - f[EZ] * f[HY], b = f[EZ] * f[HX] - f[EX] * f[HZ];
-                    const float cc = f[EX] * f[HY] - f[EY] * f[HX];
-                    G(m.Sa)[i] += sa_sqrt(a * a + b * b + cc * cc) * dt;
-                }
-                if (m.E2) G(m.E2)[i] += (f[EX] * f[EX] + f[EY] * f[EY] + f[EZ] * f[EZ]) * dt;
+                if (m.Sa) G(m.Sa)[i] = sa_step3(G(m.Sa)[i], f[EX], f[EY], f[EZ], f[HX], f[HY], f[HZ], dt);
+                if (m.E2) G(m.E2)[i] = e2_step3(G(m.E2)[i], f[EX], f[EY], f[EZ], dt);
             }
             exf = f[EX];
         }
@@ -400,9 +354,9 @@ __global__ __launch_bounds__(256) void k_tfsf_gather(const DevModel *__restrict_
 }
 
 void launch_tfsf_gather(const DevModel *m, int nmodels, int slot, int maxn, bool integ,
-                        hipStream_t s) {
+                        hipStream_t s, bool pp) {
     const int nb = std::max(1, (maxn + 255) / 256);
-    hipLaunchKernelGGL(k_tfsf_gather, dim3(nb, nmodels), dim3(256), 0, s, m, slot, integ ? 1 : 0);
+    hipLaunchKernelGGL(k_tfsf_gather, dim3(nb, nmodels), dim3(256), 0, s, m, slot, integ ? 1 : 0, pp ? 1 : 0);
 }
 
 void launch_tfsf_aux(const DevModel *m, int nmodels, int slot, hipStream_t s) {

@@ -93,6 +93,77 @@ __device__ __forceinline__ bool tfsf_touched_box(const int (&b)[6], int ix, int 
     return fx || fy || fz;
 }
 
+// Every correction that lands on cell c3, applied to its fields f[] in the reference's face
+// order (x, y, z; lower side, upper side; src/sources.jl:725-774).  b is the box, co the
+// factors (cm, ce, cd) per axis, live(axis, kk) whether a table carries values and
+// inc(axis, side, kk, u, v) the incident value.  With xh (an x-face cell whose E, D and
+// integrals the E half-step epilogue finished) only the x faces' H corrections.  Returns
+// the mask of the components written.
+template <class LIVE, class INC>
+__device__ __forceinline__ unsigned cell_fix(const int (&b)[6], const float (&co)[3][3], LIVE live, INC inc,
+                                             const int (&c3)[3], bool xh, float (&f)[9]) {
+    unsigned dirty = 0;
+#pragma unroll
+    for (int axis = 0; axis < 3; axis++) {
+        const int ua = axis == 0 ? 1 : 0, va = axis == 2 ? 1 : 2;
+        const int U1 = b[2 * ua], U2 = b[2 * ua + 1];
+        const int V1 = b[2 * va], V2 = b[2 * va + 1];
+        const int N1 = b[2 * axis], N2 = b[2 * axis + 1];
+        const int u = c3[ua], v = c3[va], n = c3[axis];
+        if (u < U1 || u > U2 || v < V1 || v > V2) continue;
+        if (xh && axis > 0) continue;
+        constexpr int cH[3][2] = {{HY, HZ}, {HX, HZ}, {HX, HY}};
+        constexpr int cE[3][2] = {{EY, EZ}, {EX, EZ}, {EX, EY}};
+        constexpr int cD[3][2] = {{DY, DZ}, {DX, DZ}, {DX, DY}};
+        const int cHu = cH[axis][0], cHv = cH[axis][1], cEu = cE[axis][0], cEv = cE[axis][1];
+        const int cDu = cD[axis][0], cDv = cD[axis][1];
+        const float cm = co[axis][0], ce = co[axis][1], cd = co[axis][2];
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const int nH = side ? N2 : N1 - 1, nE = side ? N2 : N1;
+            if (n != nH && n != nE) continue;
+            const float sg = side ? -1.0f : 1.0f;
+            const float s0 = (axis == 1 ? 1.0f : -1.0f) * sg;
+            if (n == nH) {
+                if (u <= U2 && v <= V2 - 1 && live(axis, 3)) {
+                    f[cHu] = __builtin_fmaf(s0 * cm, inc(axis, side, 3, u, v), f[cHu]);
+                    dirty |= 1u << cHu;
+                }
+                if (u <= U2 - 1 && v <= V2 && live(axis, 2)) {
+                    f[cHv] = __builtin_fmaf(-s0 * cm, inc(axis, side, 2, u, v), f[cHv]);
+                    dirty |= 1u << cHv;
+                }
+            }
+            if (n == nE && !xh) {
+                if (u <= U2 - 1 && v <= V2 && live(axis, 1)) {
+                    const float iv = inc(axis, side, 1, u, v);
+                    f[cEu] = __builtin_fmaf(-s0 * ce, iv, f[cEu]);
+                    f[cDu] = __builtin_fmaf(-s0 * cd, iv, f[cDu]);
+                    dirty |= (1u << cEu) | (1u << cDu);
+                }
+                if (u <= U2 && v <= V2 - 1 && live(axis, 0)) {
+                    const float iv = inc(axis, side, 0, u, v);
+                    f[cEv] = __builtin_fmaf(s0 * ce, iv, f[cEv]);
+                    f[cDv] = __builtin_fmaf(s0 * cd, iv, f[cDv]);
+                    dirty |= (1u << cEv) | (1u << cDv);
+                }
+            }
+        }
+    }
+    return dirty;
+}
+
+// cell_fix with the tables, factors and box of source S
+template <class SR>
+__device__ __forceinline__ unsigned tfsf_cell_fix(SR &S, int Nz, int it0, const int (&c3)[3], bool xh, float (&f)[9]) {
+    const int b[6] = {S.box[0], S.box[1], S.box[2], S.box[3], S.box[4], S.box[5]};
+    const float co[3][3] = {{S.cm[0], S.ce[0], S.cd[0]}, {S.cm[1], S.ce[1], S.cd[1]}, {S.cm[2], S.ce[2], S.cd[2]}};
+    return cell_fix(
+        b, co, [&](int axis, int kk) { return tfsf_live(S, axis, kk); },
+        [&](int axis, int side, int kk, int u, int v) { return tfsf_inc(S, Nz, axis, side, kk, u, v, it0); }, c3, xh,
+        f);
+}
+
 template <class SR>
 __device__ __forceinline__ bool tfsf_xhandled(SR &S, int ix, int iy, int iz, bool stateful) {
     const int b[6] = {S.box[0], S.box[1], S.box[2], S.box[3], S.box[4], S.box[5]};

@@ -325,11 +325,12 @@ def slab_1d(nz=600, steps=900, dispersive=True, kmax=None):
     return model, mons
 
 
-def tfsf_faces_3d(n=40, steps=150, pml=4, lo=9, hi=30):
+def tfsf_faces_3d(n=40, steps=150, pml=4, lo=9, hi=30, stateful=True):
     """A TFSF-only 3D model whose box faces cut through materials: a Lorentz sphere
     centred on the lower x face, a dielectric sphere on the upper x face, a Debye sphere
     on the x/z edge and a Kerr sphere on a y face, so that face cells hold stateless,
-    stateful and nonlinear materials.  Box = cells lo..hi (0-based) on every axis."""
+    stateful and nonlinear materials (stateful=False: the Lorentz and Debye spheres are
+    plain dielectrics).  Box = cells lo..hi (0-based) on every axis."""
     d = 40e-9
     L = (n - 1) * d
     grid = Grid(xmin=0, xmax=L, Nx=n, ymin=0, ymax=L, Ny=n, zmin=0, zmax=L, Nz=n)
@@ -340,12 +341,12 @@ def tfsf_faces_3d(n=40, steps=150, pml=4, lo=9, hi=30):
         return lambda X, Y, Z: (X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2 <= r ** 2
 
     w0 = 2 * np.pi * C0 / 800e-9
+    chi_l = LorentzSusceptibility(deps=1.0, w0=1.2 * w0, delta=1e14) if stateful else None
+    chi_d = DebyeSusceptibility(deps=1.5, tau=5e-15) if stateful else None
     mats = [
-        Material(geometry=ball((x[lo], mid, mid), 4 * d), eps=1.8,
-                 chi=LorentzSusceptibility(deps=1.0, w0=1.2 * w0, delta=1e14)),
+        Material(geometry=ball((x[lo], mid, mid), 4 * d), eps=1.8, chi=chi_l),
         Material(geometry=ball((x[hi], mid + 3 * d, mid), 4 * d), eps=3.0),
-        Material(geometry=ball((x[lo], mid, x[lo]), 3.5 * d), eps=2.0,
-                 chi=DebyeSusceptibility(deps=1.5, tau=5e-15)),
+        Material(geometry=ball((x[lo], mid, x[lo]), 3.5 * d), eps=2.0, chi=chi_d),
         Material(geometry=ball((mid, x[lo], mid + 4 * d), 3.5 * d), eps=2.1, chi3=1e-19),
     ]
     w = gauss_waveform(lam0=800e-9, tau0=5e-15, dt0=12e-15)

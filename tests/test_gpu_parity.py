@@ -206,6 +206,44 @@ def test_tfsf_faces_through_materials(kform):
     assert np.max(np.abs(ho.field("Ex"))) > 1e6       # the pulse is in the box
 
 
+@pytest.mark.parametrize("case", ["plane_wave", "plane_wave_monitors", "tables", "kerr_faces", "stateful_faces"])
+def test_tfsf_fold_is_bitwise_neutral(case):
+    """With MFD_OPT_TFSF_FOLD = 1 (the default) the fused step stores the corrected H of
+    the x-face cells it finishes, and the face launch after it walks only the other
+    touched cells (y and z faces, edges, cells with ADE state); with 0 the face launch
+    also writes those H corrections.  The two give the same bits, with 2 and 4 rows per
+    lane and z chunks that put the z faces on tile edges, for plane-wave and table
+    sources, Kerr cells on a face, and stateful cells on the faces.  The first run also
+    matches the FP64 oracle."""
+    if case.startswith("plane_wave"):
+        model, mons = cases.spheres_3d(n=38, nx=70, steps=70, soft=False, nspheres=6)
+        mons = mons if case == "plane_wave_monitors" else None
+    elif case == "tables":
+        model, mons = cases.tfsf_tables_model(3), None
+    else:
+        model, mons = cases.tfsf_faces_3d(n=36, steps=90, lo=8, hi=26, stateful=case == "stateful_faces")
+    desc = model.describe(monitors=mons, integrals=3)
+    outs = []
+    for fold, rows, zc in ((1, 4, 16), (0, 4, 16), (1, 2, 7), (0, 2, 9)):
+        h = Handle(desc, _lib.engine())
+        for opt, v in ((_lib.MFD_OPT_KERNEL_FORM, 3), (_lib.MFD_OPT_TFSF_FOLD, fold), (_lib.MFD_OPT_ROWS, rows),
+                       (_lib.MFD_OPT_ZCHUNK, zc)):
+            h.lib.check(h.lib.model_set_options(h.h, opt, v), "set_options")
+        h.step(model.Nt)
+        outs.append([h.field(c) for c in ("Hx", "Hy", "Hz", "Dx", "Dy", "Dz", "Ex", "Ey", "Ez")]
+                    + [h.output(0), h.output(1)]
+                    + [np.ravel(h.monitor(q, c)) for q in range(len(mons or [])) for c in ("Ex", "Hy")])
+        if len(outs) == 1:
+            ho = Handle(desc, oracle("f64"))
+            ho.step(model.Nt)
+            assert_parity(h, ho, desc, model)
+            assert np.max(np.abs(ho.field("Ex"))) > 0
+        h.close()
+    for o in outs[1:]:
+        for k, (a, b) in enumerate(zip(outs[0], o)):
+            assert np.array_equal(a, b), k
+
+
 def test_plain_tfsf_without_stored_d():
     """TFSF as the only source with plain dielectrics: the E kernel keeps no D for plain
     cells (D = E/Me, rebuilt on read).  Parity against the oracle on H, D and E, the same
