@@ -82,14 +82,14 @@ struct DevMat {
     float K2, K3;               // Kerr Pade: Mk2*Me^2, Mk3*Me^3 (src/models.jl:1140-1149)
     // ADE poles and plasma current: coefficients and state in FP64 (material_e)
     const double *Aq, *Bq, *Cq; // ade_coefficients (src/materials.jl:509-577)
-    double2 *P;                 // [nq][nc][cnt]: (P^n, P^{n-1})
+    double *P;                  // [2][nq][nc][cnt]: half (it & 1) P^n, the other P^{n-1}
     int cnt;                    // state slots (cells of this material)
     int rate_kind, rate_n, pad;
     double Rava, ksi, Cprho0, Marho0, rate_c, rate_k, rho0;
     double Ap, Bp;
     const double *rate_x, *rate_y;
     double2 *rhod;              // [cnt]: (rho, drho), FP64
-    double2 *Pp;                // [nc][cnt]: (Pp^n, Pp^{n-1})
+    double *Pp;                 // [2][nc][cnt]: half (it & 1) Pp^n, the other Pp^{n-1}
     double *Pa;                 // [nc][cnt]
     float *JE;                  // [cnt]
     const int *cells;           // [cnt]: linear cell index of each state slot (stateful materials)
@@ -102,14 +102,14 @@ struct DevMat64 {
     int flags, nq;
     double K2, K3;
     const double *Aq, *Bq, *Cq;
-    double2 *P;                 // [nq][nc][cnt]: (P^n, P^{n-1})
+    double *P;                  // [2][nq][nc][cnt], as DevMat's
     int cnt;
     int rate_kind, rate_n, pad;
     double Rava, ksi, Cprho0, Marho0, rate_c, rate_k, rho0;
     double Ap, Bp;
     const double *rate_x, *rate_y;
     double2 *rhod;              // [cnt]: (rho, drho)
-    double2 *Pp;                // [nc][cnt]: (Pp^n, Pp^{n-1})
+    double *Pp;                 // [2][nc][cnt], as DevMat's
     double *Pa;                 // [nc][cnt]
     double *JE;                 // [cnt]
 };

@@ -302,8 +302,8 @@ int Model::upload() {
             UP(upload_vec(arena, &p, hm.Aq, io)); v.Aq = p;
             UP(upload_vec(arena, &p, hm.Bq, io)); v.Bq = p;
             UP(upload_vec(arena, &p, hm.Cq, io)); v.Cq = p;
-            UP(arena.alloc(&v.P, (size_t)hm.d.nq * nc * c));
-            UP(hipMemsetAsync(v.P, 0, (size_t)hm.d.nq * nc * c * sizeof(double2), io));
+            UP(arena.alloc(&v.P, (size_t)2 * hm.d.nq * nc * c));
+            UP(hipMemsetAsync(v.P, 0, (size_t)2 * hm.d.nq * nc * c * sizeof(double), io));
         }
         if (hm.d.isplasma) {
             v.Rava = hm.d.Rava; v.ksi = hm.d.ksi; v.rho0 = hm.d.rho0;
@@ -317,8 +317,8 @@ int Model::upload() {
             UP(upload_vec(arena, &dp, hm.ry, io)); v.rate_y = dp;
             UP(arena.alloc(&v.rhod, c));
             UP(hipMemsetAsync(v.rhod, 0, c * sizeof(double2), io));
-            UP(arena.alloc(&v.Pp, (size_t)nc * c));
-            UP(hipMemsetAsync(v.Pp, 0, (size_t)nc * c * sizeof(double2), io));
+            UP(arena.alloc(&v.Pp, (size_t)2 * nc * c));
+            UP(hipMemsetAsync(v.Pp, 0, (size_t)2 * nc * c * sizeof(double), io));
             UP(arena.alloc(&v.Pa, (size_t)nc * c));
             UP(hipMemsetAsync(v.Pa, 0, (size_t)nc * c * sizeof(double), io));
             UP(arena.alloc(&v.JE, c));
@@ -346,8 +346,8 @@ int Model::upload() {
                 UP(upload_vec(arena, &p, hm.Aq, io)); v.Aq = p;
                 UP(upload_vec(arena, &p, hm.Bq, io)); v.Bq = p;
                 UP(upload_vec(arena, &p, hm.Cq, io)); v.Cq = p;
-                UP(arena.alloc(&v.P, (size_t)hm.d.nq * nc * c));
-                UP(hipMemsetAsync(v.P, 0, (size_t)hm.d.nq * nc * c * sizeof(double2), io));
+                UP(arena.alloc(&v.P, (size_t)2 * hm.d.nq * nc * c));
+                UP(hipMemsetAsync(v.P, 0, (size_t)2 * hm.d.nq * nc * c * sizeof(double), io));
             }
             if (hm.d.isplasma) {
                 const DevMat &f = hmats[k + 1];
@@ -356,8 +356,8 @@ int Model::upload() {
                 v.rate_kind = f.rate_kind; v.rate_c = f.rate_c; v.rate_k = f.rate_k;
                 v.rate_n = f.rate_n; v.rate_x = f.rate_x; v.rate_y = f.rate_y;
                 v.rhod = f.rhod;
-                UP(arena.alloc(&v.Pp, (size_t)nc * c));
-                UP(hipMemsetAsync(v.Pp, 0, (size_t)nc * c * sizeof(double2), io));
+                UP(arena.alloc(&v.Pp, (size_t)2 * nc * c));
+                UP(hipMemsetAsync(v.Pp, 0, (size_t)2 * nc * c * sizeof(double), io));
                 UP(arena.alloc(&v.Pa, (size_t)nc * c));
                 UP(hipMemsetAsync(v.Pa, 0, (size_t)nc * c * sizeof(double), io));
                 UP(arena.alloc(&v.JE, c));

@@ -116,9 +116,9 @@ __global__ __launch_bounds__(256) void k_e1d(const DevModel *__restrict__ models
         T e[1] = {(T)0};
         if (MODE == MODE_FULL && (mt.flags & (MAT_DISP | MAT_PLASMA))) {
             e[0] = Ex[iz];
-            material_e<1>(mt, m.slot[iz], d, e, m.dt, P::dt(m));
+            material_e<1>(mt, m.slot[iz], *m.it & 1, d, e, m.dt, P::dt(m));
         } else if (MODE == MODE_FULL && (mt.flags & MAT_KERR)) {
-            material_e<1>(mt, 0, d, e, m.dt, P::dt(m));
+            material_e<1>(mt, 0, 0, d, e, m.dt, P::dt(m));
         } else {
             e[0] = d[0] * (T)mt.Me;
         }

@@ -299,9 +299,9 @@ __global__ __launch_bounds__(256) void k_e3d(const DevModel *__restrict__ models
             float e[3] = {0.0f, 0.0f, 0.0f};
             if (MODE == MODE_FULL && (mt.flags & (MAT_DISP | MAT_PLASMA))) {
                 e[0] = Ex[i]; e[1] = Ey[i]; e[2] = Ez[i];
-                material_e<3>(mt, m.slot[i], d, e, m.dt, m.dtf);
+                material_e<3>(mt, m.slot[i], *m.it & 1, d, e, m.dt, m.dtf);
             } else if (MODE == MODE_FULL && (mt.flags & MAT_KERR)) {
-                material_e<3>(mt, 0, d, e, m.dt, m.dtf);
+                material_e<3>(mt, 0, 0, d, e, m.dt, m.dtf);
             } else {
                 const float Me = mt.Me;
                 e[0] = d[0] * Me; e[1] = d[1] * Me; e[2] = d[2] * Me;

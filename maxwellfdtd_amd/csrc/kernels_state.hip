@@ -17,7 +17,8 @@ __global__ __launch_bounds__(256) void k_e_state(const DevModel *__restrict__ mo
                                                  const int4 *__restrict__ chunks, int nchunks) {
     const int4 ch = chunks[blockIdx.x];
     const DevModel &m = models[ch.x];
-    if (*m.it >= m.Nt) return;
+    const int it = *m.it;
+    if (it >= m.Nt) return;
     // the material record through the global address space: its scalars are uniform
     // over the block and load as such, with no flat access in the per-lane work
     const __attribute__((address_space(1))) DevMat &mt =
@@ -37,7 +38,7 @@ __global__ __launch_bounds__(256) void k_e_state(const DevModel *__restrict__ mo
             d[c] = G((const float *)m.F[cD[c]])[i];
             e[c] = G((const float *)m.F[cE[c]])[i];
         }
-        material_e<NC>(mt, s, d, e, m.dt, m.dtf);
+        material_e<NC>(mt, s, it & 1, d, e, m.dt, m.dtf);
 #pragma unroll
         for (int c = 0; c < NC; c++) G(m.F[cE[c]])[i] = e[c];
         e0 = e[0];

@@ -36,7 +36,10 @@ __device__ inline double ionrate(MR &mt, double I) {
 }
 
 // d[c]: updated D; e[c]: on entry E^n (read only when the material has ADE/plasma
-// state), on exit E^{n+1}.  s is the cell's state slot within its material.  mt is a
+// state), on exit E^{n+1}.  s is the cell's state slot within its material; par the
+// model's completed steps mod 2: half par of the P and Pp arrays holds step n, the other
+// half step n - 1, which this update overwrites with step n + 1 (a Debye pole, B = 0,
+// reads no step n - 1).  mt is a
 // DevMat (T = float) or a DevMat64 (T = double), generic or global-address-space; dtT
 // is dt in T.  Whatever T, the pole and plasma-current recursions run in FP64 on FP64
 // state and coefficients, and D - sum P is formed in FP64: a Drude pole (and the plasma
@@ -45,7 +48,7 @@ __device__ inline double ionrate(MR &mt, double I) {
 // into a drift of P and D over thousands of steps.  The plasma density and rate are
 // FP64 as well.
 template <int NC, class T, class MR>
-__device__ __forceinline__ void material_e(MR &mt, int s, const T (&d)[NC], T (&e)[NC],
+__device__ __forceinline__ void material_e(MR &mt, int s, int par, const T (&d)[NC], T (&e)[NC],
                                            double dt, T dtT) {
     const int flags = mt.flags;
     double sumP[NC];
@@ -53,18 +56,18 @@ __device__ __forceinline__ void material_e(MR &mt, int s, const T (&d)[NC], T (&
     for (int c = 0; c < NC; c++) sumP[c] = 0.0;
     if (flags & MAT_DISP) {
         const int nq = mt.nq, cnt = mt.cnt;
+        const size_t half = (size_t)nq * NC * cnt;
+        const gp<double> cur = G(mt.P) + (par ? half : 0), old = G(mt.P) + (par ? 0 : half);
         for (int q = 0; q < nq; q++) {
             const double A = G(mt.Aq)[q], B = G(mt.Bq)[q], Cc = G(mt.Cq)[q];
 #pragma unroll
             for (int c = 0; c < NC; c++) {
                 // pole state is read and written once per step: streamed past the caches
-                gp<v2d> pp = (gp<v2d>)(mt.P + (size_t)(q * NC + c) * cnt + s);
-                const v2d p = __builtin_nontemporal_load(pp);
-                const double pn = A * p.x + B * p.y + Cc * (double)e[c];
-                v2d np;
-                np.x = pn;
-                np.y = p.x;
-                __builtin_nontemporal_store(np, pp);
+                const size_t j = (size_t)(q * NC + c) * cnt + s;
+                const double p = __builtin_nontemporal_load(cur + j);
+                const double pm = B != 0.0 ? __builtin_nontemporal_load(old + j) : 0.0;
+                const double pn = A * p + B * pm + Cc * (double)e[c];
+                __builtin_nontemporal_store(pn, old + j);
                 sumP[c] += pn;
             }
         }
@@ -79,17 +82,16 @@ __device__ __forceinline__ void material_e(MR &mt, int s, const T (&d)[NC], T (&
         v2d rd = __builtin_nontemporal_load(prd);
         const double cp = mt.Cprho0 * rd.x;
         double J[NC];
+        const size_t half = (size_t)NC * cnt;
+        const gp<double> cur = G(mt.Pp) + (par ? half : 0), old = G(mt.Pp) + (par ? 0 : half);
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            gp<v2d> pp = (gp<v2d>)(mt.Pp + (size_t)c * cnt + s);
-            const v2d p = __builtin_nontemporal_load(pp);
-            const double pn = (double)mt.Ap * p.x + (double)mt.Bp * p.y + cp * (double)e[c];
-            v2d np;
-            np.x = pn;
-            np.y = p.x;
-            __builtin_nontemporal_store(np, pp);
+            const size_t j = (size_t)c * cnt + s;
+            const double p = __builtin_nontemporal_load(cur + j), pm = __builtin_nontemporal_load(old + j);
+            const double pn = (double)mt.Ap * p + (double)mt.Bp * pm + cp * (double)e[c];
+            __builtin_nontemporal_store(pn, old + j);
             sumP[c] += pn;
-            J[c] = (pn - p.y) / (2.0 * dt);
+            J[c] = (pn - pm) / (2.0 * dt);
         }
         const double invE2 = (E2 >= DBL_EPS_) ? 1.0 / E2 : 0.0;
         const double ca = mt.Marho0 * rd.y * invE2;
