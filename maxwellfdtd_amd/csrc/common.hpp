@@ -47,6 +47,12 @@ __device__ __forceinline__ double sqrt_of(double x) { return sqrt(x); }
 // Two-element vectors usable through address-space-1 pointers (HIP's float2/double2
 // classes only bind to generic references).
 typedef float v2f __attribute__((ext_vector_type(2)));
+__host__ __device__ inline v2f mk2(float a, float b) {
+    v2f v;
+    v.x = a;
+    v.y = b;
+    return v;
+}
 typedef double v2d __attribute__((ext_vector_type(2)));
 
 enum { HX = 0, HY, HZ, DX, DY, DZ, EX, EY, EZ };
@@ -65,9 +71,13 @@ struct DevLayer {
     int pad;
     const float *K, *A, *B;
     const float *Ki;        // 1/K (rounded once from FP64), for the pipelined 3D kernels
+    // 1D / 2D models (and FP64 models, whose arrays hold doubles): one array per psi
     float *psiE0, *psiE1;   // H half-step psi (x: Eyx,Ezx  y: Exy,Ezy  z: Exz,Eyz)
     float *psiH0, *psiH1;   // E half-step psi (x: Hyx,Hzx  y: Hxy,Hzy  z: Hxz,Hyz)
-    float *psiE0b, *psiE1b; // fused 3D step: the other buffer of the H half-step psi
+    // 3D FP32 models: the two psi of a cell interleaved, (psi0, psi1) per entry, so one
+    // 8-byte access and one base pointer serve both
+    v2f *pE, *pH;           // H and E half-step psi pairs
+    v2f *pEb;               // fused 3D step: the other buffer of the H half-step pairs
 };
 
 // Material coefficient table entry (index 0 = vacuum) with the material's compact

@@ -405,13 +405,26 @@ int Model::upload() {
         UP(upload_vec(arena, &p, to_f32(L[f].B), io)); l.B = p;
     }
     // FP32 psi arrays: the two faces of an axis share one allocation, the upper face's
-    // array right after the lower face's, so a kernel can address either from one base
+    // array right after the lower face's, so a kernel can address either from one base;
+    // 3D models keep the two psi of a cell interleaved (pE, pH)
     if (!f64)
         for (int a = 0; a < 3; a++) {
             DevLayer &l0 = m.L[2 * a], &l1 = m.L[2 * a + 1];
             const size_t n0 = l0.present ? psi_len(*this, 2 * a, l0.nb) : 0;
             const size_t n1 = l1.present ? psi_len(*this, 2 * a + 1, l1.nb) : 0;
             if (n0 + n1 == 0) continue;
+            if (d.ndims == 3) {
+                v2f **d0[2] = {&l0.pE, &l0.pH};
+                v2f **d1[2] = {&l1.pE, &l1.pH};
+                for (int k = 0; k < 2; k++) {
+                    v2f *q;
+                    UP(arena.alloc(&q, n0 + n1));
+                    UP(hipMemsetAsync(q, 0, (n0 + n1) * sizeof(v2f), io));
+                    if (l0.present) *d0[k] = q;
+                    if (l1.present) *d1[k] = q + n0;
+                }
+                continue;
+            }
             float **d0[4] = {&l0.psiE0, &l0.psiE1, &l0.psiH0, &l0.psiH1};
             float **d1[4] = {&l1.psiE0, &l1.psiE1, &l1.psiH0, &l1.psiH1};
             for (int k = 0; k < 4; k++) {
@@ -620,15 +633,11 @@ int Model::ensure_pp() {
         const size_t n0 = l0.present ? psi_len(*this, 2 * a, l0.nb) : 0;
         const size_t n1 = l1.present ? psi_len(*this, 2 * a + 1, l1.nb) : 0;
         if (n0 + n1 == 0) continue;
-        float **d0[2] = {&l0.psiE0b, &l0.psiE1b};
-        float **d1[2] = {&l1.psiE0b, &l1.psiE1b};
-        for (int k = 0; k < 2; k++) {
-            float *q;
-            MFD_HIP(arena.alloc(&q, n0 + n1));
-            MFD_HIP(hipMemsetAsync(q, 0, (n0 + n1) * sizeof(float), io));
-            if (l0.present) *d0[k] = q;
-            if (l1.present) *d1[k] = q + n0;
-        }
+        v2f *q;
+        MFD_HIP(arena.alloc(&q, n0 + n1));
+        MFD_HIP(hipMemsetAsync(q, 0, (n0 + n1) * sizeof(v2f), io));
+        if (l0.present) l0.pEb = q;
+        if (l1.present) l1.pEb = q + n0;
     }
     MFD_HIP(hipStreamSynchronize(io));
     pp_alloc = true;
@@ -665,10 +674,7 @@ int Model::ensure_touched_yz() {
 void Model::swap_pp() {
     const int cs[6] = {HX, HY, HZ, EX, EY, EZ};
     for (int c : cs) std::swap(dm.F[c], dm.Falt[c]);
-    for (int f = 0; f < 6; f++) {
-        std::swap(dm.L[f].psiE0, dm.L[f].psiE0b);
-        std::swap(dm.L[f].psiE1, dm.L[f].psiE1b);
-    }
+    for (int f = 0; f < 6; f++) std::swap(dm.L[f].pE, dm.L[f].pEb);
     pp_parity ^= 1;
 }
 
@@ -1156,8 +1162,8 @@ int Runner::refresh() {
         DevModel &h = hd[k];
         for (int c = 0; c < 9; c++) { h.F[c] = m->dm.F[c]; h.Falt[c] = m->dm.Falt[c]; }
         for (int f = 0; f < 6; f++) {
-            h.L[f].psiE0 = m->dm.L[f].psiE0; h.L[f].psiE1 = m->dm.L[f].psiE1;
-            h.L[f].psiE0b = m->dm.L[f].psiE0b; h.L[f].psiE1b = m->dm.L[f].psiE1b;
+            h.L[f].pE = m->dm.L[f].pE;
+            h.L[f].pEb = m->dm.L[f].pEb;
         }
         MFD_HIP(copy_sync(d_models + k, &h, sizeof(DevModel), hipMemcpyHostToDevice, stream));
         pp_seen[k] = m->pp_parity;

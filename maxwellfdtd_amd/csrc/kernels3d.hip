@@ -78,9 +78,10 @@ __global__ __launch_bounds__(256) void k_h3d(const DevModel *__restrict__ models
                 if (l.present && ix <= l.ib) {
                     const int p = ix;
                     const long long j = p + (long long)l.nb * (iy + (long long)Ny * iz);
-                    const float a0 = l.B[p] * l.psiE0[j] + l.A[p] * dEyx;
-                    const float a1 = l.B[p] * l.psiE1[j] + l.A[p] * dEzx;
-                    l.psiE0[j] = a0; l.psiE1[j] = a1;
+                    const v2f q = l.pE[j];
+                    const float a0 = l.B[p] * q.x + l.A[p] * dEyx;
+                    const float a1 = l.B[p] * q.y + l.A[p] * dEzx;
+                    l.pE[j] = mk2(a0, a1);
                     hy += Mh * a1; hz -= Mh * a0;
                     dEyx = dEyx / l.K[p]; dEzx = dEzx / l.K[p];
                 }
@@ -90,9 +91,10 @@ __global__ __launch_bounds__(256) void k_h3d(const DevModel *__restrict__ models
                 if (l.present && ix >= l.ib) {
                     const int p = ix - l.ib;
                     const long long j = p + (long long)l.nb * (iy + (long long)Ny * iz);
-                    const float a0 = l.B[p] * l.psiE0[j] + l.A[p] * dEyx;
-                    const float a1 = l.B[p] * l.psiE1[j] + l.A[p] * dEzx;
-                    l.psiE0[j] = a0; l.psiE1[j] = a1;
+                    const v2f q = l.pE[j];
+                    const float a0 = l.B[p] * q.x + l.A[p] * dEyx;
+                    const float a1 = l.B[p] * q.y + l.A[p] * dEzx;
+                    l.pE[j] = mk2(a0, a1);
                     hy += Mh * a1; hz -= Mh * a0;
                     dEyx = dEyx / l.K[p]; dEzx = dEzx / l.K[p];
                 }
@@ -102,9 +104,10 @@ __global__ __launch_bounds__(256) void k_h3d(const DevModel *__restrict__ models
                 if (l.present && iy <= l.ib) {
                     const int p = iy;
                     const long long j = ix + (long long)Nx * (p + (long long)l.nb * iz);
-                    const float a0 = l.B[p] * l.psiE0[j] + l.A[p] * dExy;
-                    const float a1 = l.B[p] * l.psiE1[j] + l.A[p] * dEzy;
-                    l.psiE0[j] = a0; l.psiE1[j] = a1;
+                    const v2f q = l.pE[j];
+                    const float a0 = l.B[p] * q.x + l.A[p] * dExy;
+                    const float a1 = l.B[p] * q.y + l.A[p] * dEzy;
+                    l.pE[j] = mk2(a0, a1);
                     hx -= Mh * a1; hz += Mh * a0;
                     dExy = dExy / l.K[p]; dEzy = dEzy / l.K[p];
                 }
@@ -114,9 +117,10 @@ __global__ __launch_bounds__(256) void k_h3d(const DevModel *__restrict__ models
                 if (l.present && iy >= l.ib) {
                     const int p = iy - l.ib;
                     const long long j = ix + (long long)Nx * (p + (long long)l.nb * iz);
-                    const float a0 = l.B[p] * l.psiE0[j] + l.A[p] * dExy;
-                    const float a1 = l.B[p] * l.psiE1[j] + l.A[p] * dEzy;
-                    l.psiE0[j] = a0; l.psiE1[j] = a1;
+                    const v2f q = l.pE[j];
+                    const float a0 = l.B[p] * q.x + l.A[p] * dExy;
+                    const float a1 = l.B[p] * q.y + l.A[p] * dEzy;
+                    l.pE[j] = mk2(a0, a1);
                     hx -= Mh * a1; hz += Mh * a0;
                     dExy = dExy / l.K[p]; dEzy = dEzy / l.K[p];
                 }
@@ -126,9 +130,10 @@ __global__ __launch_bounds__(256) void k_h3d(const DevModel *__restrict__ models
                 if (l.present && iz <= l.ib) {
                     const int p = iz;
                     const long long j = col + sz * p;
-                    const float a0 = l.B[p] * l.psiE0[j] + l.A[p] * dExz;
-                    const float a1 = l.B[p] * l.psiE1[j] + l.A[p] * dEyz;
-                    l.psiE0[j] = a0; l.psiE1[j] = a1;
+                    const v2f q = l.pE[j];
+                    const float a0 = l.B[p] * q.x + l.A[p] * dExz;
+                    const float a1 = l.B[p] * q.y + l.A[p] * dEyz;
+                    l.pE[j] = mk2(a0, a1);
                     hx += Mh * a1; hy -= Mh * a0;
                     dExz = dExz / l.K[p]; dEyz = dEyz / l.K[p];
                 }
@@ -138,9 +143,10 @@ __global__ __launch_bounds__(256) void k_h3d(const DevModel *__restrict__ models
                 if (l.present && iz >= l.ib) {
                     const int p = iz - l.ib;
                     const long long j = col + sz * p;
-                    const float a0 = l.B[p] * l.psiE0[j] + l.A[p] * dExz;
-                    const float a1 = l.B[p] * l.psiE1[j] + l.A[p] * dEyz;
-                    l.psiE0[j] = a0; l.psiE1[j] = a1;
+                    const v2f q = l.pE[j];
+                    const float a0 = l.B[p] * q.x + l.A[p] * dExz;
+                    const float a1 = l.B[p] * q.y + l.A[p] * dEyz;
+                    l.pE[j] = mk2(a0, a1);
                     hx += Mh * a1; hy -= Mh * a0;
                     dExz = dExz / l.K[p]; dEyz = dEyz / l.K[p];
                 }
@@ -225,9 +231,10 @@ __global__ __launch_bounds__(256) void k_e3d(const DevModel *__restrict__ models
                 if (l.present && ix <= l.ib) {
                     const int p = ix;
                     const long long j = p + (long long)l.nb * (iy + (long long)Ny * iz);
-                    const float a0 = l.B[p] * l.psiH0[j] + l.A[p] * dHyx;
-                    const float a1 = l.B[p] * l.psiH1[j] + l.A[p] * dHzx;
-                    l.psiH0[j] = a0; l.psiH1[j] = a1;
+                    const v2f q = l.pH[j];
+                    const float a0 = l.B[p] * q.x + l.A[p] * dHyx;
+                    const float a1 = l.B[p] * q.y + l.A[p] * dHzx;
+                    l.pH[j] = mk2(a0, a1);
                     d[1] -= Md2 * a1; d[2] += Md2 * a0;
                     dHyx = dHyx / l.K[p]; dHzx = dHzx / l.K[p];
                 }
@@ -237,9 +244,10 @@ __global__ __launch_bounds__(256) void k_e3d(const DevModel *__restrict__ models
                 if (l.present && ix >= l.ib) {
                     const int p = ix - l.ib;
                     const long long j = p + (long long)l.nb * (iy + (long long)Ny * iz);
-                    const float a0 = l.B[p] * l.psiH0[j] + l.A[p] * dHyx;
-                    const float a1 = l.B[p] * l.psiH1[j] + l.A[p] * dHzx;
-                    l.psiH0[j] = a0; l.psiH1[j] = a1;
+                    const v2f q = l.pH[j];
+                    const float a0 = l.B[p] * q.x + l.A[p] * dHyx;
+                    const float a1 = l.B[p] * q.y + l.A[p] * dHzx;
+                    l.pH[j] = mk2(a0, a1);
                     d[1] -= Md2 * a1; d[2] += Md2 * a0;
                     dHyx = dHyx / l.K[p]; dHzx = dHzx / l.K[p];
                 }
@@ -249,9 +257,10 @@ __global__ __launch_bounds__(256) void k_e3d(const DevModel *__restrict__ models
                 if (l.present && iy <= l.ib) {
                     const int p = iy;
                     const long long j = ix + (long long)Nx * (p + (long long)l.nb * iz);
-                    const float a0 = l.B[p] * l.psiH0[j] + l.A[p] * dHxy;
-                    const float a1 = l.B[p] * l.psiH1[j] + l.A[p] * dHzy;
-                    l.psiH0[j] = a0; l.psiH1[j] = a1;
+                    const v2f q = l.pH[j];
+                    const float a0 = l.B[p] * q.x + l.A[p] * dHxy;
+                    const float a1 = l.B[p] * q.y + l.A[p] * dHzy;
+                    l.pH[j] = mk2(a0, a1);
                     d[0] += Md2 * a1; d[2] -= Md2 * a0;
                     dHxy = dHxy / l.K[p]; dHzy = dHzy / l.K[p];
                 }
@@ -261,9 +270,10 @@ __global__ __launch_bounds__(256) void k_e3d(const DevModel *__restrict__ models
                 if (l.present && iy >= l.ib) {
                     const int p = iy - l.ib;
                     const long long j = ix + (long long)Nx * (p + (long long)l.nb * iz);
-                    const float a0 = l.B[p] * l.psiH0[j] + l.A[p] * dHxy;
-                    const float a1 = l.B[p] * l.psiH1[j] + l.A[p] * dHzy;
-                    l.psiH0[j] = a0; l.psiH1[j] = a1;
+                    const v2f q = l.pH[j];
+                    const float a0 = l.B[p] * q.x + l.A[p] * dHxy;
+                    const float a1 = l.B[p] * q.y + l.A[p] * dHzy;
+                    l.pH[j] = mk2(a0, a1);
                     d[0] += Md2 * a1; d[2] -= Md2 * a0;
                     dHxy = dHxy / l.K[p]; dHzy = dHzy / l.K[p];
                 }
@@ -273,9 +283,10 @@ __global__ __launch_bounds__(256) void k_e3d(const DevModel *__restrict__ models
                 if (l.present && iz <= l.ib) {
                     const int p = iz;
                     const long long j = col + sz * p;
-                    const float a0 = l.B[p] * l.psiH0[j] + l.A[p] * dHxz;
-                    const float a1 = l.B[p] * l.psiH1[j] + l.A[p] * dHyz;
-                    l.psiH0[j] = a0; l.psiH1[j] = a1;
+                    const v2f q = l.pH[j];
+                    const float a0 = l.B[p] * q.x + l.A[p] * dHxz;
+                    const float a1 = l.B[p] * q.y + l.A[p] * dHyz;
+                    l.pH[j] = mk2(a0, a1);
                     d[0] -= Md2 * a1; d[1] += Md2 * a0;
                     dHxz = dHxz / l.K[p]; dHyz = dHyz / l.K[p];
                 }
@@ -285,9 +296,10 @@ __global__ __launch_bounds__(256) void k_e3d(const DevModel *__restrict__ models
                 if (l.present && iz >= l.ib) {
                     const int p = iz - l.ib;
                     const long long j = col + sz * p;
-                    const float a0 = l.B[p] * l.psiH0[j] + l.A[p] * dHxz;
-                    const float a1 = l.B[p] * l.psiH1[j] + l.A[p] * dHyz;
-                    l.psiH0[j] = a0; l.psiH1[j] = a1;
+                    const v2f q = l.pH[j];
+                    const float a0 = l.B[p] * q.x + l.A[p] * dHxz;
+                    const float a1 = l.B[p] * q.y + l.A[p] * dHyz;
+                    l.pH[j] = mk2(a0, a1);
                     d[0] -= Md2 * a1; d[1] += Md2 * a0;
                     dHxz = dHxz / l.K[p]; dHyz = dHyz / l.K[p];
                 }

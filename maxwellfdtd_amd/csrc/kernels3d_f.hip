@@ -345,30 +345,30 @@ __global__ __launch_bounds__(64) void k_step3d_f(const DevModel *__restrict__ mo
         // the lower face has no layer): the upper face's array follows the lower face's in
         // the same allocation, so a lane of either face needs only a byte offset
         const DevLayer &LX = m.L[0].present ? m.L[0] : m.L[1];
-        const float *XE0 = LX.psiE0, *XE1 = LX.psiE1, *XH0 = LX.psiH0, *XH1 = LX.psiH1;
-        float *XE0w = LX.psiE0b, *XE1w = LX.psiE1b, *XH0w = LX.psiH0, *XH1w = LX.psiH1;
+        const v2f *XE = LX.pE, *XH = LX.pH;
+        v2f *XEw = LX.pEb, *XHw = LX.pH;
         const unsigned nbx0 = (unsigned)m.L[0].nb, nbx1 = (unsigned)m.L[1].nb;
-        const unsigned x1B = m.L[0].present ? 4u * nbx0 * (unsigned)Ny * (unsigned)Nz : 0u;
+        const unsigned x1B = m.L[0].present ? 8u * nbx0 * (unsigned)Ny * (unsigned)Nz : 0u;
         auto xo = [&](int f, int p, int iy, int iz) {
             const unsigned nb = f == 0 ? nbx0 : nbx1;
-            return (f == 1 ? x1B : 0u) + 4u * ((unsigned)p + nb * ((unsigned)iy + (unsigned)Ny * (unsigned)iz));
+            return (f == 1 ? x1B : 0u) + 8u * ((unsigned)p + nb * ((unsigned)iy + (unsigned)Ny * (unsigned)iz));
         };
         // the same for the y and z slabs
         const DevLayer &LY = m.L[2].present ? m.L[2] : m.L[3];
-        const float *YE0 = LY.psiE0, *YE1 = LY.psiE1, *YH0 = LY.psiH0, *YH1 = LY.psiH1;
-        float *YE0w = LY.psiE0b, *YE1w = LY.psiE1b, *YH0w = LY.psiH0, *YH1w = LY.psiH1;
+        const v2f *YE = LY.pE, *YH = LY.pH;
+        v2f *YEw = LY.pEb, *YHw = LY.pH;
         const unsigned nby0 = (unsigned)m.L[2].nb, nby1 = (unsigned)m.L[3].nb;
-        const unsigned y1B = m.L[2].present ? 4u * (unsigned)Nx * nby0 * (unsigned)Nz : 0u;
+        const unsigned y1B = m.L[2].present ? 8u * (unsigned)Nx * nby0 * (unsigned)Nz : 0u;
         auto yo = [&](int f, int p, int ix_, int iz) {
             const unsigned nb = f == 2 ? nby0 : nby1;
-            return (f == 3 ? y1B : 0u) + 4u * ((unsigned)ix_ + (unsigned)Nx * ((unsigned)p + nb * (unsigned)iz));
+            return (f == 3 ? y1B : 0u) + 8u * ((unsigned)ix_ + (unsigned)Nx * ((unsigned)p + nb * (unsigned)iz));
         };
         const DevLayer &LZ = m.L[4].present ? m.L[4] : m.L[5];
-        const float *ZE0 = LZ.psiE0, *ZE1 = LZ.psiE1, *ZH0 = LZ.psiH0, *ZH1 = LZ.psiH1;
-        float *ZE0w = LZ.psiE0b, *ZE1w = LZ.psiE1b, *ZH0w = LZ.psiH0, *ZH1w = LZ.psiH1;
+        const v2f *ZE = LZ.pE, *ZH = LZ.pH;
+        v2f *ZEw = LZ.pEb, *ZHw = LZ.pH;
         const unsigned z1B = m.L[4].present ? 4u * (unsigned)m.L[4].nb * (unsigned)m.sz : 0u;
         auto zof = [&](int f, int p, unsigned cellB) {   // cellB: byte offset of the cell in its plane
-            return (f == 5 ? z1B : 0u) + cellB + szB * (unsigned)p;
+            return 2u * ((f == 5 ? z1B : 0u) + cellB + szB * (unsigned)p);
         };
         // carried from one plane to the next: Ex, Ey and the id of the rows (loaded with
         // the plane below as its +z neighbour), Ey of the row below, Ex of the left column
@@ -446,21 +446,24 @@ __global__ __launch_bounds__(64) void k_step3d_f(const DevModel *__restrict__ mo
                 dc[r] = d;
                 if (bx) {
                     const unsigned j = xo(fx, px, iyc, zg);
-                    hq[r][0] = ldb<float>(XE0, j);
-                    hq[r][1] = ldb<float>(XE1, j);
-                    if (!pro) { eq[r][0] = ldb<float>(XH0, j); eq[r][1] = ldb<float>(XH1, j); }
+                    const v2f q = ldb<v2f>(XE, j);
+                    hq[r][0] = q.x;
+                    hq[r][1] = q.y;
+                    if (!pro) { const v2f w = ldb<v2f>(XH, j); eq[r][0] = w.x; eq[r][1] = w.y; }
                 }
                 if ((PM & 2) && fy[r] >= 0) {
                     const unsigned j = yo(fy[r], py[r], ixc, zg);
-                    hq[r][2] = ldb<float>(YE0, j);
-                    hq[r][3] = ldb<float>(YE1, j);
-                    if (!pro) { eq[r][2] = ldb<float>(YH0, j); eq[r][3] = ldb<float>(YH1, j); }
+                    const v2f q = ldb<v2f>(YE, j);
+                    hq[r][2] = q.x;
+                    hq[r][3] = q.y;
+                    if (!pro) { const v2f w = ldb<v2f>(YH, j); eq[r][2] = w.x; eq[r][3] = w.y; }
                 }
                 if (bz) {
                     const unsigned j = zof(fz, pz, rowB[r]);
-                    hq[r][4] = ldb<float>(ZE0, j);
-                    hq[r][5] = ldb<float>(ZE1, j);
-                    if (!pro) { eq[r][4] = ldb<float>(ZH0, j); eq[r][5] = ldb<float>(ZH1, j); }
+                    const v2f q = ldb<v2f>(ZE, j);
+                    hq[r][4] = q.x;
+                    hq[r][5] = q.y;
+                    if (!pro) { const v2f w = ldb<v2f>(ZH, j); eq[r][4] = w.x; eq[r][5] = w.y; }
                 }
             }
             const float beyn = ldb<float>(Ey, bB + zn), hexn = ldb<float>(Ex, xhB + zn);
@@ -473,13 +476,14 @@ __global__ __launch_bounds__(64) void k_step3d_f(const DevModel *__restrict__ mo
             if (!pro && ybm == 0) {
                 bexv = ldb<float>(Ex, bB + zo); bez = ldb<float>(Ez, bB + zo);
                 bhx = ldb<float>(Hx, bB + zo); bhz = ldb<float>(Hz, bB + zo);
-                if (bx) bq[0] = ldb<float>(XE0, xo(fx, px, yb, zg));
+                if (bx) bq[0] = ldb<float>(XE, xo(fx, px, yb, zg));
                 if (bby) {
                     const unsigned j = yo(fyb, pyb, ixc, zg);
-                    bq[2] = ldb<float>(YE0, j);
-                    bq[3] = ldb<float>(YE1, j);
+                    const v2f q = ldb<v2f>(YE, j);
+                    bq[2] = q.x;
+                    bq[3] = q.y;
                 }
-                if (bz) bq[5] = ldb<float>(ZE1, zof(fz, pz, bB));
+                if (bz) bq[5] = ldb<float>(ZE, zof(fz, pz, bB) + 4u);
             }
             // the left column (lane t: row t): Ey, Ez, old Hy, Hz and the psi values its Hy,
             // Hz need
@@ -492,11 +496,12 @@ __global__ __launch_bounds__(64) void k_step3d_f(const DevModel *__restrict__ mo
                 if (PM && lane < nr) {
                         if (xbx) {
                         const unsigned j = xo(fxh, pxh, hrow, zg);
-                        xq[0] = ldb<float>(XE0, j);
-                        xq[1] = ldb<float>(XE1, j);
+                        const v2f q = ldb<v2f>(XE, j);
+                        xq[0] = q.x;
+                        xq[1] = q.y;
                     }
-                    if (xby) xq[2] = ldb<float>(YE0, yo(fyh, pyh, xh, zg));
-                    if (bz) xq[4] = ldb<float>(ZE0, zof(fz, pz, 4u * (unsigned)xh + syB * (unsigned)hrow));
+                    if (xby) xq[2] = ldb<float>(YE, yo(fyh, pyh, xh, zg));
+                    if (bz) xq[4] = ldb<float>(ZE, zof(fz, pz, 4u * (unsigned)xh + syB * (unsigned)hrow));
                 }
             }
 
@@ -522,18 +527,15 @@ __global__ __launch_bounds__(64) void k_step3d_f(const DevModel *__restrict__ mo
                     const int iyc = min(y0 + r, Ny - 1);
                     if (bx) {
                         const unsigned j = xo(fx, px, iyc, zg);
-                        stb<float>(XE0w, j, hq[r][0]);
-                        stb<float>(XE1w, j, hq[r][1]);
+                        stb<v2f>(XEw, j, mk2(hq[r][0], hq[r][1]));
                     }
                     if (by) {
                         const unsigned j = yo(fy[r], py[r], ixc, zg);
-                        stb<float>(YE0w, j, hq[r][2]);
-                        stb<float>(YE1w, j, hq[r][3]);
+                        stb<v2f>(YEw, j, mk2(hq[r][2], hq[r][3]));
                     }
                     if (bz) {
                         const unsigned j = zof(fz, pz, rowB[r]);
-                        stb<float>(ZE0w, j, hq[r][4]);
-                        stb<float>(ZE1w, j, hq[r][5]);
+                        stb<v2f>(ZEw, j, mk2(hq[r][4], hq[r][5]));
                     }
                 }
             }
@@ -601,18 +603,15 @@ __global__ __launch_bounds__(64) void k_step3d_f(const DevModel *__restrict__ mo
                         const int iyc = min(iy, Ny - 1);
                         if (bx) {
                             const unsigned j = xo(fx, px, iyc, zg);
-                            stb<float>(XH0w, j, eq[r][0]);
-                            stb<float>(XH1w, j, eq[r][1]);
+                            stb<v2f>(XHw, j, mk2(eq[r][0], eq[r][1]));
                         }
                         if (by) {
                             const unsigned j = yo(fy[r], py[r], ixc, zg);
-                            stb<float>(YH0w, j, eq[r][2]);
-                            stb<float>(YH1w, j, eq[r][3]);
+                            stb<v2f>(YHw, j, mk2(eq[r][2], eq[r][3]));
                         }
                         if (bz) {
                             const unsigned j = zof(fz, pz, rowB[r]);
-                            stb<float>(ZH0w, j, eq[r][4]);
-                            stb<float>(ZH1w, j, eq[r][5]);
+                            stb<v2f>(ZHw, j, mk2(eq[r][4], eq[r][5]));
                         }
                     }
                     float e[3];
@@ -712,9 +711,9 @@ __global__ void k_swap_pp(DevModel *models, int nmodels) {
     }
     for (int f = 0; f < 6; f++) {
         DevLayer &L = m.L[f];
-        float *a0 = L.psiE0, *a1 = L.psiE1;
-        L.psiE0 = L.psiE0b; L.psiE1 = L.psiE1b;
-        L.psiE0b = a0; L.psiE1b = a1;
+        v2f *a = L.pE;
+        L.pE = L.pEb;
+        L.pEb = a;
     }
 }
 
@@ -734,9 +733,9 @@ __global__ void k_finish_swap(DevModel *models, int nmodels) {
     }
     for (int f = 0; f < 6; f++) {
         DevLayer &L = m.L[f];
-        float *a0 = L.psiE0, *a1 = L.psiE1;
-        L.psiE0 = L.psiE0b; L.psiE1 = L.psiE1b;
-        L.psiE0b = a0; L.psiE1b = a1;
+        v2f *a = L.pE;
+        L.pE = L.pEb;
+        L.pEb = a;
     }
     *m.it = it + 1;
     if (it + 1 - *m.dftdone == m.dft_block) *m.dftdone = it + 1;

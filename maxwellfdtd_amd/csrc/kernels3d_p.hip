@@ -27,14 +27,13 @@ typedef const __attribute__((address_space(1))) unsigned char gcu1;
 struct LayP {
     int present, ib, nb;
     const float *Ki, *A, *B;     // 1/K, A, B
-    float *p0, *p1;
+    v2f *p;                      // psi pairs
 };
 __device__ __forceinline__ LayP layp(const DevLayer &l, bool h) {
     LayP r;
     r.present = l.present; r.ib = l.ib; r.nb = l.nb;
     r.Ki = l.Ki; r.A = l.A; r.B = l.B;
-    r.p0 = h ? l.psiE0 : l.psiH0;
-    r.p1 = h ? l.psiE1 : l.psiH1;
+    r.p = h ? l.pE : l.pH;
     return r;
 }
 
@@ -42,10 +41,10 @@ __device__ __forceinline__ LayP layp(const DevLayer &l, bool h) {
 __device__ __forceinline__ void slab2(const LayP &L, int p, unsigned j, float &d0, float &d1,
                                       float &a0, float &a1) {
     const float B = G(L.B)[p], A = G(L.A)[p], Ki = G(L.Ki)[p];
-    a0 = B * G(L.p0)[j] + A * d0;
-    a1 = B * G(L.p1)[j] + A * d1;
-    G(L.p0)[j] = a0;
-    G(L.p1)[j] = a1;
+    const v2f q = G(L.p)[j];
+    a0 = B * q.x + A * d0;
+    a1 = B * q.y + A * d1;
+    G(L.p)[j] = mk2(a0, a1);
     d0 = d0 * Ki;      // d / K, with 1/K rounded once on the host
     d1 = d1 * Ki;
 }
@@ -55,8 +54,7 @@ __device__ __forceinline__ void slab2v(const LayP &L, unsigned j, float psi0, fl
                                        float A, float Ki, float &d0, float &d1, float &a0, float &a1) {
     a0 = B * psi0 + A * d0;
     a1 = B * psi1 + A * d1;
-    G(L.p0)[j] = a0;
-    G(L.p1)[j] = a1;
+    G(L.p)[j] = mk2(a0, a1);
     d0 = d0 * Ki;
     d1 = d1 * Ki;
 }
@@ -128,11 +126,11 @@ __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(5, 8)))
             auto slab_pre = [&](int iz, SP &S) {
                 if (xs0) {
                     const unsigned j = ix + (unsigned)L[0].nb * (iy + (unsigned)Ny * iz);
-                    S.x0 = G((const float *)L[0].p0)[j]; S.x1 = G((const float *)L[0].p1)[j];
+                    { const v2f q = G((const v2f *)L[0].p)[j]; S.x0 = q.x; S.x1 = q.y; }
                 } else if (xs1) {
                     const int p = ix - L[1].ib;
                     const unsigned j = p + (unsigned)L[1].nb * (iy + (unsigned)Ny * iz);
-                    S.x0 = G((const float *)L[1].p0)[j]; S.x1 = G((const float *)L[1].p1)[j];
+                    { const v2f q = G((const v2f *)L[1].p)[j]; S.x0 = q.x; S.x1 = q.y; }
                 }
             };
             auto compute = [&](int iz, HP &P, const SP &S) {
@@ -360,29 +358,29 @@ __global__ __launch_bounds__(64 * TY) void k_e3d_p(const DevModel *__restrict__ 
                 // z slabs: the plane's psi pair and its profile values (uniform per plane)
                 if (zs0(iz)) {
                     const unsigned j = col + sz * iz;
-                    S.z0 = G((const float *)L[4].p0)[j]; S.z1 = G((const float *)L[4].p1)[j];
+                    { const v2f q = G((const v2f *)L[4].p)[j]; S.z0 = q.x; S.z1 = q.y; }
                     S.zKi = G(L[4].Ki)[iz]; S.zA = G(L[4].A)[iz]; S.zB = G(L[4].B)[iz];
                 } else if (zs1(iz)) {
                     const int p = iz - L[5].ib;
                     const unsigned j = col + sz * p;
-                    S.z0 = G((const float *)L[5].p0)[j]; S.z1 = G((const float *)L[5].p1)[j];
+                    { const v2f q = G((const v2f *)L[5].p)[j]; S.z0 = q.x; S.z1 = q.y; }
                     S.zKi = G(L[5].Ki)[p]; S.zA = G(L[5].A)[p]; S.zB = G(L[5].B)[p];
                 }
                 if (xs0) {
                     const unsigned j = ix + (unsigned)L[0].nb * (iy + (unsigned)Ny * iz);
-                    S.x0 = G((const float *)L[0].p0)[j]; S.x1 = G((const float *)L[0].p1)[j];
+                    { const v2f q = G((const v2f *)L[0].p)[j]; S.x0 = q.x; S.x1 = q.y; }
                 } else if (xs1) {
                     const int p = ix - L[1].ib;
                     const unsigned j = p + (unsigned)L[1].nb * (iy + (unsigned)Ny * iz);
-                    S.x0 = G((const float *)L[1].p0)[j]; S.x1 = G((const float *)L[1].p1)[j];
+                    { const v2f q = G((const v2f *)L[1].p)[j]; S.x0 = q.x; S.x1 = q.y; }
                 }
                 if (ys0) {
                     const unsigned j = ix + (unsigned)Nx * (iy + (unsigned)L[2].nb * iz);
-                    S.y0 = G((const float *)L[2].p0)[j]; S.y1 = G((const float *)L[2].p1)[j];
+                    { const v2f q = G((const v2f *)L[2].p)[j]; S.y0 = q.x; S.y1 = q.y; }
                 } else if (ys1) {
                     const int p = iy - L[3].ib;
                     const unsigned j = ix + (unsigned)Nx * (p + (unsigned)L[3].nb * iz);
-                    S.y0 = G((const float *)L[3].p0)[j]; S.y1 = G((const float *)L[3].p1)[j];
+                    { const v2f q = G((const v2f *)L[3].p)[j]; S.y0 = q.x; S.y1 = q.y; }
                 }
             };
             auto compute = [&](int iz, EP &P, const SP &S) {
