@@ -273,7 +273,7 @@ void launch_e3d_p(const DevModel *m, const Tile *tiles, int ntiles, int grid, in
                   bool pml, hipStream_t s);
 // fused 3D step (one z march per step over ping-pong H/E buffers), rows = 2 or 4;
 // pmask: CPML axes the tiles reach (0 none, 1 x only, 7 any)
-void launch_step3d_f(DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int rows,
+void launch_step3d_f(DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int integ, int rows,
                      int pmask, hipStream_t s);
 void launch_swap_pp(DevModel *m, int nmodels, hipStream_t s);
 void launch_finish_swap(DevModel *m, int nmodels, hipStream_t s);

@@ -852,6 +852,12 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
         if (!ok) kform = 2;
     }
     fused = kform == 3;
+    // the fused step's integral variables: none, Sa and E2 in every model, or per model
+    integ_f = 0;
+    for (Model *m : ms)
+        if (m->int_flags) integ_f = 3;
+    for (Model *m : ms)
+        if (integ_f && (m->int_flags & 3) != 3) integ_f = 4;
     frows = (opt.ty == 2) ? 2 : 4;
     for (size_t k = 0; k < ms.size(); k++) {
         Model *m = ms[k];
@@ -1081,9 +1087,9 @@ int Runner::launch_step(bool timing, bool dft_flush) {
         hipStream_t sp = side ? side : stream;
         const int n1 = nfc[0], n7 = nfc[1], n0 = nfc[2];
         if (side) { MFD_HIP(hipEventRecord(fork_ev, stream)); MFD_HIP(hipStreamWaitEvent(side, fork_ev, 0)); }
-        launch_step3d_f(d_models, d_tiles, n1, grid_of(n1), mode, any_int, frows, 1, sp);
-        launch_step3d_f(d_models, d_tiles + n1 + n7, n0, grid_of(n0), mode, any_int, frows, 0, stream);
-        launch_step3d_f(d_models, d_tiles + n1, n7, grid_of(n7), mode, any_int, frows, 7, stream);
+        launch_step3d_f(d_models, d_tiles, n1, grid_of(n1), mode, integ_f, frows, 1, sp);
+        launch_step3d_f(d_models, d_tiles + n1 + n7, n0, grid_of(n0), mode, integ_f, frows, 0, stream);
+        launch_step3d_f(d_models, d_tiles + n1, n7, grid_of(n7), mode, integ_f, frows, 7, stream);
         if (side) { MFD_HIP(hipEventRecord(join_ev, side)); MFD_HIP(hipStreamWaitEvent(stream, join_ev, 0)); }
         if (!finish_swap) launch_swap_pp(d_models, nm, stream);
     }

@@ -177,6 +177,7 @@ struct Runner {
     int kform = 0;              // 3D kernel form in use
     bool fused = false;         // kform 3: one launch per step over ping-pong H/E buffers
     int frows = 2;              // fused step: rows per lane
+    int integ_f = 0;            // fused step: 0 no integrals, 3 Sa and E2 everywhere, 4 per model
     int nfc[3] = {0, 0, 0};     // fused step: tiles reaching x slabs only, other slabs, none
     std::vector<DevModel> hd;   // host copies of d_models
     std::vector<int> pp_seen;   // per model: the buffer parity d_models holds

@@ -193,7 +193,9 @@ __device__ __forceinline__ int bcmode(int bc) { return bc == BC_PERIODIC ? 0 : b
 
 }  // namespace
 
-template <int MODE, bool INTEG, int R, int PM>
+// INTEG: 0 no integral variables; 3 Sa and E2 in every model of the launch; 4 each
+// model's own (Sa and / or E2, read from its descriptor)
+template <int MODE, int INTEG, int R, int PM>
 __global__ __launch_bounds__(64) void k_step3d_f(const DevModel *__restrict__ models, const Tile *__restrict__ tiles,
                                                  int ntiles) {
     __shared__ float4 ctab[128];
@@ -262,7 +264,7 @@ __global__ __launch_bounds__(64) void k_step3d_f(const DevModel *__restrict__ mo
         float *Hxw = m.Falt[HX], *Hyw = m.Falt[HY], *Hzw = m.Falt[HZ];
         float *Dx = m.F[DX], *Dy = m.F[DY], *Dz = m.F[DZ];
         float *Sa = m.Sa, *E2 = m.E2;
-        const bool doSa = INTEG && Sa, doE2 = INTEG && E2;
+        const bool doSa = INTEG == 3 || (INTEG == 4 && Sa), doE2 = INTEG == 3 || (INTEG == 4 && E2);
         const unsigned char *id = m.id;
         const bool dless = m.dless;
         const bool useg = MODE != MODE_VAC || INTEG || dless;
@@ -742,11 +744,13 @@ __global__ void k_finish_swap(DevModel *models, int nmodels) {
 }
 
 template <int R, int PM>
-static void lf(DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, hipStream_t s) {
+static void lf(DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int integ, hipStream_t s) {
 #define SF_(MD, IN) hipLaunchKernelGGL((k_step3d_f<MD, IN, R, PM>), dim3(grid), dim3(64), 0, s, m, tiles, ntiles)
-    if (mode == MODE_VAC) { if (integ) SF_(MODE_VAC, true); else SF_(MODE_VAC, false); }
-    else if (mode == MODE_SIMPLE) { if (integ) SF_(MODE_SIMPLE, true); else SF_(MODE_SIMPLE, false); }
-    else { if (integ) SF_(MODE_FULL, true); else SF_(MODE_FULL, false); }
+#define SI_(MD) do { if (integ == 3) SF_(MD, 3); else if (integ) SF_(MD, 4); else SF_(MD, 0); } while (0)
+    if (mode == MODE_VAC) SI_(MODE_VAC);
+    else if (mode == MODE_SIMPLE) SI_(MODE_SIMPLE);
+    else SI_(MODE_FULL);
+#undef SI_
 #undef SF_
 }
 
@@ -759,8 +763,9 @@ void launch_finish_swap(DevModel *m, int nmodels, hipStream_t s) {
 }
 
 // pmask: the CPML axes the tiles' cells (and recomputed halo cells) may lie in; the
-// kernel is built for 0 (none), 1 (x only) or 7 (any)
-void launch_step3d_f(DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int rows,
+// kernel is built for 0 (none), 1 (x only) or 7 (any).  integ: 0 none, 3 Sa and E2 in
+// every model, else each model's own
+void launch_step3d_f(DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int integ, int rows,
                      int pmask, hipStream_t s) {
     if (ntiles <= 0) return;
 #define LF_(RR) (pmask == 0 ? lf<RR, 0>(m, tiles, ntiles, grid, mode, integ, s)   \
