@@ -859,6 +859,23 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     for (Model *m : ms)
         if (integ_f && (m->int_flags & 3) != 3) integ_f = 4;
     frows = (opt.ty == 2) ? 2 : 4;
+    fzc = opt.zchunk;
+    if (fused && !opt.shape_set) {
+        // few tiles leave most of the GPU idle: 16-plane tiles of 4 rows while they fill
+        // two rounds of waves (2 per SIMD), else 4-plane tiles of 4 rows while they fill
+        // one, else 4-plane tiles of 2 rows (3 waves per SIMD).  At 64^3-160^3 the short
+        // tiles run 1.1-3x faster; from 256^3 the long ones win.
+        auto count = [&](int zc, int rows) {
+            long long t = 0;
+            for (Model *m : ms)
+                t += (long long)((m->d.Nx + 63) / 64) * ((m->d.Ny + rows - 1) / rows) * ((m->d.Nz + zc - 1) / zc);
+            return t;
+        };
+        const long long slots = (long long)cu_count * 4 * 2;
+        if (count(16, 4) >= 2 * slots) { fzc = 16; frows = 4; }
+        else if (count(4, 4) >= slots) { fzc = 4; frows = 4; }
+        else { fzc = 4; frows = 2; }
+    }
     for (size_t k = 0; k < ms.size(); k++) {
         Model *m = ms[k];
         if (fused) {
@@ -866,7 +883,7 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
             if (st != MFD_OK) return st;
         }
         hd.push_back(m->dm);
-        make_tiles(*m, (int)k, opt.zchunk, fused ? frows : opt.ty, kform, tiles);
+        make_tiles(*m, (int)k, fused ? fzc : opt.zchunk, fused ? frows : opt.ty, kform, tiles);
         if (ndims == 3 && kform == 2 && opt.zchunk_h > 0) make_tiles(*m, (int)k, opt.zchunk_h, opt.ty, kform, tiles_h);
     }
     pp_seen.assign(ms.size(), 0);
@@ -896,7 +913,7 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
         for (const DevSource &v : m->hsrc) aux_needed |= v.kind == SRC_TFSF && v.mode == 1 && !v.auxtab;
     }
     // the pipelined 3D E kernel finishes TFSF x-face cells itself
-    const bool xface = tfsf_gather && kform >= 2 && opt.zchunk <= (fused ? FUSED_XFACE_ZC : XFACE_ZC);
+    const bool xface = tfsf_gather && kform >= 2 && (fused ? fzc <= FUSED_XFACE_ZC : opt.zchunk <= XFACE_ZC);
     if (xface)
         for (size_t k = 0; k < ms.size(); k++) hd[k].xtfsf = ms[k]->dm.src;
     // the fused step stores the x faces' H corrections itself, and the gather skips
@@ -1313,6 +1330,7 @@ int set_option(RunOptions &o, Model *m, int opt, int value) {
         case MFD_OPT_ZCHUNK:
             if (value < 1 || value > 65535) { set_error("bad z chunk"); return MFD_ERR_ARG; }
             o.zchunk = value;
+            o.shape_set = true;
             break;
         case MFD_OPT_ZCHUNK_H:
             if (value < 0 || value > 65535) { set_error("bad H z chunk"); return MFD_ERR_ARG; }
@@ -1327,6 +1345,7 @@ int set_option(RunOptions &o, Model *m, int opt, int value) {
         case MFD_OPT_ROWS:
             if (value != 1 && value != 2 && value != 4 && value != 8) { set_error("rows per workgroup must be 1, 2, 4 or 8"); return MFD_ERR_ARG; }
             o.ty = value;
+            o.shape_set = true;
             break;
         case MFD_OPT_KERNEL_FORM:
             if (value != 0 && value != 2 && value != 3) {
