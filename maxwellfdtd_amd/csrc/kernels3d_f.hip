@@ -412,7 +412,10 @@ __global__ __launch_bounds__(64) void k_step3d_f(const DevModel *__restrict__ mo
             const bool bz = (PM & 4) && fz >= 0, bx = (PM & 1) && fx >= 0;
             const bool zf_p = zg == xb[4] - 1 || zg == xb[4] || zg == xb[5], zi_p = zg >= xb[4] && zg <= xb[5];
             // ---- every load of the plane, issued together: fields, integrals, stored D,
-            // psi pairs, the halo rows and columns
+            // psi pairs, the halo rows and columns.  Values a plane does not load (psi
+            // outside the slabs, D of plain cells, integrals of the prologue plane, a
+            // halo the boundary rule replaces) are left unset: every use sits behind the
+            // condition of its load
             float exn[R], eyn[R], ez[R], hx[R], hy[R], hz[R], sa[R], e2[R], dd[R][3];
             float hq[R][6], eq[R][6];
             int gidn[R];
@@ -428,10 +431,6 @@ __global__ __launch_bounds__(64) void k_step3d_f(const DevModel *__restrict__ mo
                 hx[r] = ldb<float>(Hx, o);
                 hy[r] = ldb<float>(Hy, o);
                 hz[r] = ldb<float>(Hz, o);
-                sa[r] = 0.0f;
-                e2[r] = 0.0f;
-#pragma unroll
-                for (int c = 0; c < 6; c++) { hq[r][c] = 0.0f; eq[r][c] = 0.0f; }
                 // cells that keep D: on a TFSF face plane and, from the id, cells a source
                 // writes and Kerr / ADE / plasma cells
                 bool d = false;
@@ -472,8 +471,7 @@ __global__ __launch_bounds__(64) void k_step3d_f(const DevModel *__restrict__ mo
             const float tex = ldb<float>(Ex, tB + zo), tez = ldb<float>(Ez, tB + zo);
             const float cey = ldb<float>(Ey, eB + zo), cez = ldb<float>(Ez, eB + zo);
             // the row below: Ex, Ez, old Hx, Hz and the psi values its Hx, Hz need
-            float bexv = 0.0f, bez = 0.0f, bhx = 0.0f, bhz = 0.0f;
-            float bq[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            float bexv, bez, bhx, bhz, bq[6];
             const bool bby = (PM & 2) && fyb >= 0;
             if (!pro && ybm == 0) {
                 bexv = ldb<float>(Ex, bB + zo); bez = ldb<float>(Ez, bB + zo);
@@ -489,8 +487,7 @@ __global__ __launch_bounds__(64) void k_step3d_f(const DevModel *__restrict__ mo
             }
             // the left column (lane t: row t): Ey, Ez, old Hy, Hz and the psi values its Hy,
             // Hz need
-            float hey = 0.0f, hez = 0.0f, hhy = 0.0f, hhz = 0.0f;
-            float xq[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            float hey, hez, hhy, hhz, xq[6];
             const bool xbx = (PM & 1) && fxh >= 0, xby = (PM & 2) && fyh >= 0;
             if (!pro && xhm == 0) {
                 hey = ldb<float>(Ey, xhB + zo); hez = ldb<float>(Ez, xhB + zo);
