@@ -211,7 +211,9 @@ int32_t mfd_model_destroy(mfd_model *m);
  * have no tunables; these select launch shapes of this engine) ----------------------- */
 enum {
     MFD_OPT_GRAPH_STEPS = 0,   /* steps captured per hipGraph replay (0 = eager launches) */
-    MFD_OPT_ZCHUNK = 1,        /* z planes marched by one 3D/2D tile (1..65535)            */
+    MFD_OPT_ZCHUNK = 1,        /* z planes marched by one 3D/2D tile (1..65535); unless   */
+                               /* ZCHUNK or ROWS is set, the fused 3D step picks 16x4,    */
+                               /* 4x4 or 4x2 tiles from the tile count                    */
     MFD_OPT_PERSISTENT = 2,    /* workgroups per CU of the persistent form (0 = off)       */
     MFD_OPT_KERNEL_TIMING = 3, /* hipEvent pairs around every H and E launch (eager)       */
     MFD_OPT_DFT_BLOCK = 4,     /* time steps per DFT flush (1..64, before upload)          */
