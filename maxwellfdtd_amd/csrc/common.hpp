@@ -47,6 +47,24 @@ __device__ __forceinline__ double sqrt_of(double x) { return sqrt(x); }
 // Two-element vectors usable through address-space-1 pointers (HIP's float2/double2
 // classes only bind to generic references).
 typedef float v2f __attribute__((ext_vector_type(2)));
+// Unsigned division by an invariant d as a multiply-high and two shifts (the round-up
+// method of Granlund and Montgomery, exact for every 32-bit n): {m, s1, s2, d}
+inline uint4 udiv_magic_of(unsigned d) {
+    unsigned l = 0;
+    while ((1ull << l) < d) l++;                       // ceil(log2 d)
+    const unsigned long long m = ((1ull << 32) * ((1ull << l) - d)) / d + 1;
+    uint4 r;
+    r.x = (unsigned)m;
+    r.y = l < 1 ? l : 1;
+    r.z = l > 1 ? l - 1 : 0;
+    r.w = d;
+    return r;
+}
+__device__ __forceinline__ unsigned udiv_magic(unsigned n, const uint4 &g) {
+    const unsigned t = __umulhi(g.x, n);
+    return (t + ((n - t) >> g.y)) >> g.z;
+}
+
 __host__ __device__ inline v2f mk2(float a, float b) {
     v2f v;
     v.x = a;
@@ -198,6 +216,9 @@ struct DevModel {
     const DevSource *xtfsf;     // TFSF source whose x faces the E kernel finishes, or null
     int xface_h;                // fused 3D step: it also stores the H corrections of the
                                 // x-face cells it finishes (touched then lists the others)
+    // cell index -> coordinates without integer division: n / Nx and n / (Nx Ny) as
+    // udiv_magic(n, divx) and udiv_magic(n, divxy)
+    uint4 divx, divxy;
     int dless;                  // plain cells keep no D: D = E/Me is rebuilt on demand
     // FP64 state (1D models with MFD_OPT_F64): the fields are F64[] (F[] is null), the
     // material table is mats64 (mats keeps the FP32 table for the id lookups of the

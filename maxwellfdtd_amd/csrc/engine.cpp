@@ -189,6 +189,8 @@ int Model::upload() {
     for (int i = 0; i < 6; i++) m.bcs[i] = d.bcs[i];
     m.Nt = d.Nt;
     m.int_flags = int_flags;
+    m.divx = udiv_magic_of((unsigned)d.Nx);
+    m.divxy = udiv_magic_of((unsigned)d.Nx * (unsigned)d.Ny);
     // per-cell FP32 arrays (fields, Sa, E2) come from one pool; array k starts k*stagger
     // bytes past a 256-byte boundary so that equal cell indices of different arrays do
     // not all land on the same HBM channel and bank

@@ -314,12 +314,15 @@ __global__ __launch_bounds__(256) void k_tfsf_gather(const DevModel *__restrict_
     bool active = k < m.ntouched;
     float exf = 0.0f;
     if (active) {
-        const DevSource &S = m.src[slot];
+        // the source record through the global address space: its fields are uniform
+        // and load as scalars
+        const __attribute__((address_space(1))) DevSource &S =
+            *(const __attribute__((address_space(1))) DevSource *)(m.src + slot);
         const int tk = G(m.touched)[k];
         const unsigned i = (unsigned)(tk & TOUCHED_CELL);
-        const int Nx = m.Nx, Ny = m.Ny;
-        const int c3[3] = {(int)(i % (unsigned)Nx), (int)((i / (unsigned)Nx) % (unsigned)Ny),
-                           (int)(i / ((unsigned)Nx * (unsigned)Ny))};
+        const unsigned qz = udiv_magic(i, m.divxy), rz = i - qz * m.divxy.w;
+        const unsigned qy = udiv_magic(rz, m.divx);
+        const int c3[3] = {(int)(rz - qy * m.divx.w), (int)qy, (int)qz};
         bool xh = false;
         if (m.xtfsf) xh = tfsf_xhandled(S, c3[0], c3[1], c3[2], (tk & TOUCHED_STATEFUL) != 0);
         // pp: a fused step ran and its buffer swap comes after this launch, so this

@@ -197,6 +197,30 @@ def test_batch_3d_uneven():
     b.close()
 
 
+def test_batch_3d_mixed_integrals():
+    """A batch of 3D models asking for Sa and E2, Sa only and E2 only runs the fused step
+    built for per-model integral variables: each model's fields and integrals equal its
+    run alone, and the Sa-only and E2-only models match the FP64 oracle."""
+    models = small_3d(count=3)
+    descs = [m.describe(integrals=flags) for m, flags in zip(models, (3, 1, 2))]
+    b = Batch(descs, options={_lib.MFD_OPT_GRAPH_STEPS: 8})
+    b.run(0)
+    for d, hb in zip(descs, b.handles):
+        ha = alone(d)
+        assert_same(hb, ha, 3)
+        if d.integrals & 2:
+            assert np.array_equal(hb.output(1), ha.output(1))
+        ha.close()
+    for d, hb in zip(descs[1:], b.handles[1:]):
+        ho = Handle(d, oracle("f64"))
+        ho.step(d.Nt)
+        assert max(field_errors(hb, ho, 3).values()) <= 1e-4
+        q = 0 if d.integrals & 1 else 1
+        assert rel_l2(hb.output(q), ho.output(q)) <= 1e-4, q
+        ho.close()
+    b.close()
+
+
 def test_server_results_equal_alone():
     models = small_3d(count=5, seed=11)
     descs = [m.describe() for m in models]
