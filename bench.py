@@ -602,13 +602,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-integrals", action="store_true")
     ap.add_argument("--graph-steps", type=int, default=16)
-    ap.add_argument("--zchunk", type=int, default=16)
+    ap.add_argument("--zchunk", type=int, default=None,
+                    help="z planes per tile (default: 10 for the fused step, 16 for the other forms)")
     ap.add_argument("--zchunk-h", type=int, default=8, help="H half z planes per tile (0 = --zchunk)")
     ap.add_argument("--xcd-block", type=int, default=64,
                     help="tiles per block dealt to the XCDs (0: one chunk per XCD; default 64)")
     ap.add_argument("--xcd-h", type=int, default=1,
                     help="H tiles in per-XCD contiguous chunks (1) or in grid order (0)")
-    ap.add_argument("--ty", type=int, default=4)
+    ap.add_argument("--ty", type=int, default=None,
+                    help="rows per lane / workgroup (default: 2 for the fused step, 4 for the other forms)")
     ap.add_argument("--kform", type=int, default=3,
                     help="3D kernel form: 3 fused step (default), 2 two-pass, 0 one cell per thread")
     ap.add_argument("--snake", type=int, default=1)
@@ -639,6 +641,12 @@ def main():
         args.steps = 2000
     if args.count is None:
         args.count = 512 if args.workload == "batched_2d" else 256
+    # the 3D tile shape: 10-plane tiles of 2 rows for the fused step (the engine's own
+    # choice at these sizes), 16 x 4 for the two-pass forms
+    if args.zchunk is None:
+        args.zchunk = 10 if args.kform == 3 else 16
+    if args.ty is None:
+        args.ty = 2 if args.kform == 3 else 4
 
     from maxwellfdtd_amd import _lib
     lib = _lib.engine()
