@@ -259,11 +259,16 @@ template <> struct Prec<double> {
 // Field access of the kernels shared by FP32 and FP64 models (sources, TFSF faces,
 // monitors, integrals): values cross as FP32 inputs or outputs; additions to the state
 // happen in the model's own precision.
-__device__ __forceinline__ float fld_get(const DevModel &m, int c, long long i) {
-    return m.f64 ? (float)m.F64[c][i] : m.F[c][i];
+// pp: a fused step ran and the buffer swap comes later in the step, so this step's H
+// and E are in Falt (D stays in place)
+__device__ __forceinline__ const float *fld_ptr(const DevModel &m, int c, int pp) {
+    return pp && (c < DX || c > DZ) ? m.Falt[c] : m.F[c];
 }
-__device__ __forceinline__ double fld_get64(const DevModel &m, int c, long long i) {
-    return m.f64 ? m.F64[c][i] : (double)m.F[c][i];
+__device__ __forceinline__ float fld_get(const DevModel &m, int c, long long i, int pp = 0) {
+    return m.f64 ? (float)m.F64[c][i] : fld_ptr(m, c, pp)[i];
+}
+__device__ __forceinline__ double fld_get64(const DevModel &m, int c, long long i, int pp = 0) {
+    return m.f64 ? m.F64[c][i] : (double)fld_ptr(m, c, pp)[i];
 }
 
 constexpr unsigned char ID_MASK = 0x7f, ID_TOUCHED = 0x80;
@@ -313,7 +318,7 @@ void launch_tfsf_gather(const DevModel *m, int nmodels, int slot, int maxn, bool
                         hipStream_t s, bool pp = false);
 void launch_tfsf_aux_table(const DevSource *S, int Nz, int Nt, int bcl, int bcr, float *tab,
                            hipStream_t s);
-void launch_monitors(const DevModel *m, int nmodels, int maxn, hipStream_t s);
+void launch_monitors(const DevModel *m, int nmodels, int maxn, hipStream_t s, bool pp = false);
 // force = false: models whose DFT block completes with this step's sample (launched
 // after the monitors, before launch_finish); force = true: every model's pending partial
 // block, to be followed by launch_dft_commit
@@ -326,6 +331,6 @@ void launch_d_from_e(const DevModel &m, hipStream_t s);
 // E of the cells of stateful (ADE / plasma) materials, after the E half-step kernel
 // updated their D: one work item per state slot, chunks {model, material, s0, s1}.
 void launch_e_state(const DevModel *m, const int4 *chunks, int nchunks, int ndims, bool integ,
-                    hipStream_t s);
+                    hipStream_t s, bool pp = false);
 
 }  // namespace mfd

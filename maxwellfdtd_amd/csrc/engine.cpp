@@ -933,9 +933,10 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
             maxtouched = std::max(maxtouched, ms[k]->n_touched_yz);
         }
     }
-    // no launch of the step but the gather (which then reads the new buffers where the
-    // fused launch left them) reads the fields after the fused launch: k_finish swaps
-    finish_swap = fused && !(split_state && nchunks) && !any_mon && (tfsf_gather || (maxsrc == 0 && !any_int));
+    // the launches after the fused one (state pass, TFSF gather, monitors) read the new
+    // buffers where it left them, and k_finish swaps; point sources and the touched-cell
+    // integrals still look in F, so with them the swap comes right after the step
+    finish_swap = fused && (tfsf_gather || (maxsrc == 0 && !any_int));
     dless_models.clear();
     // plain cells without stored D, for a state the caller has not written, on the
     // pipelined 3D kernels and the 2D kernels: D stays stored where the id says Kerr,
@@ -1138,7 +1139,7 @@ int Runner::launch_step(bool timing, bool dft_flush) {
     else if (ndims == 3) launch_e3d(d_models, d_tiles_e, ntiles, grid, mode, any_int, stream);
     else if (ndims == 2) launch_e2d(d_models, d_tiles_e, ntiles, grid, mode, any_int, opt.ty, stream);
     else launch_e1d(d_models, d_tiles_e, ntiles, grid, mode, any_int, f64, stream);
-    if (split_state && nchunks) launch_e_state(d_models, d_chunks, nchunks, ndims, any_int, stream);
+    if (split_state && nchunks) launch_e_state(d_models, d_chunks, nchunks, ndims, any_int, stream, finish_swap);
     if (timing) MFD_HIP(hipEventRecord(ev[3], stream));
     if (tfsf_gather) {
         if (aux_needed) launch_tfsf_aux(d_models, nm, 0, stream);
@@ -1149,7 +1150,7 @@ int Runner::launch_step(bool timing, bool dft_flush) {
             if (src_kind_mask[s] & 2) launch_tfsf(d_models, nm, s, ndims, maxface, stream, aux_needed);
         }
     }
-    if (any_mon) launch_monitors(d_models, nm, maxmon_n, stream);
+    if (any_mon) launch_monitors(d_models, nm, maxmon_n, stream, finish_swap);
     if (dft_flush) launch_dft_flush(d_models, nm, maxmon_n, false, stream);
     if (any_int && !tfsf_gather) launch_integrals_touched(d_models, nm, maxtouched, ndims, stream);
     if (finish_swap) launch_finish_swap(d_models, nm, stream);

@@ -14,7 +14,7 @@ namespace mfd {
 
 template <int NC, bool INTEG>
 __global__ __launch_bounds__(256) void k_e_state(const DevModel *__restrict__ models,
-                                                 const int4 *__restrict__ chunks, int nchunks) {
+                                                 const int4 *__restrict__ chunks, int nchunks, int pp) {
     const int4 ch = chunks[blockIdx.x];
     const DevModel &m = models[ch.x];
     const int it = *m.it;
@@ -36,17 +36,17 @@ __global__ __launch_bounds__(256) void k_e_state(const DevModel *__restrict__ mo
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             d[c] = G((const float *)m.F[cD[c]])[i];
-            e[c] = G((const float *)m.F[cE[c]])[i];
+            e[c] = G(fld_ptr(m, cE[c], pp))[i];
         }
         material_e<NC>(mt, s, it & 1, d, e, m.dt, m.dtf);
 #pragma unroll
-        for (int c = 0; c < NC; c++) G(m.F[cE[c]])[i] = e[c];
+        for (int c = 0; c < NC; c++) G((float *)fld_ptr(m, cE[c], pp))[i] = e[c];
         e0 = e[0];
         if (INTEG && !(G(m.id)[i] & ID_TOUCHED)) {
             const float dt = m.dtf;
             if (NC == 3) {
-                const float hx = G((const float *)m.F[HX])[i], hy = G((const float *)m.F[HY])[i];
-                const float hz = G((const float *)m.F[HZ])[i];
+                const float hx = G(fld_ptr(m, HX, pp))[i], hy = G(fld_ptr(m, HY, pp))[i];
+                const float hz = G(fld_ptr(m, HZ, pp))[i];
                 const float ex = e[0], ey = e[1 % NC], ez = e[2 % NC];
                 if (m.Sa) {
                     const float a = ey * hz - ez * hy, b = ez * hx - ex * hz, c = ex * hy - ey * hx;
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void k_e_state(const DevModel *__restrict__ mo
                 }
                 if (m.E2) G(m.E2)[i] += (ex * ex + ey * ey + ez * ez) * dt;
             } else if (NC == 2) {
-                const float hy = G((const float *)m.F[HY])[i];
+                const float hy = G(fld_ptr(m, HY, pp))[i];
                 const float ex = e[0], ez = e[1 % NC];
                 if (m.Sa) {
                     const float a = -ez * hy, b = ex * hy;
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(256) void k_e_state(const DevModel *__restrict__ mo
                 }
                 if (m.E2) G(m.E2)[i] += (ex * ex + ez * ez) * dt;
             } else {
-                const float hy = G((const float *)m.F[HY])[i];
+                const float hy = G(fld_ptr(m, HY, pp))[i];
                 const float ex = e[0];
                 if (m.Sa) {
                     const float a = ex * hy;
@@ -76,9 +76,9 @@ __global__ __launch_bounds__(256) void k_e_state(const DevModel *__restrict__ mo
 }
 
 void launch_e_state(const DevModel *m, const int4 *chunks, int nchunks, int ndims, bool integ,
-                    hipStream_t s) {
+                    hipStream_t s, bool pp) {
     if (nchunks <= 0) return;
-#define ES_(NC, IN) hipLaunchKernelGGL((k_e_state<NC, IN>), dim3(nchunks), dim3(256), 0, s, m, chunks, nchunks)
+#define ES_(NC, IN) hipLaunchKernelGGL((k_e_state<NC, IN>), dim3(nchunks), dim3(256), 0, s, m, chunks, nchunks, pp ? 1 : 0)
     if (ndims == 3) { if (integ) ES_(3, true); else ES_(3, false); }
     else if (ndims == 2) { if (integ) ES_(2, true); else ES_(2, false); }
     else { if (integ) ES_(1, true); else ES_(1, false); }

@@ -391,7 +391,7 @@ __device__ double block_sum(double v, double *sh) {
     return r;
 }
 
-__global__ __launch_bounds__(256) void k_monitors(const DevModel *__restrict__ models) {
+__global__ __launch_bounds__(256) void k_monitors(const DevModel *__restrict__ models, int pp) {
     __shared__ double sh[256];
     const DevModel &m = models[blockIdx.y];
     const int it0 = *m.it;
@@ -404,28 +404,28 @@ __global__ __launch_bounds__(256) void k_monitors(const DevModel *__restrict__ m
             if (blockIdx.x != 0) continue;
             for (int c = 0; c < M.ncomp; c++) {
                 double acc = 0.0;
-                for (int k = threadIdx.x; k < M.n; k += blockDim.x) acc += fld_get64(m, M.comps[c], M.cells[k]);
+                for (int k = threadIdx.x; k < M.n; k += blockDim.x) acc += fld_get64(m, M.comps[c], M.cells[k], pp);
                 const double tot = block_sum(acc, sh);
                 if (threadIdx.x == 0) M.sum[(size_t)it0 * M.ncomp + c] = tot;
             }
         } else if (!M.spectral) {
             for (int k = gid; k < M.n; k += stride)
                 for (int c = 0; c < M.ncomp; c++)
-                    M.rec[((size_t)it0 * M.ncomp + c) * M.n + k] = fld_get(m, M.comps[c], M.cells[k]);
+                    M.rec[((size_t)it0 * M.ncomp + c) * M.n + k] = fld_get(m, M.comps[c], M.cells[k], pp);
         } else {
             const int kk = it0 % m.dft_block;
             for (int k = gid; k < M.n; k += stride)
                 for (int c = 0; c < M.ncomp; c++)
-                    M.samp[((size_t)kk * M.ncomp + c) * M.n + k] = fld_get(m, M.comps[c], M.cells[k]);
+                    M.samp[((size_t)kk * M.ncomp + c) * M.n + k] = fld_get(m, M.comps[c], M.cells[k], pp);
         }
     }
 }
 
-void launch_monitors(const DevModel *m, int nmodels, int maxn, hipStream_t s) {
+void launch_monitors(const DevModel *m, int nmodels, int maxn, hipStream_t s, bool pp) {
     int nb = (maxn + 255) / 256;
     if (nb < 1) nb = 1;
     if (nb > 1024) nb = 1024;
-    hipLaunchKernelGGL(k_monitors, dim3(nb, nmodels), dim3(256), 0, s, m);
+    hipLaunchKernelGGL(k_monitors, dim3(nb, nmodels), dim3(256), 0, s, m, pp ? 1 : 0);
 }
 
 // ---- time-blocked DFT (src/dft.jl:27-51): S += sum_k F_k * 2 exp(-i w (t_k - tshift)) dt
