@@ -238,11 +238,14 @@ enum {
                                   plasma state, source waveforms, TFSF tables and Sa/E2
                                   in FP64 (the reference's CPU{Float64} run; set before
                                   upload); monitor samples are stored in FP32           */
-    MFD_OPT_TFSF_FOLD = 18     /* fused 3D step with a TFSF source alone: 1 (default)
+    MFD_OPT_TFSF_FOLD = 18,    /* fused 3D step with a TFSF source alone: 1 (default)
                                   stores the x-face H corrections in the step launch, so
                                   the face launch after it walks only the y/z-face cells;
                                   0 leaves them to the face launch; results are bitwise
                                   equal                                                  */
+    MFD_OPT_FUSED_2D = 19      /* 2D models: 1 runs the fused step (one z march per step
+                                  over two Hy/Ex/Ez buffer sets), 0 (default) the two-pass
+                                  kernels; results agree to rounding                    */
 };
 /* Changing an option rebuilds the launch plan; field state is kept.                   */
 int32_t mfd_model_set_options(mfd_model *m, int32_t opt, int32_t value);

@@ -16,7 +16,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmaxwellfdtd_amd.so")
 PROBE_LIB = os.path.join(HERE, "libmfd_hbm_probe.so")
-SOURCES = ["kernels3d.hip", "kernels3d_p.hip", "kernels3d_f.hip", "kernels2d.hip", "kernels1d.hip", "step_kernels.hip",
+SOURCES = ["kernels3d.hip", "kernels3d_p.hip", "kernels3d_f.hip", "kernels2d.hip", "kernels2d_f.hip", "kernels1d.hip",
+           "step_kernels.hip",
            "kernels_state.hip", "energy.hip", "engine.cpp", "batch.cpp"]
 PROBE_SOURCES = ["hbm_probe.hip"]
 
@@ -30,7 +31,7 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=f
 # writes its fused multiply-adds out, so that a cell's H recomputed by a neighbouring
 # tile rounds exactly as the owner's (-ffp-contract=fast would let the backend fuse other
 # pairs, differently in each kernel instantiation)
-NO_CONTRACT = {"kernels3d_f.hip"}
+NO_CONTRACT = {"kernels3d_f.hip", "kernels2d_f.hip"}
 
 
 def _compile(src):

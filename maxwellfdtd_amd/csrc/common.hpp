@@ -96,6 +96,8 @@ struct DevLayer {
     // 8-byte access and one base pointer serve both
     v2f *pE, *pH;           // H and E half-step psi pairs
     v2f *pEb;               // fused 3D step: the other buffer of the H half-step pairs
+    float *psiEb;           // fused 2D step: the other buffer of the face's H half-step
+                            // psi (psiE1 on x faces, psiE0 on z faces)
 };
 
 // Material coefficient table entry (index 0 = vacuum) with the material's compact
@@ -303,6 +305,8 @@ void launch_step3d_f(DevModel *m, const Tile *tiles, int ntiles, int grid, int m
                      int pmask, hipStream_t s);
 void launch_swap_pp(DevModel *m, int nmodels, hipStream_t s);
 void launch_finish_swap(DevModel *m, int nmodels, hipStream_t s);
+void launch_step2d_f(DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int integ, bool pml,
+                     hipStream_t s);
 void launch_h2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, int tw, hipStream_t s);
 void launch_e2d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ, int tw, hipStream_t s);
 void launch_h1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool f64,

@@ -79,11 +79,16 @@ int64_t Model::footprint() const {
     }
     if (int_flags & MFD_INT_SA) b += N * 4;
     if (int_flags & MFD_INT_E2) b += N * 4;
-    // the fused 3D step's second H/E buffers and H-half psi arrays
+    // the fused step's second H/E buffers and H-half psi arrays
     if (d.ndims == 3 && !f64) {
         b += 6 * N * 4;
         for (int f = 0; f < 6; f++)
             if (L[f].present) b += (int64_t)2 * psi_len(*this, f, L[f].nb) * 4;
+    }
+    if (d.ndims == 2 && !f64) {
+        b += 3 * N * 4;
+        for (int f = 0; f < 6; f++)
+            if (L[f].present) b += (int64_t)psi_len(*this, f, L[f].nb) * 4;
     }
     return b;
 }
@@ -619,13 +624,14 @@ int Model::upload() {
     return MFD_OK;
 }
 
-// The second H/E buffers and H-half psi arrays of the fused 3D step, allocated the first
-// time a runner steps the model in that form.  Each step writes every cell of them, so
-// their initial contents never reach a result; they start zeroed all the same.
+// The second H/E buffers and H-half psi arrays of the fused step (3D or 2D), allocated
+// the first time a runner steps the model in that form.  Each step writes every cell of
+// them, so their initial contents never reach a result; they start zeroed all the same.
 int Model::ensure_pp() {
     if (pp_alloc) return MFD_OK;
     const int cs[6] = {HX, HY, HZ, EX, EY, EZ};
     for (int c : cs) {
+        if (!comp_present(c)) continue;
         MFD_HIP(arena.alloc(&dm.Falt[c], (size_t)N));
         MFD_HIP(hipMemsetAsync(dm.Falt[c], 0, (size_t)N * sizeof(float), io));
     }
@@ -635,11 +641,19 @@ int Model::ensure_pp() {
         const size_t n0 = l0.present ? psi_len(*this, 2 * a, l0.nb) : 0;
         const size_t n1 = l1.present ? psi_len(*this, 2 * a + 1, l1.nb) : 0;
         if (n0 + n1 == 0) continue;
-        v2f *q;
-        MFD_HIP(arena.alloc(&q, n0 + n1));
-        MFD_HIP(hipMemsetAsync(q, 0, (n0 + n1) * sizeof(v2f), io));
-        if (l0.present) l0.pEb = q;
-        if (l1.present) l1.pEb = q + n0;
+        if (d.ndims == 3) {
+            v2f *q;
+            MFD_HIP(arena.alloc(&q, n0 + n1));
+            MFD_HIP(hipMemsetAsync(q, 0, (n0 + n1) * sizeof(v2f), io));
+            if (l0.present) l0.pEb = q;
+            if (l1.present) l1.pEb = q + n0;
+        } else {
+            float *q;
+            MFD_HIP(arena.alloc(&q, n0 + n1));
+            MFD_HIP(hipMemsetAsync(q, 0, (n0 + n1) * sizeof(float), io));
+            if (l0.present) l0.psiEb = q;
+            if (l1.present) l1.psiEb = q + n0;
+        }
     }
     MFD_HIP(hipStreamSynchronize(io));
     pp_alloc = true;
@@ -672,11 +686,16 @@ int Model::ensure_touched_yz() {
     return MFD_OK;
 }
 
-// Host mirror of the buffer swap the fused step's last tile makes on the device.
+// Host mirror of the buffer swap made on the device after a fused step (k_swap_pp or
+// k_finish_swap).
 void Model::swap_pp() {
     const int cs[6] = {HX, HY, HZ, EX, EY, EZ};
     for (int c : cs) std::swap(dm.F[c], dm.Falt[c]);
-    for (int f = 0; f < 6; f++) std::swap(dm.L[f].pE, dm.L[f].pEb);
+    for (int f = 0; f < 6; f++) {
+        DevLayer &L = dm.L[f];
+        if (d.ndims == 3) std::swap(L.pE, L.pEb);
+        else std::swap(f < 2 ? L.psiE1 : L.psiE0, L.psiEb);
+    }
     pp_parity ^= 1;
 }
 
@@ -854,6 +873,17 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
         if (!ok) kform = 2;
     }
     fused = kform == 3;
+    // the fused 2D step (MFD_OPT_FUSED_2D): FP32 2D models with mu = 1 everywhere, at least
+    // two cells along x and z and no cell inside both CPML layers of one axis; otherwise the
+    // two-pass 2D kernels run
+    fused2 = ndims == 2 && opt.fused_2d && !f64 && !h_mats;
+    if (fused2)
+        for (Model *m : ms) {
+            fused2 &= m->d.Nx >= 2 && m->d.Nz >= 2 && m->N < (1LL << 30);
+            for (int a = 0; a < 3; a += 2)
+                if (m->L[2 * a].present && m->L[2 * a + 1].present) fused2 &= m->L[2 * a + 1].ib > m->L[2 * a].ib;
+        }
+    pp_step = fused || fused2;
     // the fused step's integral variables: none, Sa and E2 in every model, or per model
     integ_f = 0;
     for (Model *m : ms)
@@ -881,12 +911,12 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     }
     for (size_t k = 0; k < ms.size(); k++) {
         Model *m = ms[k];
-        if (fused) {
+        if (pp_step) {
             int st = m->ensure_pp();
             if (st != MFD_OK) return st;
         }
         hd.push_back(m->dm);
-        make_tiles(*m, (int)k, fused ? fzc : opt.zchunk, fused ? frows : opt.ty, kform, tiles);
+        make_tiles(*m, (int)k, fused ? fzc : opt.zchunk, fused ? frows : fused2 ? 1 : opt.ty, kform, tiles);
         if (ndims == 3 && kform == 2 && opt.zchunk_h > 0) make_tiles(*m, (int)k, opt.zchunk_h, opt.ty, kform, tiles_h);
     }
     pp_seen.assign(ms.size(), 0);
@@ -936,7 +966,7 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     // the launches after the fused one (state pass, TFSF gather, monitors) read the new
     // buffers where it left them, and k_finish swaps; point sources and the touched-cell
     // integrals still look in F, so with them the swap comes right after the step
-    finish_swap = fused && (tfsf_gather || (maxsrc == 0 && !any_int));
+    finish_swap = pp_step && (tfsf_gather || (maxsrc == 0 && !any_int));
     dless_models.clear();
     // plain cells without stored D, for a state the caller has not written, on the
     // pipelined 3D kernels and the 2D kernels: D stays stored where the id says Kerr,
@@ -1124,6 +1154,10 @@ int Runner::launch_step(bool timing, bool dft_flush) {
         if (side) { MFD_HIP(hipEventRecord(join_ev, side)); MFD_HIP(hipStreamWaitEvent(stream, join_ev, 0)); }
     }
     else if (ndims == 3) launch_h3d(d_models, d_tiles, ntiles_h, grid_h, hmode, stream);
+    else if (fused2) {
+        launch_step2d_f(d_models, d_tiles, ntiles_h, grid_h, mode, integ_f, any_pml, stream);
+        if (!finish_swap) launch_swap_pp(d_models, nm, stream);
+    }
     else if (ndims == 2) launch_h2d(d_models, d_tiles, ntiles_h, grid_h, hmode, opt.ty, stream);
     else launch_h1d(d_models, d_tiles, ntiles_h, grid_h, hmode, f64, stream);
     if (timing) { MFD_HIP(hipEventRecord(ev[1], stream)); MFD_HIP(hipEventRecord(ev[2], stream)); }
@@ -1137,6 +1171,9 @@ int Runner::launch_step(bool timing, bool dft_flush) {
         if (side) { MFD_HIP(hipEventRecord(join_ev, side)); MFD_HIP(hipStreamWaitEvent(stream, join_ev, 0)); }
     }
     else if (ndims == 3) launch_e3d(d_models, d_tiles_e, ntiles, grid, mode, any_int, stream);
+    else if (fused2) {
+        // the fused launch did the E half too
+    }
     else if (ndims == 2) launch_e2d(d_models, d_tiles_e, ntiles, grid, mode, any_int, opt.ty, stream);
     else launch_e1d(d_models, d_tiles_e, ntiles, grid, mode, any_int, f64, stream);
     if (split_state && nchunks) launch_e_state(d_models, d_chunks, nchunks, ndims, any_int, stream, finish_swap);
@@ -1173,7 +1210,7 @@ void Runner::advance(size_t k) {
     if (m->it >= m->d.Nt) return;
     m->it++;
     if (m->it - m->dft_done == dft_block) m->dft_done = m->it;
-    if (fused) {
+    if (pp_step) {
         m->swap_pp();
         pp_seen[k] = m->pp_parity;
     }
@@ -1191,6 +1228,9 @@ int Runner::refresh() {
         for (int f = 0; f < 6; f++) {
             h.L[f].pE = m->dm.L[f].pE;
             h.L[f].pEb = m->dm.L[f].pEb;
+            h.L[f].psiE0 = m->dm.L[f].psiE0;
+            h.L[f].psiE1 = m->dm.L[f].psiE1;
+            h.L[f].psiEb = m->dm.L[f].psiEb;
         }
         MFD_HIP(copy_sync(d_models + k, &h, sizeof(DevModel), hipMemcpyHostToDevice, stream));
         pp_seen[k] = m->pp_parity;
@@ -1377,6 +1417,7 @@ int set_option(RunOptions &o, Model *m, int opt, int value) {
             m->f64 = value != 0;
             break;
         case MFD_OPT_TFSF_FOLD: o.tfsf_fold = value != 0; break;
+        case MFD_OPT_FUSED_2D: o.fused_2d = value != 0; break;
         case MFD_OPT_STAGGER:
             if (!m || m->uploaded || value < 0) { set_error("stagger must be >= 0, set on a model before upload"); return MFD_ERR_ARG; }
             m->stagger = value;

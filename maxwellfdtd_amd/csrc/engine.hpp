@@ -155,6 +155,7 @@ struct RunOptions {
     bool xcd_h = true;          // 3D H tile list dealt to the XCDs like E's (else grid order)
     int xcd_block = 64;         // 3D: tiles per block dealt to the XCDs (0: one chunk per XCD)
     bool tfsf_fold = true;      // fused 3D step: x-face H corrections inside the launch
+    bool fused_2d = false;      // 2D models: the fused step instead of the two-pass kernels
     bool shape_set = false;     // ZCHUNK or ROWS set by the caller (else the fused step
                                 // picks its tile shape from the tile count)
 };
@@ -178,6 +179,8 @@ struct Runner {
     int maxtouched = 1;
     int kform = 0;              // 3D kernel form in use
     bool fused = false;         // kform 3: one launch per step over ping-pong H/E buffers
+    bool fused2 = false;        // the same for 2D models
+    bool pp_step = false;       // fused || fused2: the models' buffers swap every step
     int frows = 2;              // fused step: rows per lane
     int fzc = 16;               // fused step: z planes per tile
     int integ_f = 0;            // fused step: 0 no integrals, 3 Sa and E2 everywhere, 4 per model

@@ -29,6 +29,7 @@
 #include "common.hpp"
 #include "material.hpp"
 #include "tfsf.hpp"
+#include "fused.hpp"
 
 #include <algorithm>
 
@@ -36,60 +37,7 @@
 namespace mfd {
 namespace {
 
-typedef const __attribute__((address_space(1))) char gcC;
-typedef __attribute__((address_space(1))) char gC;
-
-// global accesses at a byte offset from a base pointer
-template <class T>
-__device__ __forceinline__ T ldb(const void *base, unsigned off) {
-    return *(const __attribute__((address_space(1))) T *)((gcC *)base + off);
-}
-template <class T>
-__device__ __forceinline__ void stb(void *base, unsigned off, T v) {
-    *(__attribute__((address_space(1))) T *)((gC *)base + off) = v;
-}
-__device__ __forceinline__ float ldnt(const void *base, unsigned off) {
-    return __builtin_nontemporal_load((const __attribute__((address_space(1))) float *)((gcC *)base + off));
-}
-__device__ __forceinline__ void stnt(void *base, unsigned off, float v) {
-    __builtin_nontemporal_store(v, (__attribute__((address_space(1))) float *)((gC *)base + off));
-}
-
-__device__ __forceinline__ float rdlane(float v, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-// value of lane + 1 / lane - 1 (DPP wave shifts; the edge lane's result is not used)
-__device__ __forceinline__ float from_next(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
-}
-__device__ __forceinline__ float from_prev(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
-}
-
-// profile values of one CPML slab at one position: B, A and 1/K (src/pml.jl:68-81)
-struct Prof {
-    float B, A, Ki;
-};
-
-// CPML slab of axis a at coordinate c of that axis: face index (-1 none) and profile index
-__device__ __forceinline__ int slab_of(const DevModel &m, int a, int c, int &p) {
-    const DevLayer &L0 = m.L[2 * a], &L1 = m.L[2 * a + 1];
-    if (L0.present && c <= L0.ib) { p = c; return 2 * a; }
-    if (L1.present && c >= L1.ib) { p = c - L1.ib; return 2 * a + 1; }
-    p = 0;
-    return -1;
-}
-
-__device__ __forceinline__ Prof prof_of(const DevModel &m, int f, int p) {
-    Prof r{0.0f, 0.0f, 1.0f};
-    if (f >= 0) {
-        const DevLayer &L = m.L[f];
-        r.B = G(L.B)[p];
-        r.A = G(L.A)[p];
-        r.Ki = G(L.Ki)[p];
-    }
-    return r;
-}
+using namespace fused;
 
 struct Hv {
     float x, y, z;
@@ -183,13 +131,6 @@ __device__ __forceinline__ void e_cell(float Md1, float Md2, float idx, float id
     d1 = __builtin_fmaf(Md1, d1, Md2 * (dHxz - dHzx));
     d2 = __builtin_fmaf(Md1, d2, Md2 * (dHyx - dHxy));
 }
-
-// boundary rule for a neighbour outside the grid: 0 the loaded (periodic) value, 1 zero
-// (Dirichlet), 2 the cell's own value (Neumann), as in src/models.jl:755-802,920-973
-__device__ __forceinline__ float bcsel(int mode, float loaded, float own) {
-    return mode == 0 ? loaded : mode == 1 ? 0.0f : own;
-}
-__device__ __forceinline__ int bcmode(int bc) { return bc == BC_PERIODIC ? 0 : bc == BC_DIRICHLET ? 1 : 2; }
 
 }  // namespace
 
@@ -710,9 +651,16 @@ __global__ void k_swap_pp(DevModel *models, int nmodels) {
     }
     for (int f = 0; f < 6; f++) {
         DevLayer &L = m.L[f];
-        v2f *a = L.pE;
-        L.pE = L.pEb;
-        L.pEb = a;
+        if (m.ndims == 3) {
+            v2f *a = L.pE;
+            L.pE = L.pEb;
+            L.pEb = a;
+        } else {
+            float *&e = f < 2 ? L.psiE1 : L.psiE0;
+            float *a = e;
+            e = L.psiEb;
+            L.psiEb = a;
+        }
     }
 }
 
@@ -732,9 +680,16 @@ __global__ void k_finish_swap(DevModel *models, int nmodels) {
     }
     for (int f = 0; f < 6; f++) {
         DevLayer &L = m.L[f];
-        v2f *a = L.pE;
-        L.pE = L.pEb;
-        L.pEb = a;
+        if (m.ndims == 3) {
+            v2f *a = L.pE;
+            L.pE = L.pEb;
+            L.pEb = a;
+        } else {
+            float *&e = f < 2 ? L.psiE1 : L.psiE0;
+            float *a = e;
+            e = L.psiEb;
+            L.psiEb = a;
+        }
     }
     *m.it = it + 1;
     if (it + 1 - *m.dftdone == m.dft_block) *m.dftdone = it + 1;

@@ -226,8 +226,9 @@ def nonlinear_3d(n=24, steps=150, I0=1e17, rate="power"):
     return model
 
 
-def lorentz_2d(nx=96, nz=80, steps=200, seed=3, tfsf=True, kmax=None):
-    """2D (x, z) Lorentz rods and a dielectric slab, TFSF plane wave + soft point source."""
+def lorentz_2d(nx=96, nz=80, steps=200, seed=3, tfsf=True, kmax=None, bc="periodic", pml=5):
+    """2D (x, z) Lorentz rods and a dielectric slab, TFSF plane wave + soft point source
+    (faces `bc`, CPML `pml` cells thick)."""
     rng = np.random.default_rng(seed)
     d = 40e-9
     grid = Grid(xmin=0, xmax=(nx - 1) * d, Nx=nx, zmin=0, zmax=(nz - 1) * d, Nz=nz)
@@ -247,7 +248,7 @@ def lorentz_2d(nx=96, nz=80, steps=200, seed=3, tfsf=True, kmax=None):
         srcs.append(TFSFSource(box=(8 * d, Lx - 8 * d, 8 * d, Lz - 8 * d), waveform=w,
                                component="Ex", z0=6 * d))
     dt = 0.99 / C0 / np.sqrt(2 / d ** 2)
-    model = Model(grid, srcs, tmax=(steps - 1) * dt, CN=0.99, pml=_cpml(5, d, kmax), material=mats)
+    model = Model(grid, srcs, tmax=(steps - 1) * dt, CN=0.99, pml=_cpml(pml, d, kmax), material=mats, bc=bc)
     w0 = 2 * np.pi * C0 / 800e-9
     mons = [FieldMonitor((0.5 * Lx, 0.8 * Lz)),
             SpectralMonitor(lambda x, z: (np.abs(z - 0.8 * Lz) < d / 2) & (x > 0.2 * Lx) & (x < 0.8 * Lx),

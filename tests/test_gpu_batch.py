@@ -45,16 +45,21 @@ def assert_same(h1, h2, ndims, sa_tol=0.0):
             assert rel_l2(h1.output(0), h2.output(0)) <= sa_tol
 
 
-def test_batch_2d_equals_alone_and_oracle():
+@pytest.mark.parametrize("fused", [0, 1])
+def test_batch_2d_equals_alone_and_oracle(fused):
+    """A batch of uneven 2D solves (two-pass kernels, or the fused 2D step) equals each
+    solve run alone, bit for bit, and the FP64 oracle."""
     models = small_2d()
     descs = [m.describe() for m in models]
     assert len({d.Nt for d in descs}) > 1 and len({d.dims for d in descs}) > 1
-    b = Batch(descs)
+    b = Batch(descs, options={_lib.MFD_OPT_FUSED_2D: fused})
     b.run(0)
     assert b.active == 0
     for d, hb in zip(descs, b.handles):
         assert hb.it == d.Nt
-        ha = alone(d)
+        ha = Handle(d, _lib.engine())
+        ha.lib.check(ha.lib.model_set_options(ha.h, _lib.MFD_OPT_FUSED_2D, fused), "set_options")
+        ha.step(d.Nt)
         assert_same(hb, ha, 2)
         ha.close()
     for d, hb in list(zip(descs, b.handles))[:2]:

@@ -137,18 +137,21 @@ def test_nonlinear_3d(rate):
     assert_parity(hg, ho, desc, model)
 
 
-def test_nonlinear_2d():
+@pytest.mark.parametrize("fused", [0, 1])
+def test_nonlinear_2d(fused):
     """2D Kerr + plasma and Kerr-only cells keep their D (the E kernel loads it once the
-    id is known); the plain cells around them run without it."""
+    id is known); the plain cells around them run without it.  Two-pass kernels and the
+    fused 2D step."""
     model = cases.nonlinear_2d()
-    hg, ho, desc = run_pair(model)
+    hg, ho, desc = run_pair(model, options={_lib.MFD_OPT_FUSED_2D: fused})
     assert_parity(hg, ho, desc, model)
     assert np.max(ho.output(3)) > 0          # the pulse ionised the medium
 
 
-def test_lorentz_2d():
+@pytest.mark.parametrize("fused", [0, 1])
+def test_lorentz_2d(fused):
     model, mons = cases.lorentz_2d()
-    hg, ho, desc = run_pair(model, mons)
+    hg, ho, desc = run_pair(model, mons, options={_lib.MFD_OPT_FUSED_2D: fused})
     assert_parity(hg, ho, desc, model)
 
 
@@ -399,6 +402,36 @@ def test_fused_step_tiling_is_bitwise_neutral(bc):
     for o in outs[1:]:
         for a, b in zip(outs[0], o):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("bc,pml", [("periodic", 5), ("dirichlet", 4), ("neumann", 0)])
+def test_fused_2d_step_tiling_is_bitwise_neutral(bc, pml):
+    """The fused 2D step (MFD_OPT_FUSED_2D: one z march per step over ping-pong Hy/Ex/Ez buffers, Hy
+    recomputed one column left of and one row below each tile) gives the same bits for
+    z chunks of 1 to 64 planes, with faces `bc`; the grid (150 x 77) leaves tiles partial
+    in x and z.  It matches the FP64 oracle, and the two-pass 2D kernels to rounding."""
+    model, mons = cases.lorentz_2d(nx=150, nz=77, steps=140, bc=bc, pml=pml)
+    desc = model.describe(monitors=mons, integrals=3)
+    outs = []
+    for zc in (16, 1, 7, 64):
+        h = Handle(desc, _lib.engine())
+        h.lib.check(h.lib.model_set_options(h.h, _lib.MFD_OPT_FUSED_2D, 1), "set_options")
+        h.lib.check(h.lib.model_set_options(h.h, _lib.MFD_OPT_ZCHUNK, zc), "set_options")
+        h.step(model.Nt)
+        outs.append([h.field(c) for c in ("Hy", "Dx", "Dz", "Ex", "Ez")] + [h.output(0), h.output(1)]
+                    + [np.ravel(h.monitor(q, "Ez")) for q in range(len(mons))])
+        if len(outs) == 1:
+            ho = Handle(desc, oracle("f64"))
+            ho.step(model.Nt)
+            assert_parity(h, ho, desc, model)
+            two = Handle(desc, _lib.engine())
+            two.step(model.Nt)
+            errs = field_errors(h, two, 2)
+            assert max(errs.values()) <= 1e-5, errs
+        h.close()
+    for o in outs[1:]:
+        for k, (a, b) in enumerate(zip(outs[0], o)):
+            assert np.array_equal(a, b), k
 
 
 def test_fused_step_buffers_follow_the_model():
