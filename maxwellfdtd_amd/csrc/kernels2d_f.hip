@@ -167,6 +167,34 @@ __global__ __launch_bounds__(64) void k_step2d_f(const DevModel *__restrict__ mo
             return (f == 5 ? z1B : 0u) + cB + 4u * (unsigned)Nx * (unsigned)p;
         };
 
+        // Loads of a plane, in two groups.  The unconditional group (U) of plane p + 1 is
+        // issued while plane p is computed, so its latency overlaps that work: the row's
+        // Ez, Hy and integrals, Ex and the id of the plane above, the halo columns.  The
+        // group that depends on the cells (stored D, CPML psi) is issued at the start of
+        // the plane, before the next U group, so that waiting for it leaves the U group in
+        // flight (vector loads complete in order).  Values a plane does not load (psi
+        // outside the slabs, D of plain cells, the prologue plane's integrals, a halo the
+        // boundary rule replaces) stay unset: every use sits behind the condition of its
+        // load.
+        struct U2 {
+            float exn, ez, hy, hexn, cez, hez, hhy, sa, e2;
+            int gidn;
+        };
+        auto loadU = [&](int p, U2 &u) {
+            const int zg = p < 0 ? p + Nz : p;
+            const unsigned zo = szB * (unsigned)zg, zn = szB * wz(p + 1), o = colB + zo;
+            u.exn = ldb<float>(Ex, colB + zn);
+            u.gidn = useg ? (int)ldb<unsigned char>(id, (colB + zn) >> 2) : 0;
+            u.ez = ldb<float>(Ez, o);
+            u.hy = ldb<float>(Hy, o);
+            u.hexn = ldb<float>(Ex, xhB + zn);
+            u.cez = ldb<float>(Ez, xeB + zo);
+            u.hez = ldb<float>(Ez, xhB + zo);
+            u.hhy = ldb<float>(Hy, xhB + zo);
+            if (doSa) u.sa = ldnt(Sa, o);
+            if (doE2) u.e2 = ldnt(E2, o);
+        };
+
         // carried from one plane to the next: Ex and the id of the row (loaded with the
         // plane below as its +z neighbour), Ex of the left column, the new Hy below
         float ex, hex, hym = 0.0f;
@@ -180,11 +208,12 @@ __global__ __launch_bounds__(64) void k_step2d_f(const DevModel *__restrict__ mo
         bool zm_own = zbm == 2;
         int nonfin = 0;
 
-        for (int p = pb; p < z1; p++) {
+        auto plane = [&](int p, const U2 &u, U2 &un) {
             const int zg = p < 0 ? p + Nz : p;
             const int ztm = zg == Nz - 1 ? zhm : 0;   // rule for the +z neighbour of this plane
             const bool pro = p < z0;                   // the plane below the tile: Hy only
-            const unsigned zo = szB * (unsigned)zg, zn = szB * wz(p + 1);
+            const unsigned zo = szB * (unsigned)zg;
+            const unsigned o = colB + zo;
             int fz = -1, pz = 0;
             Prof zP{0.0f, 0.0f, 1.0f};
             if (PM) {
@@ -192,15 +221,8 @@ __global__ __launch_bounds__(64) void k_step2d_f(const DevModel *__restrict__ mo
                 zP = prof_of(m, fz, pz);
             }
             const bool bz = PM && fz >= 0, bx = PM && fx >= 0, bxh = PM && fxh >= 0;
-            // ---- every load of the plane, issued together.  Values a plane does not
-            // load (psi outside the slabs, D of plain cells, the prologue plane's
-            // integrals, a halo the boundary rule replaces) stay unset: every use sits
-            // behind the condition of its load
-            const unsigned o = colB + zo;
-            const float exn = ldb<float>(Ex, colB + zn);
-            const int gidn = useg ? (int)ldb<unsigned char>(id, (colB + zn) >> 2) : 0;
-            const float ez = ldb<float>(Ez, o), hy = ldb<float>(Hy, o);
-            float sa, e2, d0s, d1s, qhx, qhz, qex, qez;
+            // ---- the cell-dependent loads of this plane, then the U group of the next
+            float d0s, d1s, qhx, qhz, qex, qez, qxh, qzh;
             bool d = false;
             if (!pro) {
                 d = !dless;
@@ -208,8 +230,6 @@ __global__ __launch_bounds__(64) void k_step2d_f(const DevModel *__restrict__ mo
                     d = (gid & ID_TOUCHED) || (MODE == MODE_FULL && ctab[gid & ID_MASK].w != 0.0f);
                 }
                 if (d) { d0s = ldb<float>(Dx, o); d1s = ldb<float>(Dz, o); }
-                if (doSa) sa = ldnt(Sa, o);
-                if (doE2) e2 = ldnt(E2, o);
             }
             if (bx) {
                 const unsigned j = xo(fx, px, zg);
@@ -221,21 +241,17 @@ __global__ __launch_bounds__(64) void k_step2d_f(const DevModel *__restrict__ mo
                 qhz = ldb<float>(ZE, j);
                 if (!pro) qez = ldb<float>(ZH, j);
             }
-            const float hexn = ldb<float>(Ex, xhB + zn);
-            const float cez = ldb<float>(Ez, xeB + zo);
-            float hez, hhy, qxh, qzh;
             if (!pro && xhm == 0) {
-                hez = ldb<float>(Ez, xhB + zo);
-                hhy = ldb<float>(Hy, xhB + zo);
                 if (bxh) qxh = ldb<float>(XE, xo(fxh, pxh, zg));
                 if (bz) qzh = ldb<float>(ZE, zof(fz, pz, xhB));
             }
+            if (p + 1 < z1) loadU(p + 1, un);
 
             // ---- new Hy of the row (src/models.jl:455-530)
-            float ezx = from_next(ez);
-            if (lane == lx) ezx = bcsel(xem, cez, ez);
-            const float exz = ztm == 0 ? exn : bcsel(ztm, 0.0f, ex);
-            const float hn = h_cell2(Mh, idx, idz, ex, ez, ezx, exz, hy, bx, xP, bz, zP, qhx, qhz);
+            float ezx = from_next(u.ez);
+            if (lane == lx) ezx = bcsel(xem, u.cez, u.ez);
+            const float exz = ztm == 0 ? u.exn : bcsel(ztm, 0.0f, ex);
+            const float hn = h_cell2(Mh, idx, idz, ex, u.ez, ezx, exz, u.hy, bx, xP, bz, zP, qhx, qhz);
             if (PM && !pro && xin) {
                 if (bx) stb<float>(XEw, xo(fx, px, zg), qhx);
                 if (bz) stb<float>(ZEw, zof(fz, pz, colB), qhz);
@@ -244,9 +260,9 @@ __global__ __launch_bounds__(64) void k_step2d_f(const DevModel *__restrict__ mo
                 // ---- new Hy of the column left of lane 0 (recomputed; its tile stores it)
                 float hyh = 0.0f;
                 if (xhm == 0) {
-                    const float ezxh = bcsel(xhn, rdlane(ez, 0), hez);
-                    const float exzh = ztm == 0 ? hexn : bcsel(ztm, 0.0f, hex);
-                    hyh = h_cell2(Mh, idx, idz, hex, hez, ezxh, exzh, hhy, bxh, xhP, bz, zP, qxh, qzh);
+                    const float ezxh = bcsel(xhn, rdlane(u.ez, 0), u.hez);
+                    const float exzh = ztm == 0 ? u.hexn : bcsel(ztm, 0.0f, hex);
+                    hyh = h_cell2(Mh, idx, idz, hex, u.hez, ezxh, exzh, u.hhy, bxh, xhP, bz, zP, qxh, qzh);
                 }
                 // ---- new E of the row (src/models.jl:590-725)
                 if (zm_own) hym = hn;
@@ -255,7 +271,7 @@ __global__ __launch_bounds__(64) void k_step2d_f(const DevModel *__restrict__ mo
                 const int g = gid & ID_MASK;
                 const float4 cf = MODE != MODE_VAC ? ctab[g] : ctab[0];
                 const float Md1 = cf.x, Md2 = d ? cf.y : (MODE == MODE_FULL ? cw[g] : cf.w);
-                const float exo = ex, ezo = ez;
+                const float exo = ex, ezo = u.ez;
                 float d0 = d ? d0s : exo;
                 float d1 = d ? d1s : ezo;
                 e_cell2(Md1, Md2, idx, idz, hn, hyx, hym, d0, d1, bx, xP, bz, zP, qex, qez);
@@ -287,16 +303,25 @@ __global__ __launch_bounds__(64) void k_step2d_f(const DevModel *__restrict__ mo
                     stb<float>(Ezw, o, e1);
                     if (!defer && !isfinite(e0)) nonfin |= isnan(e0) ? 1 : 2;
                     if (INTEG && !defer && !(gid & ID_TOUCHED)) {   // src/outputs.jl:322-325
-                        if (doSa) stnt(Sa, o, sa_step2(sa, e0, e1, hn, dtf));
-                        if (doE2) stnt(E2, o, e2_step2(e2, e0, e1, dtf));
+                        if (doSa) stnt(Sa, o, sa_step2(u.sa, e0, e1, hn, dtf));
+                        if (doE2) stnt(E2, o, e2_step2(u.e2, e0, e1, dtf));
                     }
                 }
                 zm_own = false;
             }
             hym = hn;
-            ex = exn;
-            gid = gidn;
-            hex = hexn;
+            ex = u.exn;
+            gid = u.gidn;
+            hex = u.hexn;
+        };
+
+        // planes in pairs, the two U groups alternating between A and B
+        U2 A, B;
+        loadU(pb, A);
+        for (int p = pb; p < z1; p += 2) {
+            plane(p, A, B);
+            if (p + 1 >= z1) break;
+            plane(p + 1, B, A);
         }
         if (__any(nonfin != 0) && nonfin) atomicOr(m.flag, nonfin);
         // the LDS tables are rewritten by the next tile
