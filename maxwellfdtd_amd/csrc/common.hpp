@@ -222,6 +222,8 @@ struct DevModel {
     // udiv_magic(n, divx) and udiv_magic(n, divxy)
     uint4 divx, divxy;
     int dless;                  // plain cells keep no D: D = E/Me is rebuilt on demand
+    int ntst;                   // fused 3D step: new E and H stored non-temporally (a lone
+                                // model whose two field sets exceed the Infinity Cache)
     // FP64 state (1D models with MFD_OPT_F64): the fields are F64[] (F[] is null), the
     // material table is mats64 (mats keeps the FP32 table for the id lookups of the
     // shared kernels), and every CPML layer array (K, A, B, psi) holds doubles

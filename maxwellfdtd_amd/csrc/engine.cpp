@@ -916,6 +916,13 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
             if (st != MFD_OK) return st;
         }
         hd.push_back(m->dm);
+        // the fused step's new fields are read by the next step only.  For a lone model
+        // whose two field sets (48 B per cell) outgrow the 256 MB Infinity Cache, storing
+        // them past the caches leaves L2 to the old buffers' rows that neighbour tiles
+        // re-read as halo: 256^3 331 -> 320 us, 192^3 156 -> 148 us, the dispersive
+        // 384x384x192 step 537 -> 518 us.  At 128^3 (100 MB) it gained nothing, and in
+        // the served batches (many models per launch) it was 3 % slower
+        hd.back().ntst = fused && ms.size() == 1 && (double)m->d.Nx * m->d.Ny * m->d.Nz * 48.0 > 256.0 * (1 << 20);
         make_tiles(*m, (int)k, fused ? fzc : opt.zchunk, fused ? frows : fused2 ? 1 : opt.ty, kform, tiles);
         if (ndims == 3 && kform == 2 && opt.zchunk_h > 0) make_tiles(*m, (int)k, opt.zchunk_h, opt.ty, kform, tiles_h);
     }

@@ -337,6 +337,7 @@ __global__ __launch_bounds__(64) void k_step3d_f(const DevModel *__restrict__ mo
         // x-face TFSF columns of this lane; xhs: their H corrections stored here
         const bool xcol = xtile && xf_l;
         const bool xhs = m.xface_h != 0;
+        const bool nts = m.ntst != 0;
         const int xside = ix == xb[1] ? 1 : 0;
 
         for (int p = pb; p < z1; p++) {
@@ -603,13 +604,21 @@ __global__ __launch_bounds__(64) void k_step3d_f(const DevModel *__restrict__ mo
                         }
                     }
                     if (act) {
-                        stb<float>(Hxw, o, Hn[r].x);
-                        stb<float>(Hyw, o, xhs ? hyI : Hn[r].y);
-                        stb<float>(Hzw, o, xhs ? hzI : Hn[r].z);
                         if (dcr) { stb<float>(Dx, o, d0); stb<float>(Dy, o, d1); stb<float>(Dz, o, d2); }
                         // a cell whose E the state pass finishes keeps E^n, which that pass reads
                         if (defer) { e[0] = ex[r]; e[1] = ey[r]; e[2] = ez[r]; }
-                        stb<float>(Exw, o, e[0]); stb<float>(Eyw, o, e[1]); stb<float>(Ezw, o, e[2]);
+                        const float h1 = xhs ? hyI : Hn[r].y, h2 = xhs ? hzI : Hn[r].z;
+                        if (nts) {   // the new fields are read by the next step only
+                            // the empty asm statements keep the two branches' stores apart:
+                            // hoisted or sunk into one, they would lose the non-temporal hint
+                            asm volatile("" ::: "memory");
+                            stnt(Hxw, o, Hn[r].x); stnt(Hyw, o, h1); stnt(Hzw, o, h2);
+                            stnt(Exw, o, e[0]); stnt(Eyw, o, e[1]); stnt(Ezw, o, e[2]);
+                            asm volatile("" ::: "memory");
+                        } else {
+                            stb<float>(Hxw, o, Hn[r].x); stb<float>(Hyw, o, h1); stb<float>(Hzw, o, h2);
+                            stb<float>(Exw, o, e[0]); stb<float>(Eyw, o, e[1]); stb<float>(Ezw, o, e[2]);
+                        }
                         if (!defer && !isfinite(e[0])) nonfin |= isnan(e[0]) ? 1 : 2;
                         if (INTEG && !defer && (!(gg & ID_TOUCHED) || xh2)) {   // src/outputs.jl:480-483
                             if (doSa) stnt(Sa, o, sa_step3(sa[r], e[0], e[1], e[2], Hn[r].x, hyI, hzI, dtf));
