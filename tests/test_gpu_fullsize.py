@@ -60,3 +60,28 @@ def test_dispersive_384x384x192_prefix_with_dft_plane():
     spec = spectra_errors(hg, ho, desc)
     assert np.max(np.abs(ho.monitor(0, "Ex"))) > 0
     assert spec["dft0_rel_l2"] <= TOL and spec["dft0_max_rel"] <= DFT_MAX_REL, spec
+
+
+def test_plane_wave_spheres_256_alone_equals_batched():
+    """A lone model whose two field sets outgrow the Infinity Cache gets its new fields
+    stored non-temporally; the same model batched with another keeps ordinary stores.
+    The cache policy changes no bit of the fields or the integrals."""
+    from maxwellfdtd_amd.batch import Batch
+
+    steps = 24
+    big = plane_wave_spheres_3d(n=256, steps=steps).describe()
+    small = plane_wave_spheres_3d(n=64, steps=steps, nspheres=4).describe()
+    print(f"256^3 alone and batched with 64^3, {steps} steps", flush=True)
+    ha = Handle(big, _lib.engine())
+    ha.step(steps)
+    b = Batch([big, small], _lib.engine())
+    b.run(0)
+    hb = b.handles[0]
+    for c in ("Hx", "Hy", "Hz", "Dx", "Dy", "Dz", "Ex", "Ey", "Ez"):
+        x, y = ha.field(c), hb.field(c)
+        assert np.array_equal(x, y), (c, float(np.max(np.abs(x - y))))
+    for k in (0, 1):
+        assert np.array_equal(ha.output(k), hb.output(k)), k
+    assert np.max(np.abs(ha.field("Ex"))) > 0
+    ha.close()
+    b.close()
