@@ -243,9 +243,13 @@ enum {
                                   the face launch after it walks only the y/z-face cells;
                                   0 leaves them to the face launch; results are bitwise
                                   equal                                                  */
-    MFD_OPT_FUSED_2D = 19      /* 2D models: 1 runs the fused step (one z march per step
+    MFD_OPT_FUSED_2D = 19,     /* 2D models: 1 runs the fused step (one z march per step
                                   over two Hy/Ex/Ez buffer sets), 0 (default) the two-pass
                                   kernels; results agree to rounding                    */
+    MFD_OPT_RUN1D = 20         /* 1D models (up to 65536 cells, no TFSF source): 1
+                                  (default) runs each mfd_model_step / mfd_batch_run call
+                                  in one launch, one workgroup per model; 0 launches the
+                                  pieces of every step separately; results are equal   */
 };
 /* Changing an option rebuilds the launch plan; field state is kept.                   */
 int32_t mfd_model_set_options(mfd_model *m, int32_t opt, int32_t value);

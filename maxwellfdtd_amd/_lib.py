@@ -139,6 +139,7 @@ MFD_OPT_DFT_BLOCK, MFD_OPT_ROWS, MFD_OPT_KERNEL_FORM, MFD_OPT_STAGGER, MFD_OPT_Q
 MFD_OPT_FILL_MCELLS = 9
 MFD_OPT_SNAKE, MFD_OPT_DLESS, MFD_OPT_PML_SPLIT, MFD_OPT_SCHED, MFD_OPT_ZCHUNK_H = 10, 11, 12, 13, 14
 MFD_OPT_XCD_H, MFD_OPT_XCD_BLOCK, MFD_OPT_F64, MFD_OPT_TFSF_FOLD, MFD_OPT_FUSED_2D = 15, 16, 17, 18, 19
+MFD_OPT_RUN1D = 20
 MFD_REQ_QUEUED, MFD_REQ_RUNNING, MFD_REQ_DONE, MFD_REQ_FAILED = 0, 1, 2, 3
 
 ENGINE_EXTRA = {

@@ -269,10 +269,10 @@ __device__ __forceinline__ const float *fld_ptr(const DevModel &m, int c, int pp
     return pp && (c < DX || c > DZ) ? m.Falt[c] : m.F[c];
 }
 __device__ __forceinline__ float fld_get(const DevModel &m, int c, long long i, int pp = 0) {
-    return m.f64 ? (float)m.F64[c][i] : fld_ptr(m, c, pp)[i];
+    return m.f64 ? (float)G(m.F64[c])[i] : G(fld_ptr(m, c, pp))[i];
 }
 __device__ __forceinline__ double fld_get64(const DevModel &m, int c, long long i, int pp = 0) {
-    return m.f64 ? m.F64[c][i] : (double)fld_ptr(m, c, pp)[i];
+    return m.f64 ? G(m.F64[c])[i] : (double)G(fld_ptr(m, c, pp))[i];
 }
 
 constexpr unsigned char ID_MASK = 0x7f, ID_TOUCHED = 0x80;
@@ -315,6 +315,10 @@ void launch_h1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int 
                 hipStream_t s);
 void launch_e1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int mode, bool integ,
                 bool f64, hipStream_t s);
+// whole runs of 1D models without TFSF sources: one workgroup per model, nsteps steps
+// (H, E, point sources, monitors and DFT, touched-cell integrals, the step counter)
+void launch_run1d(const DevModel *m, int nmodels, int nsteps, int mode, bool integ, bool f64, bool hmat,
+                  hipStream_t s);
 void launch_integrals_touched(const DevModel *m, int nmodels, int maxn, int ndims, hipStream_t s);
 void launch_point_sources(const DevModel *m, int nmodels, int slot, int maxn, hipStream_t s);
 void launch_tfsf(const DevModel *m, int nmodels, int slot, int ndims, int maxface, hipStream_t s,

@@ -994,6 +994,13 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
     for (Model *m : ms)
         for (size_t s2 = 0; s2 < m->srcs.size(); s2++)
             src_kind_mask[s2] |= m->srcs[s2].type == 0 ? 1 : 2;
+    // 1D models of up to 64 Ki cells without TFSF sources: one workgroup per model runs
+    // whole step ranges (the TFSF path keeps its per-step launches)
+    run1d = ndims == 1 && opt.run1d;
+    for (Model *m : ms) {
+        run1d &= m->d.Nz <= (1 << 16);
+        for (const auto &sv : m->srcs) run1d &= sv.type == 0;
+    }
     int_grid = (int)std::min<long long>((maxN + 255) / 256, (long long)cu_count * 16);
     MFD_HIP(hipMalloc(&d_models, hd.size() * sizeof(DevModel)));
     MFD_HIP(hipMalloc(&d_flags, hd.size() * sizeof(int)));
@@ -1316,6 +1323,14 @@ int Runner::run(int nsteps) {
     for (Model *m : dless_models) m->d_stale = true;
     int done = 0;
     const bool timing = opt.kernel_timing;
+    if (run1d && !timing && nsteps > 0) {
+        const int mode = any_full ? MODE_FULL : any_mats ? MODE_SIMPLE : MODE_VAC;
+        launch_run1d(d_models, (int)models.size(), nsteps, mode, any_int, f64, h_mats, stream);
+        MFD_HIP(hipGetLastError());
+        for (int k = 0; k < nsteps; k++)
+            for (size_t q = 0; q < models.size(); q++) advance(q);
+        return check_flags();
+    }
     const int G = graph_len();
     while (done < nsteps) {
         if (G > 0 && !timing && nsteps - done >= G) {
@@ -1425,6 +1440,7 @@ int set_option(RunOptions &o, Model *m, int opt, int value) {
             break;
         case MFD_OPT_TFSF_FOLD: o.tfsf_fold = value != 0; break;
         case MFD_OPT_FUSED_2D: o.fused_2d = value != 0; break;
+        case MFD_OPT_RUN1D: o.run1d = value != 0; break;
         case MFD_OPT_STAGGER:
             if (!m || m->uploaded || value < 0) { set_error("stagger must be >= 0, set on a model before upload"); return MFD_ERR_ARG; }
             m->stagger = value;

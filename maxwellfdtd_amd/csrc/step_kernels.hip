@@ -6,6 +6,7 @@
 #include "common.hpp"
 #include "material.hpp"
 #include "tfsf.hpp"
+#include "step_dev.hpp"
 
 #include <algorithm>
 
@@ -23,22 +24,7 @@ __global__ __launch_bounds__(256) void k_point_sources(const DevModel *__restric
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     bool wroteEx = false;
     float v = 0.0f;
-    if (k < S.n) {
-        const int c = S.cells[k];
-        const float w = S.wave[(size_t)it0 * S.n + k];
-        if (m.f64) {            // the state is FP64: the waveform and the addition too
-            double *F = m.F64[S.comp];
-            const double w64 = S.wave64[(size_t)it0 * S.n + k];
-            const double v64 = (S.kind == SRC_SOFT) ? F[c] + S.coeff64 * w64 : w64;
-            F[c] = v64;
-            v = (float)v64;
-        } else {
-            float *F = m.F[S.comp];
-            v = (S.kind == SRC_SOFT) ? F[c] + S.coeff * w : w;
-            F[c] = v;
-        }
-        wroteEx = S.comp == EX;
-    }
+    if (k < S.n) v = point_source_entry(m, S, it0, k, wroteEx);
     check_finite(v, m.flag, wroteEx);
 }
 
@@ -378,47 +364,13 @@ void launch_tfsf(const DevModel *m, int nmodels, int slot, int ndims, int maxfac
 }
 
 // ---- monitors: update_monitor! (src/monitors.jl:62-66,110-116,164-176,350-359) --------
-__device__ double block_sum(double v, double *sh) {
-    const int t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-        if (t < o) sh[t] += sh[t + o];
-        __syncthreads();
-    }
-    const double r = sh[0];
-    __syncthreads();
-    return r;
-}
-
 __global__ __launch_bounds__(256) void k_monitors(const DevModel *__restrict__ models, int pp) {
     __shared__ double sh[256];
     const DevModel &m = models[blockIdx.y];
     const int it0 = *m.it;
     if (it0 >= m.Nt) return;
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int stride = gridDim.x * blockDim.x;
-    for (int q = 0; q < m.nmon; q++) {
-        const DevMon &M = m.mon[q];
-        if (!M.spectral && M.issum) {
-            if (blockIdx.x != 0) continue;
-            for (int c = 0; c < M.ncomp; c++) {
-                double acc = 0.0;
-                for (int k = threadIdx.x; k < M.n; k += blockDim.x) acc += fld_get64(m, M.comps[c], M.cells[k], pp);
-                const double tot = block_sum(acc, sh);
-                if (threadIdx.x == 0) M.sum[(size_t)it0 * M.ncomp + c] = tot;
-            }
-        } else if (!M.spectral) {
-            for (int k = gid; k < M.n; k += stride)
-                for (int c = 0; c < M.ncomp; c++)
-                    M.rec[((size_t)it0 * M.ncomp + c) * M.n + k] = fld_get(m, M.comps[c], M.cells[k], pp);
-        } else {
-            const int kk = it0 % m.dft_block;
-            for (int k = gid; k < M.n; k += stride)
-                for (int c = 0; c < M.ncomp; c++)
-                    M.samp[((size_t)kk * M.ncomp + c) * M.n + k] = fld_get(m, M.comps[c], M.cells[k], pp);
-        }
-    }
+    monitors_model(m, it0, pp, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, blockIdx.x == 0,
+                   sh);
 }
 
 void launch_monitors(const DevModel *m, int nmodels, int maxn, hipStream_t s, bool pp) {
@@ -438,9 +390,6 @@ void launch_monitors(const DevModel *m, int nmodels, int maxn, hipStream_t s, bo
 //  - at read-back (force = 1): every model's pending partial block (it - dftdone steps),
 //    after which k_dft_commit moves dftdone to it.
 // A model that has finished (it >= Nt) is never contracted in the step.
-constexpr int WCH = 16;     // frequencies per pass: 32 FP64 accumulators per lane
-constexpr int KMAX = 64;
-
 __global__ __launch_bounds__(256) void k_dft_flush(const DevModel *__restrict__ models, int force) {
     __shared__ double2 ph[KMAX * WCH];
     __shared__ double sh[256];
@@ -456,73 +405,8 @@ __global__ __launch_bounds__(256) void k_dft_flush(const DevModel *__restrict__ 
         itn = it0 + 1;
         if (itn - base != m.dft_block) return;
     }
-    const int nk = itn - base;
-    // the sample ring holds dft_block <= KMAX steps
-    if (base < 0 || nk <= 0 || nk > m.dft_block || nk > KMAX) return;
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int stride = gridDim.x * blockDim.x;
-    const double dt = m.dt;
-    for (int q = 0; q < m.nmon; q++) {
-        const DevMon &M = m.mon[q];
-        if (!M.spectral) continue;
-        for (int w0 = 0; w0 < M.nw; w0 += WCH) {
-            const int nwc = min(WCH, M.nw - w0);
-            __syncthreads();
-            for (int e = threadIdx.x; e < nk * nwc; e += blockDim.x) {
-                const int k = e / nwc, w = e % nwc;
-                const double t = (double)(base + k) * dt;
-                const double th = M.w[w0 + w] * (t - M.tshift);
-                double sn, cs;
-                sincos(th, &sn, &cs);
-                ph[k * WCH + w] = make_double2(2.0 * cs * dt, -2.0 * sn * dt);
-            }
-            __syncthreads();
-            if (!M.issum) {
-                for (int j = gid; j < M.n; j += stride) {
-                    for (int c = 0; c < M.ncomp; c++) {
-                        double re[WCH], im[WCH];
-                        for (int w = 0; w < WCH; w++) { re[w] = 0.0; im[w] = 0.0; }
-                        for (int k = 0; k < nk; k++) {
-                            const int kk = (base + k) % m.dft_block;
-                            const double F = (double)M.samp[((size_t)kk * M.ncomp + c) * M.n + j];
-#pragma unroll
-                            for (int w = 0; w < WCH; w++) {
-                                const double2 p = ph[k * WCH + w];
-                                re[w] += F * p.x;
-                                im[w] += F * p.y;
-                            }
-                        }
-                        for (int w = 0; w < nwc; w++) {
-                            double2 *Sp = M.S + ((size_t)c * M.nw + (w0 + w)) * M.n + j;
-                            double2 sv = *Sp;
-                            sv.x += re[w];
-                            sv.y += im[w];
-                            *Sp = sv;
-                        }
-                    }
-                }
-            } else if (blockIdx.x == 0) {
-                for (int c = 0; c < M.ncomp; c++) {
-                    for (int k = 0; k < nk; k++) {
-                        const int kk = (base + k) % m.dft_block;
-                        double acc = 0.0;
-                        for (int j = threadIdx.x; j < M.n; j += blockDim.x)
-                            acc += (double)M.samp[((size_t)kk * M.ncomp + c) * M.n + j];
-                        const double Fs = block_sum(acc, sh);
-                        if ((int)threadIdx.x < nwc) {
-                            const double2 p = ph[k * WCH + threadIdx.x];
-                            double2 *Sp = M.S + (size_t)c * M.nw + w0 + threadIdx.x;
-                            double2 sv = *Sp;
-                            sv.x += Fs * p.x;
-                            sv.y += Fs * p.y;
-                            *Sp = sv;
-                        }
-                        __syncthreads();
-                    }
-                }
-            }
-        }
-    }
+    dft_flush_model(m, base, itn, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, blockIdx.x == 0,
+                    ph, sh);
 }
 
 void launch_dft_flush(const DevModel *m, int nmodels, int maxn, bool force, hipStream_t s) {
@@ -549,41 +433,7 @@ __global__ __launch_bounds__(256) void k_integrals_touched(const DevModel *__res
                                                            int ndims) {
     const DevModel &m = models[blockIdx.y];
     if (*m.it >= m.Nt || m.int_flags == 0) return;
-    const float dt = m.dtf;
-    float *Sa = m.Sa, *E2 = m.E2;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m.ntouched; k += gridDim.x * blockDim.x) {
-        const long long i = m.touched[k] & TOUCHED_CELL;
-        if (ndims == 3) {
-            const float hx = m.F[HX][i], hy = m.F[HY][i], hz = m.F[HZ][i];
-            const float ex = m.F[EX][i], ey = m.F[EY][i], ez = m.F[EZ][i];
-            if (Sa) {
-                const float a = ey * hz - ez * hy, b = ez * hx - ex * hz, c = ex * hy - ey * hx;
-                Sa[i] += sa_sqrt(a * a + b * b + c * c) * dt;
-            }
-            if (E2) E2[i] += (ex * ex + ey * ey + ez * ez) * dt;
-        } else if (ndims == 2) {
-            const float hy = m.F[HY][i], ex = m.F[EX][i], ez = m.F[EZ][i];
-            if (Sa) {
-                const float a = -ez * hy, b = ex * hy;
-                Sa[i] += sa_sqrt(a * a + b * b) * dt;
-            }
-            if (E2) E2[i] += (ex * ex + ez * ez) * dt;
-        } else if (m.f64) {
-            const double hy = m.F64[HY][i], ex = m.F64[EX][i];
-            if (m.Sa64) {
-                const double a = ex * hy;
-                m.Sa64[i] += sqrt(a * a) * m.dt;
-            }
-            if (m.E2_64) m.E2_64[i] += ex * ex * m.dt;
-        } else {
-            const float hy = m.F[HY][i], ex = m.F[EX][i];
-            if (Sa) {
-                const float a = ex * hy;
-                Sa[i] += sa_sqrt(a * a) * dt;
-            }
-            if (E2) E2[i] += ex * ex * dt;
-        }
-    }
+    integrals_touched_model(m, ndims, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 void launch_integrals_touched(const DevModel *m, int nmodels, int maxn, int ndims, hipStream_t s) {
