@@ -248,8 +248,10 @@ enum {
                                   kernels; results agree to rounding                    */
     MFD_OPT_RUN1D = 20         /* 1D models (up to 65536 cells, no TFSF source): 1
                                   (default) runs each mfd_model_step / mfd_batch_run call
-                                  in one launch, one workgroup per model; 0 launches the
-                                  pieces of every step separately; results are equal   */
+                                  in one launch, one workgroup per model, with the cells
+                                  in registers where the model allows it; 2 the same
+                                  without the register form; 0 launches the pieces of
+                                  every step separately; results are bitwise equal     */
 };
 /* Changing an option rebuilds the launch plan; field state is kept.                   */
 int32_t mfd_model_set_options(mfd_model *m, int32_t opt, int32_t value);

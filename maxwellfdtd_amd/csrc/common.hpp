@@ -319,6 +319,9 @@ void launch_e1d(const DevModel *m, const Tile *tiles, int ntiles, int grid, int 
 // (H, E, point sources, monitors and DFT, touched-cell integrals, the step counter)
 void launch_run1d(const DevModel *m, int nmodels, int nsteps, int mode, bool integ, bool f64, bool hmat,
                   hipStream_t s);
+// the same with each model's cells in registers (models the host found small enough)
+void launch_reg1d(const DevModel *m, int nmodels, int nsteps, int mode, bool integ, bool f64, bool hmat,
+                  hipStream_t s);
 void launch_integrals_touched(const DevModel *m, int nmodels, int maxn, int ndims, hipStream_t s);
 void launch_point_sources(const DevModel *m, int nmodels, int slot, int maxn, hipStream_t s);
 void launch_tfsf(const DevModel *m, int nmodels, int slot, int ndims, int maxface, hipStream_t s,

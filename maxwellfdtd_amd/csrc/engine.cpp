@@ -996,10 +996,24 @@ int Runner::init(const std::vector<Model *> &ms, const RunOptions &o, hipStream_
             src_kind_mask[s2] |= m->srcs[s2].type == 0 ? 1 : 2;
     // 1D models of up to 64 Ki cells without TFSF sources: one workgroup per model runs
     // whole step ranges (the TFSF path keeps its per-step launches)
-    run1d = ndims == 1 && opt.run1d;
+    run1d = ndims == 1 && opt.run1d != 0;
     for (Model *m : ms) {
         run1d &= m->d.Nz <= (1 << 16);
         for (const auto &sv : m->srcs) run1d &= sv.type == 0;
+    }
+    // the register form: up to 4096 cells, no Kerr / ADE / plasma material, fewer than 64
+    // materials, CPML layers of up to 64 cells inside the first and the last block of 512
+    // cells, at most 8 point-source entries
+    reg1d = run1d && opt.run1d == 1 && !any_full;
+    for (size_t k = 0; k < ms.size() && reg1d; k++) {
+        const DevModel &dm = hd[k];
+        const int Nz = dm.Nz, jl = (Nz - 1) / 512;
+        reg1d &= Nz <= 4096 && dm.nmat < 64;
+        if (dm.L[4].present) reg1d &= dm.L[4].ib < 64;
+        if (dm.L[5].present) reg1d &= Nz - dm.L[5].ib <= 64 && dm.L[5].ib / 512 == jl && dm.L[5].ib > 0;
+        int nent = 0;
+        for (const DevSource &v : ms[k]->hsrc) nent += v.n;
+        reg1d &= nent <= 8;
     }
     int_grid = (int)std::min<long long>((maxN + 255) / 256, (long long)cu_count * 16);
     MFD_HIP(hipMalloc(&d_models, hd.size() * sizeof(DevModel)));
@@ -1325,7 +1339,8 @@ int Runner::run(int nsteps) {
     const bool timing = opt.kernel_timing;
     if (run1d && !timing && nsteps > 0) {
         const int mode = any_full ? MODE_FULL : any_mats ? MODE_SIMPLE : MODE_VAC;
-        launch_run1d(d_models, (int)models.size(), nsteps, mode, any_int, f64, h_mats, stream);
+        if (reg1d) launch_reg1d(d_models, (int)models.size(), nsteps, mode, any_int, f64, h_mats, stream);
+        else launch_run1d(d_models, (int)models.size(), nsteps, mode, any_int, f64, h_mats, stream);
         MFD_HIP(hipGetLastError());
         for (int k = 0; k < nsteps; k++)
             for (size_t q = 0; q < models.size(); q++) advance(q);
@@ -1440,7 +1455,10 @@ int set_option(RunOptions &o, Model *m, int opt, int value) {
             break;
         case MFD_OPT_TFSF_FOLD: o.tfsf_fold = value != 0; break;
         case MFD_OPT_FUSED_2D: o.fused_2d = value != 0; break;
-        case MFD_OPT_RUN1D: o.run1d = value != 0; break;
+        case MFD_OPT_RUN1D:
+            if (value < 0 || value > 2) { set_error("run1d must be 0, 1 or 2"); return MFD_ERR_ARG; }
+            o.run1d = value;
+            break;
         case MFD_OPT_STAGGER:
             if (!m || m->uploaded || value < 0) { set_error("stagger must be >= 0, set on a model before upload"); return MFD_ERR_ARG; }
             m->stagger = value;

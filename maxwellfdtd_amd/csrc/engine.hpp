@@ -156,8 +156,10 @@ struct RunOptions {
     int xcd_block = 64;         // 3D: tiles per block dealt to the XCDs (0: one chunk per XCD)
     bool tfsf_fold = true;      // fused 3D step: x-face H corrections inside the launch
     bool fused_2d = false;      // 2D models: the fused step instead of the two-pass kernels
-    bool run1d = true;          // 1D models without TFSF sources: whole step ranges in one
-                                // launch, one workgroup per model (k_run1d)
+    int run1d = 1;              // 1D models without TFSF sources: whole step ranges in one
+                                // launch, one workgroup per model; 1: the cells in registers
+                                // where the model allows it (k_reg1d), else k_run1d; 2:
+                                // k_run1d; 0: launches per step
     bool shape_set = false;     // ZCHUNK or ROWS set by the caller (else the fused step
                                 // picks its tile shape from the tile count)
 };
@@ -191,6 +193,7 @@ struct Runner {
     std::vector<int> pp_seen;   // per model: the buffer parity d_models holds
     bool f64 = false;           // the models keep their state in FP64 (1D)
     bool run1d = false;         // run() steps the 1D models with k_run1d
+    bool reg1d = false;         // ... with k_reg1d
     bool split_state = false;   // stateful cells' E computed by the state pass
     bool tfsf_gather = false;   // TFSF corrections + touched integrals in one launch
     bool xface_h = false;       // the fused step stores the x faces' H corrections too;

@@ -81,6 +81,9 @@ __device__ inline void monitors_model(const DevModel &m, int it0, int pp, int gi
 constexpr int WCH = 16;     // frequencies per pass: 32 FP64 accumulators per lane
 constexpr int KMAX = 64;
 
+// WC: frequencies per pass (the per-frequency sums do not depend on it; fewer passes
+// need more accumulator registers)
+template <int WC = WCH>
 __device__ inline void dft_flush_model(const DevModel &m, int base, int itn, int gid, int stride, bool lead,
                                        double2 *ph, double *sh) {
     const int nk = itn - base;
@@ -90,8 +93,8 @@ __device__ inline void dft_flush_model(const DevModel &m, int base, int itn, int
     for (int q = 0; q < m.nmon; q++) {
         const __attribute__((address_space(1))) DevMon &M = G(m.mon)[q];
         if (!M.spectral) continue;
-        for (int w0 = 0; w0 < M.nw; w0 += WCH) {
-            const int nwc = min(WCH, M.nw - w0);
+        for (int w0 = 0; w0 < M.nw; w0 += WC) {
+            const int nwc = min(WC, M.nw - w0);
             __syncthreads();
             for (int e = threadIdx.x; e < nk * nwc; e += blockDim.x) {
                 const int k = e / nwc, w = e % nwc;
@@ -99,26 +102,26 @@ __device__ inline void dft_flush_model(const DevModel &m, int base, int itn, int
                 const double th = G(M.w)[w0 + w] * (t - M.tshift);
                 double sn, cs;
                 sincos(th, &sn, &cs);
-                ph[k * WCH + w] = make_double2(2.0 * cs * dt, -2.0 * sn * dt);
+                ph[k * WC + w] = make_double2(2.0 * cs * dt, -2.0 * sn * dt);
             }
             __syncthreads();
             if (!M.issum) {
                 for (int j = gid; j < M.n; j += stride) {
                     for (int c = 0; c < M.ncomp; c++) {
-                        double re[WCH], im[WCH];
-                        for (int w = 0; w < WCH; w++) { re[w] = 0.0; im[w] = 0.0; }
+                        double re[WC], im[WC];
+                        for (int w = 0; w < WC; w++) { re[w] = 0.0; im[w] = 0.0; }
                         for (int k = 0; k < nk; k++) {
                             const int kk = (base + k) % m.dft_block;
                             const double F = (double)G(M.samp)[((size_t)kk * M.ncomp + c) * M.n + j];
 #pragma unroll
-                            for (int w = 0; w < WCH; w++) {
-                                const double2 p = ph[k * WCH + w];
+                            for (int w = 0; w < WC; w++) {
+                                const double2 p = ph[k * WC + w];
                                 re[w] += F * p.x;
                                 im[w] += F * p.y;
                             }
                         }
                         for (int w = 0; w < nwc; w++) {
-                            const gp<double2> Sp = G(M.S) + ((size_t)c * M.nw + (w0 + w)) * M.n + j;
+                            double2 *Sp = M.S + ((size_t)c * M.nw + (w0 + w)) * M.n + j;
                             double2 sv = *Sp;
                             sv.x += re[w];
                             sv.y += im[w];
@@ -135,8 +138,8 @@ __device__ inline void dft_flush_model(const DevModel &m, int base, int itn, int
                             acc += (double)G(M.samp)[((size_t)kk * M.ncomp + c) * M.n + j];
                         const double Fs = block_sum(acc, sh);
                         if ((int)threadIdx.x < nwc) {
-                            const double2 p = ph[k * WCH + threadIdx.x];
-                            const gp<double2> Sp = G(M.S) + (size_t)c * M.nw + w0 + threadIdx.x;
+                            const double2 p = ph[k * WC + threadIdx.x];
+                            double2 *Sp = M.S + (size_t)c * M.nw + w0 + threadIdx.x;
                             double2 sv = *Sp;
                             sv.x += Fs * p.x;
                             sv.y += Fs * p.y;
@@ -146,6 +149,46 @@ __device__ inline void dft_flush_model(const DevModel &m, int base, int itn, int
                     }
                 }
             }
+        }
+    }
+}
+
+// The same block for point (not summed) spectral monitors with nk * nw <= KMAX * WCH,
+// one thread per (point, component, frequency) and the phase table of every (step,
+// frequency) in one pass: the sums, term for term, of dft_flush_model.
+__device__ inline void dft_flush_points(const DevModel &m, int base, int itn, double2 *ph) {
+    const int nk = itn - base;
+    if (base < 0 || nk <= 0 || nk > m.dft_block || nk > KMAX) return;
+    const double dt = m.dt;
+    for (int q = 0; q < m.nmon; q++) {
+        const __attribute__((address_space(1))) DevMon &M = G(m.mon)[q];
+        if (!M.spectral) continue;
+        const int nw = M.nw;
+        __syncthreads();
+        for (int e = threadIdx.x; e < nk * nw; e += blockDim.x) {
+            const int k = e / nw, w = e % nw;
+            const double t = (double)(base + k) * dt;
+            const double th = G(M.w)[w] * (t - M.tshift);
+            double sn, cs;
+            sincos(th, &sn, &cs);
+            ph[k * nw + w] = make_double2(2.0 * cs * dt, -2.0 * sn * dt);
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < M.n * M.ncomp * nw; e += blockDim.x) {
+            const int w = e % nw, c = (e / nw) % M.ncomp, j = e / (nw * M.ncomp);
+            double re = 0.0, im = 0.0;
+            for (int k = 0; k < nk; k++) {
+                const int kk = (base + k) % m.dft_block;
+                const double F = (double)G(M.samp)[((size_t)kk * M.ncomp + c) * M.n + j];
+                const double2 p = ph[k * nw + w];
+                re += F * p.x;
+                im += F * p.y;
+            }
+            double2 *Sp = M.S + ((size_t)c * nw + w) * M.n + j;
+            double2 sv = *Sp;
+            sv.x += re;
+            sv.y += im;
+            *Sp = sv;
         }
     }
 }

@@ -21,6 +21,11 @@ CHUNKS = (37, 1, 90, 200)    # 328 steps, then the rest of Nt
 def _case(name):
     if name == "slab":
         return cases.slab_1d(steps=400)
+    if name == "plain":         # simple materials: the register form
+        return cases.slab_1d(steps=400, dispersive=False)
+    if name == "config":        # the 1D configuration at full length, register form
+        from maxwellfdtd_amd.workloads import slab_1d
+        return slab_1d()
     if name == "nonlinear":
         return cases.nonlinear_1d(steps=400)
     model, _, _ = cases.ref_1d_test()     # hard source, vacuum
@@ -51,13 +56,16 @@ def _outputs(desc, plasma):
     return [k for k in range(4) if (k < 2 and desc.integrals & (1 << k)) or (k >= 2 and plasma)]
 
 
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("f64", [0, 1])
-@pytest.mark.parametrize("name", ["slab", "nonlinear", "ref"])
-def test_run1d_equals_per_step_launches(name, f64):
+@pytest.mark.parametrize("name", ["plain", "config", "slab", "nonlinear", "ref"])
+def test_run1d_equals_per_step_launches(name, f64, mode):
+    """mode 1: the register form where the model allows it (plain, config), else the
+    L2 form; mode 2: the L2 form"""
     model, mons = _case(name)
     desc = model.describe(monitors=mons)
-    print(f"1D {name}, f64={f64}: Nz={desc.dims}, Nt={desc.Nt}", flush=True)
-    ha = _run(desc, 1, f64)
+    print(f"1D {name}, f64={f64}, run1d={mode}: Nz={desc.dims}, Nt={desc.Nt}", flush=True)
+    ha = _run(desc, mode, f64)
     hb = _run(desc, 0, f64)
     for c in ("Hy", "Dx", "Ex"):
         x, y = ha.field(c), hb.field(c)
@@ -72,7 +80,10 @@ def test_run1d_equals_per_step_launches(name, f64):
     ho = Handle(desc, oracle("f64"))
     ho.step(desc.Nt)
     errs = field_errors(ha, ho, 1)
-    assert max(errs.values()) <= (1e-9 if f64 else 1e-4), errs
+    # FP32 state over the 8000 steps of the configuration: 1.0e-4 to 1.2e-4, FP32's own
+    # drift once the pulse has left (the oracle's FP32 build gives the same)
+    tol = 1e-9 if f64 else (1.5e-4 if name == "config" else 1e-4)
+    assert max(errs.values()) <= tol, errs
     merrs = {}
     for q in range(len(desc.monitors)):
         for c in MONITOR_COMPONENTS[1]:
@@ -85,12 +96,13 @@ def test_run1d_equals_per_step_launches(name, f64):
         h.close()
 
 
-def test_run1d_batch_of_uneven_models():
-    """Several 1D models of different lengths and Nt in one batch: one workgroup each;
-    every model equals its own run."""
+@pytest.mark.parametrize("dispersive", [False, True])
+def test_run1d_batch_of_uneven_models(dispersive):
+    """Several 1D models of different lengths and Nt in one batch: one workgroup each
+    (the register form for the simple materials); every model equals its own run."""
     descs = []
     for nz, steps in ((300, 120), (600, 260), (450, 200)):
-        model, mons = cases.slab_1d(nz=nz, steps=steps)
+        model, mons = cases.slab_1d(nz=nz, steps=steps, dispersive=dispersive)
         descs.append(model.describe(monitors=mons))
     b = Batch(descs, _lib.engine())
     b.run(0)
